@@ -1,0 +1,1512 @@
+// Host side of libqsim_hip.so: gate handling, circuit optimizer, LDS pass
+// planner, compressed block store with its stage loop, and the C ABI
+// declared in include/qsim.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qsim.h"
+#include "qsim_internal.h"
+
+namespace qsim {
+
+
+hipError_t launch_pass(int in_mode, int out_mode, int tb, const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles,
+                       hipStream_t st);
+hipError_t launch_fill_basis(double2* v, uint64_t n, uint64_t idx, hipStream_t st);
+hipError_t launch_fill_u8(uint8_t* p, uint64_t n, uint8_t val, hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long val, hipStream_t st);
+hipError_t launch_scale(double2* v, uint64_t n, double s, hipStream_t st);
+hipError_t launch_commit_null(uint8_t* nf, uint8_t* nz, uint64_t nb, hipStream_t st);
+unsigned reduce_grid(uint64_t n);
+hipError_t launch_sv_norm(const double2* a, uint64_t n, double* partials, unsigned grid, hipStream_t st);
+hipError_t launch_sv_inner(const double2* a, const double2* b, uint64_t n, double* partials, unsigned grid,
+                           hipStream_t st);
+hipError_t launch_sv_pauli(const double2* a, uint64_t n, uint64_t f, uint64_t z, double* partials, unsigned grid,
+                           hipStream_t st);
+hipError_t launch_sv_apply_pauli(const double2* a, double2* out, uint64_t n, uint64_t f, uint64_t z, double cr,
+                                 double ci, int acc, hipStream_t st);
+hipError_t launch_block_bytes(const uint64_t* desc, const uint8_t* nullf, uint64_t nb, uint64_t cpb,
+                              unsigned long long* out, hipStream_t st);
+
+using cplx = std::complex<double>;
+static thread_local std::string g_err;
+
+#define HIPCHECK(x)                                                                                   \
+  do {                                                                                                \
+    hipError_t e_ = (x);                                                                              \
+    if (e_ != hipSuccess)                                                                             \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " in " + #x);      \
+  } while (0)
+
+// ------------------------------------------------------------------ context
+static hipStream_t stream() {
+  static hipStream_t st = [] {
+    hipStream_t s;
+    HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+  }();
+  return st;
+}
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t b) {
+    release();
+    if (b == 0) b = 16;
+    HIPCHECK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  void ensure(size_t b) {
+    if (b > bytes) alloc(b);
+  }
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+static void upload(DBuf& d, const void* h, size_t b) {
+  d.ensure(b);
+  if (b) HIPCHECK(hipMemcpyAsync(d.p, h, b, hipMemcpyHostToDevice, stream()));
+}
+
+// -------------------------------------------------------------------- gates
+struct HGate {
+  std::vector<uint32_t> t, tprop, c, cval, pauli;
+  std::vector<cplx> m;  // 2^k x 2^k row-major
+  uint32_t flags = 0;
+  int32_t param = -1;
+  bool parametric() const { return flags & QSIM_GATE_PARAMETRIC; }
+};
+
+static std::vector<HGate> parse_gates(const qsim_gate* g, uint64_t n, const double* pool, uint64_t pool_len) {
+  std::vector<HGate> out(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    const qsim_gate& q = g[i];
+    if (q.n_targets > QSIM_MAX_GATE_QUBITS || q.n_controls > QSIM_MAX_GATE_QUBITS)
+      throw std::invalid_argument("gate " + std::to_string(i) + ": too many qubits");
+    HGate& h = out[i];
+    h.t.assign(q.targets, q.targets + q.n_targets);
+    h.tprop.assign(q.target_commute, q.target_commute + q.n_targets);
+    h.pauli.assign(q.pauli_ids, q.pauli_ids + q.n_targets);
+    h.c.assign(q.controls, q.controls + q.n_controls);
+    h.cval.assign(q.control_values, q.control_values + q.n_controls);
+    h.flags = q.flags;
+    h.param = q.param_index;
+    const uint64_t dim = 1ull << q.n_targets;
+    if (q.matrix_offset + dim * dim > pool_len / 2)
+      throw std::invalid_argument("gate " + std::to_string(i) + ": matrix outside the pool");
+    h.m.resize(dim * dim);
+    for (uint64_t k = 0; k < dim * dim; ++k)
+      h.m[k] = cplx(pool[2 * (q.matrix_offset + k)], pool[2 * (q.matrix_offset + k) + 1]);
+    std::vector<uint32_t> all = h.t;
+    all.insert(all.end(), h.c.begin(), h.c.end());
+    std::sort(all.begin(), all.end());
+    if (std::adjacent_find(all.begin(), all.end()) != all.end())
+      throw std::invalid_argument("gate " + std::to_string(i) + ": repeated qubit");
+  }
+  return out;
+}
+
+static void check_qubits(const std::vector<HGate>& gs, uint32_t n) {
+  for (size_t i = 0; i < gs.size(); ++i) {
+    for (auto q : gs[i].t)
+      if (q >= n) throw std::invalid_argument("gate " + std::to_string(i) + ": target out of range");
+    for (auto q : gs[i].c)
+      if (q >= n) throw std::invalid_argument("gate " + std::to_string(i) + ": control out of range");
+  }
+}
+
+// ----------------------------------------------------------- the optimizer
+// Commutation of two gates by their qubit roles (src/cppsim/gate.cpp:12-31,
+// src/cppsim/qubit_info.cpp:4-31).
+static bool gates_commute(const HGate& a, const HGate& b) {
+  for (size_t i = 0; i < a.t.size(); ++i) {
+    for (size_t j = 0; j < b.t.size(); ++j)
+      if (a.t[i] == b.t[j] && (a.tprop[i] & b.tprop[j]) == 0) return false;
+    for (size_t j = 0; j < b.c.size(); ++j)
+      if (a.t[i] == b.c[j] && !(a.tprop[i] & QSIM_COMMUTE_Z)) return false;
+  }
+  for (size_t i = 0; i < a.c.size(); ++i)
+    for (size_t j = 0; j < b.t.size(); ++j)
+      if (a.c[i] == b.t[j] && !(b.tprop[j] & QSIM_COMMUTE_Z)) return false;
+  return true;
+}
+
+// Matrix of gate g on the ordered target list nt (index bit i <-> nt[i]);
+// g's controls that appear in nt act as conditions, the others are assumed
+// satisfied.
+static std::vector<cplx> extend_matrix(const HGate& g, const std::vector<uint32_t>& nt) {
+  const size_t K = nt.size(), D = 1ull << K;
+  auto pos = [&](uint32_t q) -> int {
+    for (size_t i = 0; i < K; ++i)
+      if (nt[i] == q) return (int)i;
+    return -1;
+  };
+  std::vector<int> tpos(g.t.size());
+  uint64_t gmask = 0;
+  for (size_t i = 0; i < g.t.size(); ++i) {
+    tpos[i] = pos(g.t[i]);
+    gmask |= 1ull << tpos[i];
+  }
+  uint64_t cmask = 0, cval = 0;
+  for (size_t i = 0; i < g.c.size(); ++i) {
+    int p = pos(g.c[i]);
+    if (p >= 0) {
+      cmask |= 1ull << p;
+      if (g.cval[i]) cval |= 1ull << p;
+    }
+  }
+  const size_t gd = 1ull << g.t.size();
+  std::vector<cplx> M(D * D, cplx(0, 0));
+  for (uint64_t col = 0; col < D; ++col) {
+    if ((col & cmask) != cval) {
+      M[col * D + col] = 1.0;
+      continue;
+    }
+    uint64_t cg = 0;
+    for (size_t i = 0; i < g.t.size(); ++i) cg |= ((col >> tpos[i]) & 1ull) << i;
+    for (uint64_t rg = 0; rg < gd; ++rg) {
+      uint64_t row = col & ~gmask;
+      for (size_t i = 0; i < g.t.size(); ++i) row |= ((rg >> i) & 1ull) << tpos[i];
+      M[row * D + col] = g.m[rg * gd + cg];
+    }
+  }
+  return M;
+}
+
+// gate::merge (src/cppsim/gate_merge.cpp:19-149, 316-356): qubit roles of the
+// product, targets sorted, matrix = second * first.
+static HGate merge_gates(const HGate& g1, const HGate& g2) {
+  struct TI { uint32_t q, prop; };
+  std::vector<TI> nt;
+  std::vector<std::pair<uint32_t, uint32_t>> nc;
+  auto find_t = [](const HGate& g, uint32_t q) -> int {
+    for (size_t i = 0; i < g.t.size(); ++i)
+      if (g.t[i] == q) return (int)i;
+    return -1;
+  };
+  auto find_c = [](const HGate& g, uint32_t q) -> int {
+    for (size_t i = 0; i < g.c.size(); ++i)
+      if (g.c[i] == q) return (int)i;
+    return -1;
+  };
+  for (size_t i = 0; i < g1.t.size(); ++i) {
+    int j = find_t(g2, g1.t[i]);
+    if (j >= 0) { nt.push_back({g1.t[i], g1.tprop[i] & g2.tprop[j]}); continue; }
+    j = find_c(g2, g1.t[i]);
+    if (j >= 0) { nt.push_back({g1.t[i], g1.tprop[i] & QSIM_COMMUTE_Z}); continue; }
+    nt.push_back({g1.t[i], g1.tprop[i]});
+  }
+  for (size_t i = 0; i < g1.c.size(); ++i) {
+    int j = find_t(g2, g1.c[i]);
+    if (j >= 0) { nt.push_back({g1.c[i], g2.tprop[j] & QSIM_COMMUTE_Z}); continue; }
+    j = find_c(g2, g1.c[i]);
+    if (j >= 0) {
+      if (g1.cval[i] == g2.cval[j]) nc.push_back({g1.c[i], g1.cval[i]});
+      else nt.push_back({g1.c[i], QSIM_COMMUTE_Z});
+      continue;
+    }
+    nt.push_back({g1.c[i], QSIM_COMMUTE_Z});
+  }
+  for (size_t i = 0; i < g2.t.size(); ++i) {
+    if (find_t(g1, g2.t[i]) >= 0 || find_c(g1, g2.t[i]) >= 0) continue;
+    nt.push_back({g2.t[i], g2.tprop[i]});
+  }
+  for (size_t i = 0; i < g2.c.size(); ++i) {
+    if (find_t(g1, g2.c[i]) >= 0 || find_c(g1, g2.c[i]) >= 0) continue;
+    nt.push_back({g2.c[i], QSIM_COMMUTE_Z});
+  }
+  std::sort(nt.begin(), nt.end(), [](const TI& a, const TI& b) { return a.q < b.q; });
+  std::sort(nc.begin(), nc.end());
+  HGate out;
+  for (auto& x : nt) { out.t.push_back(x.q); out.tprop.push_back(x.prop); out.pauli.push_back(0); }
+  for (auto& x : nc) { out.c.push_back(x.first); out.cval.push_back(x.second); }
+  const auto M1 = extend_matrix(g1, out.t);
+  const auto M2 = extend_matrix(g2, out.t);
+  const size_t D = 1ull << out.t.size();
+  out.m.assign(D * D, cplx(0, 0));
+  for (size_t r = 0; r < D; ++r)
+    for (size_t k = 0; k < D; ++k) {
+      const cplx a = M2[r * D + k];
+      if (a == cplx(0, 0)) continue;
+      for (size_t c = 0; c < D; ++c) out.m[r * D + c] += a * M1[k * D + c];
+    }
+  out.flags = g1.flags & g2.flags;
+  out.param = -1;
+  return out;
+}
+
+// QuantumCircuitOptimizer::optimize(circuit, max_block_size, swap_level=0)
+// (src/cppsim/circuit_optimizer.cpp:113-168, helpers 35-111).
+static std::vector<HGate> optimize_circuit(std::vector<HGate> gl, uint32_t max_block) {
+  auto rightmost = [&](size_t i) {
+    size_t cur = i + 1;
+    for (; cur < gl.size(); ++cur)
+      if (!gates_commute(gl[i], gl[cur])) break;
+    return cur - 1;
+  };
+  auto leftmost = [&](size_t i) {
+    long cur = (long)i - 1;
+    for (; cur >= 0; --cur)
+      if (!gates_commute(gl[i], gl[cur])) break;
+    return (size_t)(cur + 1);
+  };
+  auto merged_size = [&](size_t a, size_t b) {
+    std::set<uint32_t> s;
+    for (auto q : gl[a].t) s.insert(q);
+    for (auto q : gl[b].t) s.insert(q);
+    for (auto q : gl[a].c) s.insert(q);
+    for (auto q : gl[b].c) s.insert(q);
+    return (uint32_t)s.size();
+  };
+  bool merged = true;
+  while (merged) {
+    merged = false;
+    for (size_t i1 = 0; i1 < gl.size(); ++i1) {
+      for (size_t i2 = i1 + 1; i2 < gl.size(); ++i2) {
+        if (gl[i1].parametric() || gl[i2].parametric()) continue;
+        if (merged_size(i1, i2) > max_block) continue;
+        if (leftmost(i2) > rightmost(i1) + 1) continue;
+        HGate mg = merge_gates(gl[i1], gl[i2]);
+        const size_t ins = std::max(leftmost(i2), i1 + 1);
+        gl.erase(gl.begin() + i2);
+        gl.insert(gl.begin() + ins, std::move(mg));
+        gl.erase(gl.begin() + i1);
+        --i2;
+        merged = true;
+      }
+    }
+  }
+  return gl;
+}
+
+// ---------------------------------------------------------------- planner
+// A gate rewritten in the local qubit numbering of one pass program.
+struct LGate {
+  bool diag = false;
+  std::vector<uint32_t> N;      // non-diagonal targets (matrix bit order)
+  std::vector<uint32_t> S;      // selector qubits
+  std::vector<cplx> mats;       // 2^|S| matrices of 2^|N| x 2^|N|
+  uint64_t ident = 0;
+  std::vector<uint32_t> Q;      // diagonal form
+  std::vector<cplx> d;
+};
+
+static bool is_identity(const cplx* m, size_t D) {
+  for (size_t r = 0; r < D; ++r)
+    for (size_t c = 0; c < D; ++c)
+      if (m[r * D + c] != (r == c ? cplx(1, 0) : cplx(0, 0))) return false;
+  return true;
+}
+
+static LGate lower_gate(const HGate& g, const std::vector<int>& qmap) {
+  LGate L;
+  const size_t K = g.t.size(), D = 1ull << K;
+  auto lq = [&](uint32_t q) -> uint32_t {
+    if (q >= qmap.size() || qmap[q] < 0) throw std::logic_error("gate qubit outside the pass program");
+    return (uint32_t)qmap[q];
+  };
+  std::vector<int> diag_in(K, 1);
+  for (size_t i = 0; i < K; ++i)
+    for (size_t r = 0; r < D && diag_in[i]; ++r)
+      for (size_t c = 0; c < D; ++c)
+        if ((((r ^ c) >> i) & 1u) && g.m[r * D + c] != cplx(0, 0)) { diag_in[i] = 0; break; }
+  std::vector<size_t> ni, di;
+  for (size_t i = 0; i < K; ++i) (diag_in[i] ? di : ni).push_back(i);
+  const size_t nc = g.c.size();
+  if (ni.empty()) {
+    L.diag = true;
+    for (size_t i = 0; i < K; ++i) L.Q.push_back(lq(g.t[i]));
+    for (size_t i = 0; i < nc; ++i) L.Q.push_back(lq(g.c[i]));
+    const size_t QD = 1ull << (K + nc);
+    L.d.resize(QD);
+    for (size_t x = 0; x < QD; ++x) {
+      bool ok = true;
+      for (size_t i = 0; i < nc; ++i)
+        if (((x >> (K + i)) & 1u) != g.cval[i]) ok = false;
+      const size_t tx = x & (D - 1);
+      L.d[x] = ok ? g.m[tx * D + tx] : cplx(1, 0);
+    }
+    return L;
+  }
+  if (ni.size() > (size_t)kMaxDenseK) throw std::invalid_argument("gate with more than 5 non-diagonal targets");
+  if (di.size() + nc > (size_t)kMaxSel) throw std::invalid_argument("gate with more than 6 diagonal/control qubits");
+  for (auto i : ni) L.N.push_back(lq(g.t[i]));
+  for (auto i : di) L.S.push_back(lq(g.t[i]));
+  for (size_t i = 0; i < nc; ++i) L.S.push_back(lq(g.c[i]));
+  const size_t nN = ni.size(), ND = 1ull << nN, nS = L.S.size(), SD = 1ull << nS;
+  L.mats.assign(SD * ND * ND, cplx(0, 0));
+  for (size_t s = 0; s < SD; ++s) {
+    cplx* Ms = &L.mats[s * ND * ND];
+    bool ctl_ok = true;
+    for (size_t i = 0; i < nc; ++i)
+      if (((s >> (di.size() + i)) & 1u) != g.cval[i]) ctl_ok = false;
+    if (!ctl_ok) {
+      for (size_t r = 0; r < ND; ++r) Ms[r * ND + r] = 1.0;
+      L.ident |= 1ull << s;
+      continue;
+    }
+    uint64_t dbits = 0;
+    for (size_t j = 0; j < di.size(); ++j) dbits |= ((s >> j) & 1ull) << di[j];
+    for (size_t rn = 0; rn < ND; ++rn)
+      for (size_t cn = 0; cn < ND; ++cn) {
+        uint64_t r = dbits, c = dbits;
+        for (size_t j = 0; j < nN; ++j) {
+          r |= ((rn >> j) & 1ull) << ni[j];
+          c |= ((cn >> j) & 1ull) << ni[j];
+        }
+        Ms[rn * ND + cn] = g.m[r * D + c];
+      }
+    if (is_identity(Ms, ND)) L.ident |= 1ull << s;
+  }
+  return L;
+}
+
+struct PassPlan {
+  uint32_t t, c;
+  std::vector<uint32_t> E;
+  std::vector<size_t> gates;
+};
+
+static int choose_c(const std::set<uint32_t>& R, uint32_t t, uint32_t cmin) {
+  for (int c = (int)t; c >= (int)cmin; --c) {
+    uint32_t cnt = 0;
+    for (auto q : R)
+      if (q >= (uint32_t)c) ++cnt;
+    if ((uint32_t)c + cnt <= t) return c;
+  }
+  return -1;
+}
+
+struct Program {
+  uint32_t L = 0;
+  std::vector<PassArgs> passes;
+  std::vector<Op> ops;
+  std::vector<PhaseTerm> terms;
+  std::vector<DiagTable> diags;
+  std::vector<double> coefs;
+  DBuf d_args, d_ops, d_terms, d_diags, d_coefs;
+  bool uploaded = false;
+
+  void upload_all() {
+    upload(d_ops, ops.data(), ops.size() * sizeof(Op));
+    upload(d_terms, terms.data(), terms.size() * sizeof(PhaseTerm));
+    upload(d_diags, diags.data(), diags.size() * sizeof(DiagTable));
+    upload(d_coefs, coefs.data(), coefs.size() * sizeof(double));
+    for (auto& p : passes) {
+      p.ops = d_ops.as<Op>();
+      p.terms = d_terms.as<PhaseTerm>();
+      p.diags = d_diags.as<DiagTable>();
+      p.coefs = d_coefs.as<double>();
+    }
+    upload(d_args, passes.data(), passes.size() * sizeof(PassArgs));
+    HIPCHECK(hipStreamSynchronize(stream()));
+    uploaded = true;
+  }
+  const PassArgs* dargs(size_t i) const { return d_args.as<PassArgs>() + i; }
+};
+
+static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& lg) {
+  PassArgs a{};
+  a.t = pp.t;
+  a.c = pp.c;
+  a.nE = (uint32_t)pp.E.size();
+  a.L = P.L;
+  for (size_t i = 0; i < pp.E.size(); ++i) a.E[i] = pp.E[i];
+  a.op_begin = (uint32_t)P.ops.size();
+  auto tpos = [&](uint32_t q) -> int {
+    if (q < pp.c) return (int)q;
+    for (size_t i = 0; i < pp.E.size(); ++i)
+      if (pp.E[i] == q) return (int)(pp.c + i);
+    return -1;
+  };
+  bool elem_open = false;
+  for (size_t gi : pp.gates) {
+    const LGate& g = lg[gi];
+    if (g.diag) {
+      if (!elem_open) {
+        Op op{};
+        op.kind = OP_ELEM;
+        op.term_off = (uint32_t)P.terms.size();
+        op.diag_off = (uint32_t)P.diags.size();
+        P.ops.push_back(op);
+        elem_open = true;
+      }
+      Op& op = P.ops.back();
+      bool unit = true;
+      for (auto& v : g.d)
+        if (std::fabs(std::norm(v) - 1.0) > 1e-13) unit = false;
+      if (unit) {
+        for (size_t x = 0; x < g.d.size(); ++x) {
+          const double ang = std::atan2(g.d[x].imag(), g.d[x].real());
+          if (ang == 0.0) continue;
+          PhaseTerm T{};
+          for (size_t j = 0; j < g.Q.size(); ++j) {
+            const uint32_t q = g.Q[j];
+            const uint64_t bit = (x >> j) & 1ull;
+            const int p = tpos(q);
+            if (p >= 0) {
+              T.tmask |= 1u << p;
+              T.tval |= (uint32_t)bit << p;
+            } else {
+              T.bmask |= 1ull << q;
+              T.bval |= bit << q;
+            }
+          }
+          T.angle = ang;
+          P.terms.push_back(T);
+          op.nterms++;
+        }
+      } else {
+        if (g.Q.size() > 8) throw std::invalid_argument("non-unitary diagonal gate on more than 8 qubits");
+        DiagTable Dt{};
+        Dt.m = (uint32_t)g.Q.size();
+        Dt.coef_off = (uint32_t)(P.coefs.size() / 2);
+        for (size_t j = 0; j < g.Q.size(); ++j) {
+          const int p = tpos(g.Q[j]);
+          Dt.tile[j] = p >= 0 ? 1 : 0;
+          Dt.pos[j] = (uint16_t)(p >= 0 ? p : g.Q[j]);
+        }
+        for (auto& v : g.d) { P.coefs.push_back(v.real()); P.coefs.push_back(v.imag()); }
+        P.diags.push_back(Dt);
+        op.ndiag++;
+      }
+      continue;
+    }
+    elem_open = false;
+    Op op{};
+    op.kind = OP_DENSE;
+    op.k = (uint32_t)g.N.size();
+    op.nsel = (uint32_t)g.S.size();
+    op.ident_mask = g.ident;
+    std::vector<uint16_t> srt;
+    for (size_t j = 0; j < g.N.size(); ++j) {
+      const int p = tpos(g.N[j]);
+      if (p < 0) throw std::logic_error("planner: dense target outside the tile");
+      op.tpos[j] = (uint16_t)p;
+      srt.push_back((uint16_t)p);
+    }
+    std::sort(srt.begin(), srt.end());
+    for (size_t j = 0; j < srt.size(); ++j) op.tsorted[j] = srt[j];
+    for (size_t j = 0; j < g.S.size(); ++j) {
+      const int p = tpos(g.S[j]);
+      op.sel_tile[j] = p >= 0 ? 1 : 0;
+      op.sel_pos[j] = (uint16_t)(p >= 0 ? p : g.S[j]);
+    }
+    op.mat_off = (uint32_t)(P.coefs.size() / 2);
+    for (auto& v : g.mats) { P.coefs.push_back(v.real()); P.coefs.push_back(v.imag()); }
+    P.ops.push_back(op);
+  }
+  a.op_end = (uint32_t)P.ops.size();
+  P.passes.push_back(a);
+}
+
+// Greedy split of a gate list into LDS passes: a pass takes gates in order
+// while the union of their non-diagonal targets fits into a tile of 2^t
+// amplitudes whose low c >= cmin qubits are contiguous.  Diagonal gates fit
+// every pass.
+static void plan_program(Program& P, const std::vector<LGate>& lg, uint32_t L, uint32_t tmax, uint32_t cmin,
+                         bool at_least_one_pass) {
+  P.L = L;
+  const uint32_t t = std::min(tmax, L);
+  cmin = std::min(cmin, t);
+  std::vector<PassPlan> plans;
+  std::set<uint32_t> R;
+  PassPlan cur{t, t, {}, {}};
+  bool open = false;
+  auto close = [&]() {
+    int c = choose_c(R, t, cmin);
+    if (c < 0) throw std::logic_error("planner: tile overflow");
+    cur.c = (uint32_t)c;
+    cur.E.clear();
+    for (auto q : R)
+      if (q >= (uint32_t)c) cur.E.push_back(q);
+    plans.push_back(cur);
+    cur = PassPlan{t, t, {}, {}};
+    R.clear();
+    open = false;
+  };
+  for (size_t i = 0; i < lg.size(); ++i) {
+    const LGate& g = lg[i];
+    if (!g.diag) {
+      std::set<uint32_t> R2 = R;
+      for (auto q : g.N) R2.insert(q);
+      if (choose_c(R2, t, cmin) < 0) {
+        if (!open) throw std::invalid_argument("gate does not fit in one LDS tile");
+        close();
+        R2.clear();
+        for (auto q : g.N) R2.insert(q);
+        if (choose_c(R2, t, cmin) < 0) throw std::invalid_argument("gate does not fit in one LDS tile");
+      }
+      R = R2;
+    }
+    cur.gates.push_back(i);
+    open = true;
+  }
+  if (open || (at_least_one_pass && plans.empty())) close();
+  for (auto& pp : plans) emit_pass(P, pp, lg);
+}
+
+// --------------------------------------------------------- reductions (DD)
+struct DDAcc {
+  double s = 0.0, c = 0.0;
+  void add(double x) {
+    const double t = s + x;
+    const double bp = t - s;
+    c += (s - (t - bp)) + (x - bp);
+    s = t;
+  }
+  void merge(double s2, double c2) { add(s2); c += c2; }
+  double value() const { return s + c; }
+};
+
+struct Partials {
+  DBuf d;
+  std::vector<double> h;
+  void ensure(size_t nv, unsigned grid) {
+    d.ensure(sizeof(double) * 2 * nv * grid);
+    h.resize(2 * nv * grid);
+  }
+  void fetch(size_t nv, unsigned grid) {
+    HIPCHECK(hipMemcpyAsync(h.data(), d.p, sizeof(double) * 2 * nv * grid, hipMemcpyDeviceToHost, stream()));
+    HIPCHECK(hipStreamSynchronize(stream()));
+  }
+  void accumulate(size_t nv, unsigned grid, DDAcc* acc) const {
+    for (unsigned b = 0; b < grid; ++b)
+      for (size_t v = 0; v < nv; ++v) acc[v].merge(h[(b * nv + v) * 2], h[(b * nv + v) * 2 + 1]);
+  }
+};
+
+}  // namespace qsim
+
+// ====================================================================== SV
+using namespace qsim;
+
+struct qsim_sv {
+  uint32_t n = 0;
+  uint64_t dim = 0;
+  DBuf d;
+  mutable Partials part;
+  double2* ptr() const { return reinterpret_cast<double2*>(d.p); }
+};
+
+namespace qsim {
+static uint32_t default_tile_bits() {
+  const char* e = std::getenv("QOMPSIM_TILE_BITS");
+  if (e) {
+    int v = std::atoi(e);
+    if (v >= 5 && v <= kMaxTileBits) return (uint32_t)v;
+  }
+  return 12;
+}
+
+static void sv_apply_gates(qsim_sv* sv, const std::vector<HGate>& gs) {
+  if (gs.empty()) return;
+  std::vector<int> qmap(sv->n);
+  std::iota(qmap.begin(), qmap.end(), 0);
+  std::vector<LGate> lg;
+  lg.reserve(gs.size());
+  for (auto& g : gs) lg.push_back(lower_gate(g, qmap));
+  Program P;
+  plan_program(P, lg, sv->n, default_tile_bits(), 5, false);
+  if (P.passes.empty()) return;
+  P.upload_all();
+  for (size_t i = 0; i < P.passes.size(); ++i) {
+    PassDyn dy{};
+    dy.raw = reinterpret_cast<double*>(sv->d.p);
+    const uint32_t t = P.passes[i].t;
+    HIPCHECK(launch_pass(IN_RAW, OUT_RAW, (int)t, P.dargs(i), dy, 1ull << (sv->n - t), stream()));
+  }
+  HIPCHECK(hipStreamSynchronize(stream()));
+}
+}  // namespace qsim
+
+// ====================================================================== CS
+namespace qsim {
+struct CStage {
+  std::vector<uint32_t> gate_ids;
+  std::vector<uint32_t> H, O;
+  std::vector<uint32_t> b2g;  // as registered (reference layout)
+  uint32_t num_groups = 0;
+  std::unique_ptr<Program> prog;
+};
+}  // namespace qsim
+
+struct qsim_cs {
+  uint32_t n = 0, B = 0;
+  uint64_t nb = 0, cpb = 0, dim = 0;
+  qsim_cs_config cfg{};
+  std::vector<HGate> gates;
+  std::vector<CStage> stages;
+  DBuf desc[2], arena[2], cursors[2];
+  uint64_t cap_words[2] = {0, 0};
+  DBuf null_flags, nonzero, overflow, bmap, scratch, bytes_tmp;
+  uint64_t scratch_amps = 0;
+  int cur = 0;
+  std::vector<uint8_t> h_null;
+  std::vector<uint64_t> bytes_hist;
+  uint64_t cur_bytes = 0;
+  std::vector<std::array<double, 5>> stats;
+  double timers[QSIM_TIMER_COUNT] = {0};
+  Program io;  // zero-op program over one block (load / readout)
+  Partials part;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace qsim {
+static uint64_t words_for_cap(uint64_t bytes) {
+  uint64_t w = (bytes + 7) / 8;
+  w = (w + kSubArenas - 1) / kSubArenas * kSubArenas;
+  return w;
+}
+
+static void cs_alloc_arena(qsim_cs* cs, int which, uint64_t words) {
+  cs->arena[which].alloc(words * 8);
+  cs->cap_words[which] = words;
+}
+
+static void cs_reset_out(qsim_cs* cs, int out) {
+  HIPCHECK(launch_fill_u64(cs->cursors[out].as<unsigned long long>(), (uint64_t)kSubArenas * 16, 0ull, stream()));
+  HIPCHECK(launch_fill_u8(cs->overflow.as<uint8_t>(), sizeof(uint32_t), 0, stream()));
+}
+
+static uint64_t cs_read_cursor_words(qsim_cs* cs, int which, uint64_t* maxsub) {
+  std::vector<unsigned long long> h((size_t)kSubArenas * 16);
+  HIPCHECK(hipMemcpyAsync(h.data(), cs->cursors[which].p, h.size() * 8, hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  uint64_t tot = 0, mx = 0;
+  for (int s = 0; s < kSubArenas; ++s) {
+    tot += h[(size_t)s * 16];
+    mx = std::max<uint64_t>(mx, h[(size_t)s * 16]);
+  }
+  if (maxsub) *maxsub = mx;
+  return tot;
+}
+
+static bool cs_check_overflow(qsim_cs* cs, int out) {
+  uint32_t ov = 0;
+  HIPCHECK(hipMemcpyAsync(&ov, cs->overflow.p, sizeof(ov), hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  if (!ov) return false;
+  uint64_t mx = 0;
+  cs_read_cursor_words(cs, out, &mx);
+  const uint64_t need = mx * kSubArenas;
+  uint64_t nw = std::max<uint64_t>(cs->cap_words[out] * 2, need + need / 4);
+  nw = words_for_cap(nw * 8);
+  cs_alloc_arena(cs, out, nw);
+  return true;
+}
+
+static PassDyn cs_dyn(qsim_cs* cs, int out) {
+  PassDyn dy{};
+  dy.null_in = cs->null_flags.as<uint8_t>();
+  dy.desc_in = cs->desc[cs->cur].as<uint64_t>();
+  dy.arena_in = cs->arena[cs->cur].as<uint64_t>();
+  dy.desc_out = cs->desc[out].as<uint64_t>();
+  dy.arena_out = cs->arena[out].as<uint64_t>();
+  dy.sub_cap_words = cs->cap_words[out] / kSubArenas;
+  dy.cursors = cs->cursors[out].as<unsigned long long>();
+  dy.overflow = cs->overflow.as<uint32_t>();
+  dy.nonzero = cs->nonzero.as<uint8_t>();
+  return dy;
+}
+
+static void cs_fill_codec_fields(qsim_cs* cs, Program& P, uint32_t khi, int zero_mode) {
+  for (auto& a : P.passes) {
+    a.B = cs->B;
+    a.khi = khi;
+    a.cpb = cs->cpb;
+    a.eps_rel = cs->cfg.eps_rel;
+    a.eps_abs = cs->cfg.eps_abs;
+    a.guard = cs->cfg.guard_bits;
+    a.zero_mode = zero_mode;
+    a.zero_thr = cs->cfg.zero_threshold;
+  }
+}
+
+static void cs_commit(qsim_cs* cs, int out) {
+  HIPCHECK(launch_commit_null(cs->null_flags.as<uint8_t>(), cs->nonzero.as<uint8_t>(), cs->nb, stream()));
+  HIPCHECK(hipMemcpyAsync(cs->h_null.data(), cs->null_flags.p, cs->nb, hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  cs->cur = out;
+}
+
+static uint64_t cs_current_bytes(qsim_cs* cs) {
+  const uint64_t words = cs_read_cursor_words(cs, cs->cur, nullptr);
+  uint64_t nonnull = 0;
+  for (auto v : cs->h_null) nonnull += v ? 0 : 1;
+  return 8 * (words + nonnull * cs->cpb);
+}
+
+// Encode the blocks held in scratch (block i of the batch at scratch +
+// i*2^B) into the output arena; block ids from h_bmap.
+static void cs_encode_scratch(qsim_cs* cs, int out, const std::vector<uint32_t>& ids) {
+  upload(cs->bmap, ids.data(), ids.size() * sizeof(uint32_t));
+  PassDyn dy = cs_dyn(cs, out);
+  dy.raw = cs->scratch.as<double>();
+  dy.bmap = cs->bmap.as<uint32_t>();
+  const uint32_t t = cs->io.passes[0].t;
+  HIPCHECK(launch_pass(IN_RAW, OUT_ENC, (int)t, cs->io.dargs(0), dy, (uint64_t)ids.size() << (cs->B - t), stream()));
+}
+
+static void cs_decode_to_scratch(qsim_cs* cs, const std::vector<uint32_t>& ids) {
+  upload(cs->bmap, ids.data(), ids.size() * sizeof(uint32_t));
+  PassDyn dy = cs_dyn(cs, 1 - cs->cur);
+  dy.raw = cs->scratch.as<double>();
+  dy.bmap = cs->bmap.as<uint32_t>();
+  const uint32_t t = cs->io.passes[0].t;
+  HIPCHECK(launch_pass(IN_DEC, OUT_RAW, (int)t, cs->io.dargs(0), dy, (uint64_t)ids.size() << (cs->B - t), stream()));
+}
+
+static uint64_t cs_blocks_per_batch(qsim_cs* cs) {
+  return std::max<uint64_t>(1, std::min<uint64_t>(cs->nb, cs->scratch_amps >> cs->B));
+}
+
+// Replace the whole state by a host vector (nullptr: basis state `basis`).
+static void cs_load(qsim_cs* cs, const double* host, uint64_t basis) {
+  const int out = 1 - cs->cur;
+  for (int attempt = 0;; ++attempt) {
+    HIPCHECK(launch_fill_u8(cs->null_flags.as<uint8_t>(), cs->nb, 1, stream()));
+    HIPCHECK(launch_fill_u8(cs->nonzero.as<uint8_t>(), cs->nb, 0, stream()));
+    cs_reset_out(cs, out);
+    if (host) {
+      const uint64_t bpb = cs_blocks_per_batch(cs);
+      for (uint64_t b0 = 0; b0 < cs->nb; b0 += bpb) {
+        const uint64_t cnt = std::min(bpb, cs->nb - b0);
+        HIPCHECK(hipMemcpyAsync(cs->scratch.p, host + 2 * (b0 << cs->B), (cnt << cs->B) * 16, hipMemcpyHostToDevice,
+                                stream()));
+        std::vector<uint32_t> ids(cnt);
+        std::iota(ids.begin(), ids.end(), (uint32_t)b0);
+        cs_encode_scratch(cs, out, ids);
+        HIPCHECK(hipStreamSynchronize(stream()));
+      }
+    } else {
+      const uint32_t b = (uint32_t)(basis >> cs->B);
+      HIPCHECK(launch_fill_basis(cs->scratch.as<double2>(), 1ull << cs->B, basis & ((1ull << cs->B) - 1), stream()));
+      cs_encode_scratch(cs, out, {b});
+    }
+    if (!cs_check_overflow(cs, out)) break;
+    if (attempt > 8) throw std::runtime_error("compressed store: arena growth did not converge");
+  }
+  cs_commit(cs, out);
+  cs->cur_bytes = cs_current_bytes(cs);
+}
+
+// make_block_groups (compressed_state.hpp:585-671) for one stage.
+static void stage_groups(qsim_cs* cs, CStage& S) {
+  std::set<uint32_t> inner;
+  for (auto gid : S.gate_ids) {
+    for (auto q : cs->gates[gid].t)
+      if (q >= cs->B) inner.insert(q);
+    for (auto q : cs->gates[gid].c)
+      if (q >= cs->B) inner.insert(q);
+  }
+  S.H.assign(inner.begin(), inner.end());
+  S.O.clear();
+  for (uint32_t q = cs->B; q < cs->n; ++q)
+    if (!inner.count(q)) S.O.push_back(q);
+}
+
+static std::vector<uint32_t> canonical_b2g(qsim_cs* cs, const CStage& S) {
+  std::vector<uint32_t> b2g(cs->nb);
+  for (uint64_t b = 0; b < cs->nb; ++b) {
+    uint32_t g = 0;
+    for (size_t i = 0; i < S.O.size(); ++i) g |= (uint32_t)((b >> (S.O[i] - cs->B)) & 1u) << i;
+    b2g[b] = g;
+  }
+  return b2g;
+}
+
+static void build_stage_program(qsim_cs* cs, CStage& S) {
+  const uint32_t k = (uint32_t)S.H.size();
+  const uint32_t L = cs->B + k;
+  if (L > (uint32_t)kMaxLocalBits) throw std::invalid_argument("stage touches too many high qubits");
+  std::vector<int> qmap(cs->n, -1);
+  for (uint32_t q = 0; q < cs->B; ++q) qmap[q] = (int)q;
+  for (uint32_t i = 0; i < k; ++i) qmap[S.H[i]] = (int)(cs->B + i);
+  std::vector<LGate> lg;
+  for (auto gid : S.gate_ids) lg.push_back(lower_gate(cs->gates[gid], qmap));
+  S.prog.reset(new Program());
+  const uint32_t tb = cs->cfg.tile_bits ? cs->cfg.tile_bits : default_tile_bits();
+  plan_program(*S.prog, lg, L, std::min<uint32_t>(tb, L), kChunkAmpsLog2, true);
+  cs_fill_codec_fields(cs, *S.prog, k, cs->cfg.zero_mode);
+  S.prog->upload_all();
+  const uint64_t need = (1ull << L);
+  if (S.prog->passes.size() > 1 && cs->scratch_amps < need) {
+    cs->scratch.alloc(need * 16);
+    cs->scratch_amps = need;
+  }
+}
+
+// partition_circuit (compressed_state.hpp:509-583)
+static std::vector<std::vector<uint32_t>> partition(qsim_cs* cs, uint32_t inner_size) {
+  std::vector<std::vector<uint32_t>> stages;
+  std::vector<uint32_t> current, gidx;
+  const uint32_t thr = std::max<uint32_t>(inner_size, 2u);
+  for (uint32_t i = 0; i < cs->gates.size(); ++i) {
+    const HGate& g = cs->gates[i];
+    std::vector<uint32_t> gt;
+    for (auto q : g.t)
+      if (q >= cs->B) gt.push_back(q);
+    for (auto q : g.c)
+      if (q >= cs->B) gt.push_back(q);
+    std::vector<uint32_t> tmp = gidx;
+    for (auto q : gt)
+      if (std::find(tmp.begin(), tmp.end(), q) == tmp.end()) tmp.push_back(q);
+    if (tmp.size() > thr) {
+      stages.push_back(current);
+      current.clear();
+      tmp = gt;
+    }
+    gidx = tmp;
+    current.push_back(i);
+  }
+  if (!current.empty()) stages.push_back(current);
+  return stages;
+}
+
+static void cs_run_stage(qsim_cs* cs, uint32_t si) {
+  CStage& S = cs->stages[si];
+  Program& P = *S.prog;
+  const uint32_t k = (uint32_t)S.H.size();
+  const uint32_t L = cs->B + k;
+  const uint64_t G = 1ull << k;
+  // build_stage_group_execution_plan (compressed_state.hpp:797-837) on the
+  // canonical groups: a group is active when it holds a non-null block.
+  const auto b2g = canonical_b2g(cs, S);
+  const uint64_t ng = 1ull << S.O.size();
+  std::vector<uint8_t> active(ng, 0);
+  for (uint64_t b = 0; b < cs->nb; ++b)
+    if (!cs->h_null[b]) active[b2g[b]] = 1;
+  std::vector<uint32_t> bm;
+  uint64_t nact = 0;
+  for (uint64_t g = 0; g < ng; ++g) {
+    if (!active[g]) continue;
+    uint64_t obits = 0;
+    for (size_t i = 0; i < S.O.size(); ++i) obits |= ((g >> i) & 1ull) << (S.O[i] - cs->B);
+    for (uint64_t j = 0; j < G; ++j) {
+      uint64_t b = obits;
+      for (uint32_t i = 0; i < k; ++i) b |= ((j >> i) & 1ull) << (S.H[i] - cs->B);
+      bm.push_back((uint32_t)b);
+    }
+    ++nact;
+  }
+  const uint64_t bytes_in = cs->cur_bytes;
+  if (nact == 0 || P.passes.empty()) {
+    cs->bytes_hist.push_back(bytes_in);
+    cs->stats.push_back({0.0, (double)bytes_in, (double)bytes_in, 0.0, 0.0});
+    return;
+  }
+  upload(cs->bmap, bm.data(), bm.size() * sizeof(uint32_t));
+  const int out = 1 - cs->cur;
+  float ms = 0.f;
+  for (int attempt = 0;; ++attempt) {
+    HIPCHECK(launch_fill_u8(cs->nonzero.as<uint8_t>(), cs->nb, 0, stream()));
+    cs_reset_out(cs, out);
+    HIPCHECK(hipEventRecord(cs->ev0, stream()));
+    PassDyn dy = cs_dyn(cs, out);
+    if (P.passes.size() == 1) {
+      dy.bmap = cs->bmap.as<uint32_t>();
+      const uint32_t t = P.passes[0].t;
+      HIPCHECK(launch_pass(IN_DEC, OUT_ENC, (int)t, P.dargs(0), dy, nact << (L - t), stream()));
+    } else {
+      const uint64_t gpb = std::max<uint64_t>(1, cs->scratch_amps >> L);
+      dy.raw = cs->scratch.as<double>();
+      for (uint64_t a0 = 0; a0 < nact; a0 += gpb) {
+        const uint64_t cnt = std::min(gpb, nact - a0);
+        dy.bmap = cs->bmap.as<uint32_t>() + a0 * G;
+        for (size_t p = 0; p < P.passes.size(); ++p) {
+          const int in = p == 0 ? IN_DEC : IN_RAW;
+          const int om = p + 1 == P.passes.size() ? OUT_ENC : OUT_RAW;
+          const uint32_t t = P.passes[p].t;
+          HIPCHECK(launch_pass(in, om, (int)t, P.dargs(p), dy, cnt << (L - t), stream()));
+        }
+      }
+    }
+    HIPCHECK(hipEventRecord(cs->ev1, stream()));
+    if (!cs_check_overflow(cs, out)) break;
+    if (attempt > 8) throw std::runtime_error("compressed store: arena growth did not converge");
+  }
+  HIPCHECK(hipEventElapsedTime(&ms, cs->ev0, cs->ev1));
+  cs_commit(cs, out);
+  const uint64_t bytes_out = cs_current_bytes(cs);
+  cs->cur_bytes = bytes_out;
+  cs->bytes_hist.push_back(bytes_out);
+  cs->timers[1] += ms * 1e-3;
+  cs->stats.push_back({ms * 1e-3, (double)bytes_in, (double)bytes_out, (double)(nact << L), (double)P.passes.size()});
+}
+
+static void cs_register(qsim_cs* cs, std::vector<HGate> gs, const std::vector<std::vector<uint32_t>>& stage_ids,
+                        const std::vector<std::vector<uint32_t>>* user_b2g, const std::vector<uint32_t>* user_ng) {
+  check_qubits(gs, cs->n);
+  cs->gates = std::move(gs);
+  std::vector<CStage> st(stage_ids.size());
+  for (size_t s = 0; s < stage_ids.size(); ++s) {
+    for (auto g : stage_ids[s])
+      if (g >= cs->gates.size()) throw std::out_of_range("stage gate id out of range");
+    st[s].gate_ids = stage_ids[s];
+    stage_groups(cs, st[s]);
+    const auto canon = canonical_b2g(cs, st[s]);
+    if (user_b2g) {
+      // each registered group must be a union of canonical groups
+      const auto& ub = (*user_b2g)[s];
+      if (ub.size() != cs->nb) throw std::invalid_argument("block_to_group size mismatch");
+      std::vector<int64_t> owner(1ull << st[s].O.size(), -1);
+      for (uint64_t b = 0; b < cs->nb; ++b) {
+        int64_t& o = owner[canon[b]];
+        if (o < 0) o = ub[b];
+        else if (o != (int64_t)ub[b]) throw std::out_of_range("block group splits a gate pair (block not found in group)");
+      }
+      st[s].b2g = ub;
+      st[s].num_groups = (*user_ng)[s];
+    } else {
+      st[s].b2g = canon;
+      st[s].num_groups = 1u << st[s].O.size();
+    }
+    build_stage_program(cs, st[s]);
+  }
+  cs->stages = std::move(st);
+}
+
+}  // namespace qsim
+
+// ================================================================= C ABI
+#define API_BEGIN try {
+#define API_END                                  \
+  return 0;                                      \
+  }                                              \
+  catch (const std::invalid_argument& e) {       \
+    g_err = e.what();                            \
+    return -2;                                   \
+  }                                              \
+  catch (const std::out_of_range& e) {           \
+    g_err = e.what();                            \
+    return -3;                                   \
+  }                                              \
+  catch (const std::exception& e) {              \
+    g_err = e.what();                            \
+    return -1;                                   \
+  }                                              \
+  catch (...) {                                  \
+    g_err = "unknown error";                     \
+    return -1;                                   \
+  }
+
+static void require(bool c, const char* msg) {
+  if (!c) throw std::invalid_argument(msg);
+}
+
+extern "C" {
+
+const char* qsim_last_error(void) { return g_err.c_str(); }
+int qsim_version(void) { return 1; }
+
+int qsim_device_count(int* count) {
+  API_BEGIN
+  HIPCHECK(hipGetDeviceCount(count));
+  API_END
+}
+
+int qsim_synchronize(void) {
+  API_BEGIN
+  HIPCHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+uint64_t qsim_optimize_pool_bound(uint64_t n_gates, uint64_t pool_len, uint32_t max_block_size) {
+  const uint64_t d = 1ull << std::min<uint32_t>(max_block_size, 12);
+  return pool_len + n_gates * d * d * 2 + 16;
+}
+
+static void write_gates(const std::vector<HGate>& gs, qsim_gate* out, double* pool, uint64_t cap) {
+  uint64_t off = 0;
+  for (size_t i = 0; i < gs.size(); ++i) {
+    const HGate& g = gs[i];
+    qsim_gate q{};
+    q.n_targets = (uint32_t)g.t.size();
+    q.n_controls = (uint32_t)g.c.size();
+    for (size_t j = 0; j < g.t.size(); ++j) {
+      q.targets[j] = g.t[j];
+      q.target_commute[j] = g.tprop[j];
+      q.pauli_ids[j] = j < g.pauli.size() ? g.pauli[j] : 0;
+    }
+    for (size_t j = 0; j < g.c.size(); ++j) {
+      q.controls[j] = g.c[j];
+      q.control_values[j] = g.cval[j];
+    }
+    q.flags = g.flags;
+    q.param_index = g.param;
+    q.matrix_offset = off;
+    if (2 * (off + g.m.size()) > cap) throw std::length_error("optimizer output pool too small");
+    for (size_t k = 0; k < g.m.size(); ++k) {
+      pool[2 * (off + k)] = g.m[k].real();
+      pool[2 * (off + k) + 1] = g.m[k].imag();
+    }
+    off += g.m.size();
+    out[i] = q;
+  }
+}
+
+int qsim_optimize(const qsim_gate* gates, uint64_t n_gates, const double* pool, uint64_t pool_len,
+                  uint32_t max_block_size, qsim_gate* out, uint64_t* n_out, double* pool_out, uint64_t* pool_out_len) {
+  API_BEGIN
+  auto gs = parse_gates(gates, n_gates, pool, pool_len);
+  auto opt = optimize_circuit(std::move(gs), max_block_size);
+  uint64_t need = 0;
+  for (auto& g : opt) need += 2 * g.m.size();
+  require(need <= *pool_out_len, "qsim_optimize: pool_out too small");
+  write_gates(opt, out, pool_out, *pool_out_len);
+  *n_out = opt.size();
+  *pool_out_len = need;
+  API_END
+}
+
+// ---------------------------------------------------------------- SV API
+int qsim_sv_create(uint32_t n_qubits, qsim_sv** out) {
+  API_BEGIN
+  require(n_qubits >= 1 && n_qubits <= 40, "qubit count out of range");
+  auto sv = std::make_unique<qsim_sv>();
+  sv->n = n_qubits;
+  sv->dim = 1ull << n_qubits;
+  sv->d.alloc(sv->dim * 16);
+  HIPCHECK(launch_fill_basis(sv->ptr(), sv->dim, 0, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  *out = sv.release();
+  API_END
+}
+
+int qsim_sv_destroy(qsim_sv* sv) {
+  API_BEGIN
+  delete sv;
+  API_END
+}
+
+int qsim_sv_set_basis(qsim_sv* sv, uint64_t index) {
+  API_BEGIN
+  require(index < sv->dim, "basis index out of range");
+  HIPCHECK(launch_fill_basis(sv->ptr(), sv->dim, index, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+int qsim_sv_load(qsim_sv* sv, const double* host) {
+  API_BEGIN
+  HIPCHECK(hipMemcpy(sv->d.p, host, sv->dim * 16, hipMemcpyHostToDevice));
+  API_END
+}
+
+int qsim_sv_copy(qsim_sv* dst, const qsim_sv* src) {
+  API_BEGIN
+  require(dst->n == src->n, "qubit count mismatch");
+  HIPCHECK(hipMemcpyAsync(dst->d.p, src->d.p, src->dim * 16, hipMemcpyDeviceToDevice, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+int qsim_sv_read(const qsim_sv* sv, double* host) {
+  API_BEGIN
+  HIPCHECK(hipStreamSynchronize(stream()));
+  HIPCHECK(hipMemcpy(host, sv->d.p, sv->dim * 16, hipMemcpyDeviceToHost));
+  API_END
+}
+
+int qsim_sv_apply(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, const double* pool, uint64_t pool_len) {
+  API_BEGIN
+  auto gs = parse_gates(gates, n_gates, pool, pool_len);
+  check_qubits(gs, sv->n);
+  sv_apply_gates(sv, gs);
+  API_END
+}
+
+int qsim_sv_norm_squared(const qsim_sv* sv, double* out) {
+  API_BEGIN
+  const unsigned grid = reduce_grid(sv->dim);
+  sv->part.ensure(1, grid);
+  HIPCHECK(launch_sv_norm(sv->ptr(), sv->dim, sv->part.d.as<double>(), grid, stream()));
+  sv->part.fetch(1, grid);
+  DDAcc acc;
+  sv->part.accumulate(1, grid, &acc);
+  *out = acc.value();
+  API_END
+}
+
+int qsim_sv_normalize(qsim_sv* sv, double squared_norm) {
+  API_BEGIN
+  require(squared_norm > 0.0, "normalize: non-positive norm");
+  HIPCHECK(launch_scale(sv->ptr(), sv->dim, std::sqrt(1.0 / squared_norm), stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+int qsim_sv_scale(qsim_sv* sv, double factor) {
+  API_BEGIN
+  HIPCHECK(launch_scale(sv->ptr(), sv->dim, factor, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+int qsim_sv_inner_product(const qsim_sv* bra, const qsim_sv* ket, double* re, double* im) {
+  API_BEGIN
+  require(bra->n == ket->n, "qubit count mismatch");
+  const unsigned grid = reduce_grid(bra->dim);
+  bra->part.ensure(2, grid);
+  HIPCHECK(launch_sv_inner(bra->ptr(), ket->ptr(), bra->dim, bra->part.d.as<double>(), grid, stream()));
+  bra->part.fetch(2, grid);
+  DDAcc acc[2];
+  bra->part.accumulate(2, grid, acc);
+  *re = acc[0].value();
+  *im = acc[1].value();
+  API_END
+}
+
+static void ipow(uint32_t ny, double& r, double& i) {
+  switch (ny & 3u) {
+    case 0: r = 1; i = 0; break;
+    case 1: r = 0; i = 1; break;
+    case 2: r = -1; i = 0; break;
+    default: r = 0; i = -1; break;
+  }
+}
+
+int qsim_sv_expectation_pauli(const qsim_sv* sv, uint64_t n_terms, const uint64_t* xm, const uint64_t* zm,
+                              const uint32_t* ny, const double* cre, const double* cim, double* ore, double* oim) {
+  API_BEGIN
+  const unsigned grid = reduce_grid(sv->dim);
+  sv->part.ensure(2, grid);
+  DDAcc tr, ti;
+  for (uint64_t t = 0; t < n_terms; ++t) {
+    HIPCHECK(launch_sv_pauli(sv->ptr(), sv->dim, xm[t], zm[t], sv->part.d.as<double>(), grid, stream()));
+    sv->part.fetch(2, grid);
+    DDAcc acc[2];
+    sv->part.accumulate(2, grid, acc);
+    double pr, pi;
+    ipow(ny[t], pr, pi);
+    const double sr = acc[0].value(), si = acc[1].value();
+    // coef * i^ny * (sr + i si)
+    const double fr = cre[t] * pr - cim[t] * pi, fi = cre[t] * pi + cim[t] * pr;
+    tr.add(fr * sr - fi * si);
+    ti.add(fr * si + fi * sr);
+  }
+  *ore = tr.value();
+  *oim = ti.value();
+  API_END
+}
+
+int qsim_sv_apply_pauli_sum(const qsim_sv* sv, qsim_sv* out, uint64_t n_terms, const uint64_t* xm, const uint64_t* zm,
+                            const uint32_t* ny, const double* cre, const double* cim) {
+  API_BEGIN
+  require(sv != out, "apply_pauli_sum: output must differ from input");
+  require(sv->n == out->n, "qubit count mismatch");
+  if (n_terms == 0) HIPCHECK(launch_fill_u64(out->d.as<unsigned long long>(), out->dim * 2, 0ull, stream()));
+  for (uint64_t t = 0; t < n_terms; ++t) {
+    double pr, pi;
+    ipow(ny[t], pr, pi);
+    const double fr = cre[t] * pr - cim[t] * pi, fi = cre[t] * pi + cim[t] * pr;
+    HIPCHECK(launch_sv_apply_pauli(sv->ptr(), out->ptr(), sv->dim, xm[t], zm[t], fr, fi, t > 0 ? 1 : 0, stream()));
+  }
+  HIPCHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+void* qsim_sv_device_ptr(qsim_sv* sv) { return sv ? sv->d.p : nullptr; }
+uint32_t qsim_sv_qubit_count(const qsim_sv* sv) { return sv ? sv->n : 0; }
+
+// ---------------------------------------------------------------- CS API
+int qsim_cs_default_config(qsim_cs_config* cfg) {
+  API_BEGIN
+  qsim_cs_config c{};
+  c.eps_rel = 1e-3;
+  c.eps_abs = 1e-7;
+  c.guard_bits = 1;
+  c.zero_mode = QSIM_ZERO_EXACT;
+  c.zero_threshold = 1e-6;
+  c.scratch_bytes = 0;
+  c.tile_bits = 0;
+  *cfg = c;
+  API_END
+}
+
+int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** out) {
+  API_BEGIN
+  require(n >= 5 && n <= 44, "qubit count out of range (5..44)");
+  require(B >= (uint32_t)kChunkAmpsLog2 && B <= n, "block_size must satisfy 5 <= block_size <= qubit_count");
+  require(n - B <= 30, "too many blocks (qubit_count - block_size > 30)");
+  require(B <= 26, "block_size above 26");
+  qsim_cs_config c;
+  if (cfg) c = *cfg;
+  else qsim_cs_default_config(&c);
+  if (!(c.eps_rel > 0.0)) throw std::invalid_argument("eps_rel must be > 0");
+  require(c.eps_abs >= 0.0, "eps_abs must be >= 0");
+  require(c.guard_bits >= 0 && c.guard_bits <= 20, "guard_bits out of range");
+  require(c.zero_mode == QSIM_ZERO_EXACT || c.zero_mode == QSIM_ZERO_THRESHOLD, "bad zero_mode");
+  auto cs = std::make_unique<qsim_cs>();
+  cs->n = n;
+  cs->B = B;
+  cs->cfg = c;
+  cs->nb = 1ull << (n - B);
+  cs->dim = 1ull << n;
+  cs->cpb = 1ull << (B - kChunkAmpsLog2);
+  const uint64_t raw = cs->dim * 16;
+  for (int w = 0; w < 2; ++w) {
+    cs->desc[w].alloc(cs->nb * cs->cpb * 8);
+    cs->cursors[w].alloc((size_t)kSubArenas * 16 * 8);
+    const uint64_t init = std::min<uint64_t>(std::max<uint64_t>(raw / 8, 8ull << 20), 16ull << 30);
+    cs_alloc_arena(cs.get(), w, words_for_cap(init));
+  }
+  cs->null_flags.alloc(cs->nb);
+  cs->nonzero.alloc(cs->nb);
+  cs->overflow.alloc(16);
+  cs->h_null.assign(cs->nb, 1);
+  uint64_t sbytes = c.scratch_bytes;
+  if (sbytes == 0) {
+    const char* e = std::getenv("QOMPSIM_SCRATCH_GIB");
+    sbytes = e ? (uint64_t)(std::atof(e) * (1ull << 30)) : (8ull << 30);
+  }
+  sbytes = std::min<uint64_t>(sbytes, raw);
+  sbytes = std::max<uint64_t>(sbytes, (1ull << B) * 16);
+  cs->scratch.alloc(sbytes);
+  cs->scratch_amps = sbytes / 16;
+  HIPCHECK(hipEventCreate(&cs->ev0));
+  HIPCHECK(hipEventCreate(&cs->ev1));
+  // io program: one block, no ops, exact zero detection
+  plan_program(cs->io, {}, B, std::min<uint32_t>(c.tile_bits ? c.tile_bits : default_tile_bits(), B), kChunkAmpsLog2,
+               true);
+  cs_fill_codec_fields(cs.get(), cs->io, 0, QSIM_ZERO_EXACT);
+  cs->io.upload_all();
+  cs_load(cs.get(), nullptr, 0);
+  *out = cs.release();
+  API_END
+}
+
+int qsim_cs_destroy(qsim_cs* cs) {
+  API_BEGIN
+  if (cs) {
+    if (cs->ev0) (void)hipEventDestroy(cs->ev0);
+    if (cs->ev1) (void)hipEventDestroy(cs->ev1);
+  }
+  delete cs;
+  API_END
+}
+
+int qsim_cs_partition_and_register(qsim_cs* cs, const qsim_gate* gates, uint64_t n_gates, const double* pool,
+                                   uint64_t pool_len, uint32_t inner_size) {
+  API_BEGIN
+  auto gs = parse_gates(gates, n_gates, pool, pool_len);
+  check_qubits(gs, cs->n);
+  cs->gates = gs;
+  auto st = partition(cs, inner_size);
+  cs_register(cs, std::move(gs), st, nullptr, nullptr);
+  API_END
+}
+
+int qsim_cs_register_stages(qsim_cs* cs, const qsim_gate* gates, uint64_t n_gates, const double* pool,
+                            uint64_t pool_len, uint32_t n_stages, const uint32_t* off, const uint32_t* ids,
+                            const uint32_t* b2g, const uint32_t* ng) {
+  API_BEGIN
+  auto gs = parse_gates(gates, n_gates, pool, pool_len);
+  std::vector<std::vector<uint32_t>> st(n_stages), ub(n_stages);
+  std::vector<uint32_t> ung(n_stages);
+  for (uint32_t s = 0; s < n_stages; ++s) {
+    st[s].assign(ids + off[s], ids + off[s + 1]);
+    ub[s].assign(b2g + (uint64_t)s * cs->nb, b2g + (uint64_t)(s + 1) * cs->nb);
+    ung[s] = ng[s];
+  }
+  cs_register(cs, std::move(gs), st, &ub, &ung);
+  API_END
+}
+
+int qsim_cs_num_stages(const qsim_cs* cs, uint32_t* out) {
+  API_BEGIN
+  *out = (uint32_t)cs->stages.size();
+  API_END
+}
+
+int qsim_cs_stage_info(const qsim_cs* cs, uint32_t s, uint32_t* n_gates, uint32_t* gate_ids, uint32_t* num_groups,
+                       uint32_t* b2g, uint32_t* n_passes) {
+  API_BEGIN
+  if (s >= cs->stages.size()) throw std::out_of_range("stage index out of range");
+  const CStage& S = cs->stages[s];
+  if (n_gates) *n_gates = (uint32_t)S.gate_ids.size();
+  if (gate_ids) std::copy(S.gate_ids.begin(), S.gate_ids.end(), gate_ids);
+  if (num_groups) *num_groups = S.num_groups;
+  if (b2g) std::copy(S.b2g.begin(), S.b2g.end(), b2g);
+  if (n_passes) *n_passes = (uint32_t)S.prog->passes.size();
+  API_END
+}
+
+int qsim_cs_run_stages(qsim_cs* cs, uint32_t b, uint32_t e) {
+  API_BEGIN
+  require(b <= e && e <= cs->stages.size(), "stage range out of bounds");
+  for (uint32_t s = b; s < e; ++s) cs_run_stage(cs, s);
+  API_END
+}
+
+int qsim_cs_apply_circuit(qsim_cs* cs) {
+  API_BEGIN
+  cs->stats.clear();
+  // append_total_SV_bytes before the first stage (compressed_state.hpp:880)
+  cs->bytes_hist.push_back(cs->cur_bytes);
+  for (uint32_t s = 0; s < cs->stages.size(); ++s) cs_run_stage(cs, s);
+  API_END
+}
+
+static void cs_norm_inner(qsim_cs* cs, const qsim_sv* sv, DDAcc* acc) {
+  const uint64_t bpb = cs_blocks_per_batch(cs);
+  for (uint64_t b0 = 0; b0 < cs->nb; b0 += bpb) {
+    const uint64_t cnt = std::min(bpb, cs->nb - b0);
+    std::vector<uint32_t> ids(cnt);
+    std::iota(ids.begin(), ids.end(), (uint32_t)b0);
+    cs_decode_to_scratch(cs, ids);
+    const uint64_t n = cnt << cs->B;
+    const unsigned grid = reduce_grid(n);
+    if (sv) {
+      cs->part.ensure(2, grid);
+      HIPCHECK(launch_sv_inner(sv->ptr() + (b0 << cs->B), cs->scratch.as<double2>(), n, cs->part.d.as<double>(), grid,
+                               stream()));
+      cs->part.fetch(2, grid);
+      cs->part.accumulate(2, grid, acc);
+    } else {
+      cs->part.ensure(1, grid);
+      HIPCHECK(launch_sv_norm(cs->scratch.as<double2>(), n, cs->part.d.as<double>(), grid, stream()));
+      cs->part.fetch(1, grid);
+      cs->part.accumulate(1, grid, acc);
+    }
+  }
+}
+
+int qsim_cs_norm_squared(qsim_cs* cs, double* out) {
+  API_BEGIN
+  DDAcc acc;
+  cs_norm_inner(cs, nullptr, &acc);
+  *out = acc.value();
+  API_END
+}
+
+int qsim_cs_inner_product_sv(qsim_cs* cs, const qsim_sv* sv, double* re, double* im) {
+  API_BEGIN
+  require(sv->n == cs->n, "qubit count mismatch");
+  DDAcc acc[2];
+  cs_norm_inner(cs, sv, acc);
+  *re = acc[0].value();
+  *im = acc[1].value();
+  API_END
+}
+
+int qsim_cs_read_vector(qsim_cs* cs, double* host, int normalise) {
+  API_BEGIN
+  const uint64_t bpb = cs_blocks_per_batch(cs);
+  for (uint64_t b0 = 0; b0 < cs->nb; b0 += bpb) {
+    const uint64_t cnt = std::min(bpb, cs->nb - b0);
+    std::vector<uint32_t> ids(cnt);
+    std::iota(ids.begin(), ids.end(), (uint32_t)b0);
+    cs_decode_to_scratch(cs, ids);
+    HIPCHECK(hipMemcpyAsync(host + 2 * (b0 << cs->B), cs->scratch.p, (cnt << cs->B) * 16, hipMemcpyDeviceToHost,
+                            stream()));
+    HIPCHECK(hipStreamSynchronize(stream()));
+  }
+  if (normalise) {
+    // normalise as csim normalize (update_ops_named_state.cpp): x *= sqrt(1/norm)
+    DDAcc acc;
+    for (uint64_t i = 0; i < 2 * cs->dim; ++i) acc.add(host[i] * host[i]);
+    const double nrm = acc.value();
+    if (nrm > 0.0) {
+      const double f = std::sqrt(1.0 / nrm);
+      for (uint64_t i = 0; i < 2 * cs->dim; ++i) host[i] *= f;
+    }
+  }
+  API_END
+}
+
+int qsim_cs_read_block(qsim_cs* cs, uint64_t block, double* host) {
+  API_BEGIN
+  if (block >= cs->nb) throw std::out_of_range("block index out of range");
+  cs_decode_to_scratch(cs, {(uint32_t)block});
+  HIPCHECK(hipMemcpyAsync(host, cs->scratch.p, (1ull << cs->B) * 16, hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+int qsim_cs_set_basis(qsim_cs* cs, uint64_t index) {
+  API_BEGIN
+  require(index < cs->dim, "basis index out of range");
+  cs_load(cs, nullptr, index);
+  API_END
+}
+
+int qsim_cs_load(qsim_cs* cs, const double* host) {
+  API_BEGIN
+  cs_load(cs, host, 0);
+  API_END
+}
+
+int qsim_cs_null_bitmap(const qsim_cs* cs, uint8_t* out) {
+  API_BEGIN
+  std::copy(cs->h_null.begin(), cs->h_null.end(), out);
+  API_END
+}
+
+int qsim_cs_block_bytes(qsim_cs* cs, uint64_t* out) {
+  API_BEGIN
+  cs->bytes_tmp.ensure(cs->nb * 8);
+  HIPCHECK(launch_block_bytes(cs->desc[cs->cur].as<uint64_t>(), cs->null_flags.as<uint8_t>(), cs->nb, cs->cpb,
+                              cs->bytes_tmp.as<unsigned long long>(), stream()));
+  HIPCHECK(hipMemcpyAsync(out, cs->bytes_tmp.p, cs->nb * 8, hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+int qsim_cs_bytes_history(const qsim_cs* cs, uint64_t* out, uint64_t cap, uint64_t* n) {
+  API_BEGIN
+  *n = cs->bytes_hist.size();
+  for (uint64_t i = 0; i < std::min<uint64_t>(cap, cs->bytes_hist.size()); ++i) out[i] = cs->bytes_hist[i];
+  API_END
+}
+
+int qsim_cs_max_sv_bytes(const qsim_cs* cs, uint64_t* out) {
+  API_BEGIN
+  uint64_t m = 0;
+  for (auto v : cs->bytes_hist) m = std::max(m, v);
+  *out = m;
+  API_END
+}
+
+int qsim_cs_stage_stats(const qsim_cs* cs, double* out5, uint64_t cap, uint64_t* n) {
+  API_BEGIN
+  *n = cs->stats.size();
+  for (uint64_t i = 0; i < std::min<uint64_t>(cap, cs->stats.size()); ++i)
+    for (int j = 0; j < 5; ++j) out5[i * 5 + j] = cs->stats[i][j];
+  API_END
+}
+
+int qsim_cs_times(const qsim_cs* cs, double* out) {
+  API_BEGIN
+  for (int i = 0; i < QSIM_TIMER_COUNT; ++i) out[i] = cs->timers[i];
+  API_END
+}
+
+uint32_t qsim_cs_qubit_count(const qsim_cs* cs) { return cs ? cs->n : 0; }
+uint32_t qsim_cs_block_size(const qsim_cs* cs) { return cs ? cs->B : 0; }
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// Shared layout between the C++ host planner and the HIP kernels.
+//
+// A "pass" streams a (group-local) state vector through LDS in tiles of 2^t
+// amplitudes.  Tile element p (t bits) maps to local amplitude
+//   base(tau) | off(p),  off(p) = (p & (2^c-1)) | sum_i bit(p, c+i) << E[i]
+// where the low c local qubits are contiguous in every tile row and E[] are the
+// extra local qubits that the pass needs in the tile.  Every op of the pass
+// then works on LDS only.
+//
+// Compressed block store (per block of 2^B amplitudes = 2^(B+1) doubles):
+//   chunk  = 64 consecutive doubles (32 amplitudes), one double per lane of a
+//            wave64;
+//   desc   = u64 per chunk: bits 0..39 payload offset in 8-byte words,
+//            bits 40..47 width w (0 = all-zero chunk, 64 = verbatim doubles),
+//            bits 48..63 signed step exponent e (value = k * 2^e);
+//   payload= 64 lanes x w bits, lane l's zig-zag code at bit l*w (LSB first,
+//            little-endian 64-bit words), 8*w bytes per chunk.
+#pragma once
+#include <stdint.h>
+
+namespace qsim {
+
+constexpr int kMaxDenseK = 5;      // non-diagonal targets of one dense op
+constexpr int kMaxSel = 6;         // selector qubits (diagonal targets + controls)
+constexpr int kMaxTileBits = 13;   // 2^13 complex128 = 128 KiB of LDS
+constexpr int kThreads = 256;      // work-group size of the pass kernels
+constexpr int kChunkAmpsLog2 = 5;  // 32 amplitudes = 64 doubles per codec chunk
+constexpr int kSubArenas = 64;     // independent allocation cursors per arena
+constexpr int kMaxLocalBits = 48;
+
+enum OpKind : uint32_t { OP_DENSE = 0, OP_ELEM = 1 };
+enum InMode : int { IN_RAW = 0, IN_DEC = 1 };
+enum OutMode : int { OUT_RAW = 0, OUT_ENC = 1 };
+enum ZeroMode : int { ZERO_EXACT = 0, ZERO_THRESHOLD = 1 };
+
+// One op of a pass.  Dense: 2^nsel matrices of 2^k x 2^k (row-major, matrix
+// index bit i <-> tile bit tpos[i]); the matrix is selected by the selector
+// bits (each a tile bit or a base bit).  Elementwise: a run of diagonal gates,
+// given as phase terms (angle added when the condition holds) and diagonal
+// tables.
+struct Op {
+  uint32_t kind;
+  uint32_t k;
+  uint32_t nsel;
+  uint32_t mat_off;        // complex offset into the pass coefficient array
+  uint64_t ident_mask;     // selector value s is the identity when bit s set
+  uint16_t tpos[8];        // tile bit of target i
+  uint16_t tsorted[8];     // ascending copy of tpos (zero insertion)
+  uint16_t sel_pos[8];
+  uint8_t sel_tile[8];     // 1: tile bit, 0: base (local index) bit
+  uint32_t nterms, term_off;
+  uint32_t ndiag, diag_off;
+};
+
+struct PhaseTerm {
+  uint32_t tmask, tval;    // condition on the tile element index p
+  uint64_t bmask, bval;    // condition on the tile base (local index)
+  double angle;
+  double pad;
+};
+
+struct DiagTable {
+  uint32_t m;              // number of qubits
+  uint32_t coef_off;       // complex offset, 2^m entries
+  uint16_t pos[8];
+  uint8_t tile[8];
+};
+
+struct PassArgs {
+  // tile map
+  uint32_t t, c, nE, L;
+  uint32_t E[32];
+  uint32_t op_begin, op_end;
+  const Op* ops;
+  const PhaseTerm* terms;
+  const DiagTable* diags;
+  const double* coefs;     // interleaved complex
+  // compressed side
+  uint32_t B;              // block bits
+  uint32_t khi;            // L - B: group-local high bits (blocks per group = 2^khi)
+  uint64_t cpb;            // chunks per block
+  double eps_rel, eps_abs;
+  int guard;
+  int zero_mode;
+  double zero_thr;
+};
+
+// Per-launch buffers of a pass (passed by value).
+struct PassDyn {
+  double* raw;              // group slot g at raw + 2*(g << L) doubles
+  const uint32_t* bmap;     // [slot][2^khi] -> block id
+  const uint8_t* null_in;   // [nb] null flags at stage start
+  const uint64_t* desc_in;
+  const uint64_t* arena_in;
+  uint64_t* desc_out;
+  uint64_t* arena_out;
+  uint64_t sub_cap_words;   // capacity of each sub-arena (words)
+  unsigned long long* cursors;  // [kSubArenas] (stride 16 words)
+  uint32_t* overflow;
+  uint8_t* nonzero;         // [nb]: set when a block null at start holds data
+  uint64_t tile0;           // first tile of this launch
+};
+
+__host__ __device__ inline uint64_t desc_pack(uint64_t off_words, uint32_t w, int e) {
+  return (off_words & ((1ull << 40) - 1)) | ((uint64_t)(w & 0xff) << 40) |
+         ((uint64_t)(uint16_t)(int16_t)e << 48);
+}
+__host__ __device__ inline uint64_t desc_off(uint64_t d) { return d & ((1ull << 40) - 1); }
+__host__ __device__ inline uint32_t desc_w(uint64_t d) { return (uint32_t)((d >> 40) & 0xff); }
+__host__ __device__ inline int desc_e(uint64_t d) { return (int)(int16_t)(uint16_t)(d >> 48); }
+
+}  // namespace qsim

@@ -1,0 +1,661 @@
+// HIP kernels for gfx950 (CDNA4, wave64).
+//
+// pass_kernel<IN, OUT, TB>: one work-group owns one tile of 2^TB amplitudes of
+// a group-local state vector.  It loads the tile (raw complex128 from HBM, or
+// decodes codec chunks of the compressed block store), applies every op of the
+// pass in LDS, and stores it back (raw, or re-encoded into the store with the
+// all-zero check of blocks that were null at stage start).
+//
+// Codec (per 64-double chunk, one double per lane):
+//   e   = min over nonzero lanes of ilogb(2*fma(eps_rel,|x|,eps_abs)) - guard
+//   k   = rint(x * 2^-e)       |x - k*2^e| <= 2^(e-1) <= eps_abs + eps_rel*|x|
+//   u   = zig-zag(k), w = max bit length over the wave (wave64 DPP reduction)
+//   out = lane-major bit packing, chunk offsets by a work-group prefix sum and
+//         one atomic per tile on one of 64 sub-arena cursors.
+// Chunks whose codes would need more than 52 bits are stored verbatim (w=64).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+#include "qsim_internal.h"
+
+namespace qsim {
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ uint32_t ins0(uint32_t x, uint32_t pos) {
+  uint32_t lo = x & ((1u << pos) - 1u);
+  return lo | ((x >> pos) << (pos + 1));
+}
+__device__ __forceinline__ uint64_t ins0_64(uint64_t x, uint32_t pos) {
+  uint64_t lo = x & ((1ull << pos) - 1ull);
+  return lo | ((x >> pos) << (pos + 1));
+}
+
+struct TileMapDev {
+  uint32_t t, c, nE;
+  const uint32_t* E;
+  __device__ uint64_t base(uint64_t tau) const {
+    uint64_t x = tau << c;
+    for (uint32_t i = 0; i < nE; ++i) x = ins0_64(x, E[i]);
+    return x;
+  }
+  __device__ uint64_t off(uint32_t p) const {
+    uint64_t o = p & ((1u << c) - 1u);
+    for (uint32_t i = 0; i < nE; ++i) o |= (uint64_t)((p >> (c + i)) & 1u) << E[i];
+    return o;
+  }
+};
+
+// wave64 reductions through DPP (quad perms, row mirrors, row broadcasts);
+// result is read from lane 63.  All 64 lanes must be active.
+__device__ __forceinline__ int wave_min_i32(int v) {
+  const int id = 0x7fffffff;
+  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0xb1, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x4e, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xa, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+  const int id = (int)0x80000000;
+  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0xb1, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x4e, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xa, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+
+// ------------------------------------------------------------------ codec
+struct CodecParams {
+  double eps_rel, eps_abs;
+  int guard;
+};
+
+// Per-lane exponent candidate; INT_MAX for zero lanes, INT_MIN requests the
+// verbatim mode (non-finite value or a zero error budget).
+__device__ __forceinline__ int lane_exponent(double x, const CodecParams& cp) {
+  if (x == 0.0) return 0x7fffffff;
+  if (!isfinite(x)) return (int)0x80000000;
+  const double y = 2.0 * fma(cp.eps_rel, fabs(x), cp.eps_abs);
+  if (!(y > 0.0) || !isfinite(y)) return (int)0x80000000;
+  return __builtin_amdgcn_frexp_exp(y) - 1;  // == ilogb(y)
+}
+
+// Quantise one chunk (wave-uniform call).  Returns w (0, 1..52, or 64) and e.
+__device__ __forceinline__ void chunk_plan(double x, const CodecParams& cp, uint32_t& w, int& e) {
+  const uint64_t bal = ballot64(x != 0.0);
+  if (bal == 0) { w = 0; e = 0; return; }
+  const int emin = wave_min_i32(lane_exponent(x, cp));
+  if (emin == (int)0x80000000 || emin < -1000) { w = 64; e = 0; return; }
+  e = emin - cp.guard;
+  const double s = ldexp(x, -e);
+  const bool bad = !(fabs(s) < 2251799813685248.0);  // 2^51
+  if (ballot64(bad) != 0) { w = 64; e = 0; return; }
+  const double k = rint(s);
+  const double uz = (k >= 0.0) ? 2.0 * k : -2.0 * k - 1.0;
+  const uint64_t u = (uint64_t)uz;
+  const int bl = (u == 0) ? 0 : 64 - __clzll(u);
+  w = (uint32_t)wave_max_i32(bl);
+}
+
+__device__ __forceinline__ uint64_t chunk_code(double x, uint32_t w, int e) {
+  if (w == 64) return (uint64_t)__double_as_longlong(x);
+  if (w == 0) return 0;
+  const double k = rint(ldexp(x, -e));
+  const double uz = (k >= 0.0) ? 2.0 * k : -2.0 * k - 1.0;
+  return (uint64_t)uz;
+}
+
+__device__ __forceinline__ double decode_lane(uint64_t d, const uint64_t* __restrict__ arena, int lane) {
+  const uint32_t w = desc_w(d);
+  if (w == 0) return 0.0;
+  const uint64_t* P = arena + desc_off(d);
+  if (w == 64) return __longlong_as_double((long long)P[lane]);
+  const uint32_t bit = (uint32_t)lane * w;
+  const uint32_t q = bit >> 6, s = bit & 63u;
+  uint64_t v = P[q] >> s;
+  if (s + w > 64u) v |= P[q + 1] << (64u - s);
+  v &= (1ull << w) - 1ull;
+  const int64_t k = (int64_t)(v >> 1) ^ -(int64_t)(v & 1ull);
+  return ldexp((double)k, desc_e(d));
+}
+
+// ------------------------------------------------------------ pass kernel
+
+__device__ __forceinline__ void cmad(double& re, double& im, double mr, double mi, double vr, double vi) {
+  re = fma(mr, vr, re);
+  re = fma(-mi, vi, re);
+  im = fma(mr, vi, im);
+  im = fma(mi, vr, im);
+}
+
+template <int K, int TB>
+__device__ void dense_op(const Op& op, double2* tile, uint64_t base, const double* __restrict__ coefs) {
+  constexpr uint32_t NG = 1u << (TB - K);
+  constexpr int D = 1 << K;
+  uint32_t ts[K], tp[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) { ts[j] = op.tsorted[j]; tp[j] = op.tpos[j]; }
+  const uint32_t nsel = op.nsel;
+  const uint64_t ident = op.ident_mask;
+  // selector contribution of base bits (uniform over the tile)
+  uint32_t sel_base = 0, sel_tile_mask = 0;
+  for (uint32_t s = 0; s < nsel; ++s) {
+    if (op.sel_tile[s]) sel_tile_mask |= 1u << s;
+    else sel_base |= (uint32_t)((base >> op.sel_pos[s]) & 1ull) << s;
+  }
+  for (uint32_t gi = threadIdx.x; gi < NG; gi += kThreads) {
+    uint32_t i0 = gi;
+#pragma unroll
+    for (int j = 0; j < K; ++j) i0 = ins0(i0, ts[j]);
+    uint32_t sel = sel_base;
+    if (sel_tile_mask) {
+      for (uint32_t s = 0; s < nsel; ++s)
+        if ((sel_tile_mask >> s) & 1u) sel |= ((i0 >> op.sel_pos[s]) & 1u) << s;
+    }
+    if ((ident >> sel) & 1ull) continue;
+    const double* M = coefs + 2ull * (op.mat_off + (uint64_t)sel * (uint64_t)(D * D));
+    uint32_t idx[D];
+    double vr[D], vi[D];
+#pragma unroll
+    for (int cc = 0; cc < D; ++cc) {
+      uint32_t o = i0;
+#pragma unroll
+      for (int j = 0; j < K; ++j) if ((cc >> j) & 1) o |= 1u << tp[j];
+      idx[cc] = o;
+      const double2 v = tile[o];
+      vr[cc] = v.x; vi[cc] = v.y;
+    }
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      double re = 0.0, im = 0.0;
+#pragma unroll
+      for (int cc = 0; cc < D; ++cc) cmad(re, im, M[2 * (r * D + cc)], M[2 * (r * D + cc) + 1], vr[cc], vi[cc]);
+      tile[idx[r]] = make_double2(re, im);
+    }
+  }
+}
+
+template <int TB>
+__device__ void elem_op(const Op& op, double2* tile, uint64_t base, const PhaseTerm* __restrict__ terms,
+                        const DiagTable* __restrict__ diags, const double* __restrict__ coefs) {
+  constexpr uint32_t NT = 1u << TB;
+  constexpr int EPT = (NT + kThreads - 1) / kThreads;
+  double ang[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) ang[e] = 0.0;
+  double cang = 0.0;
+  bool any = false;
+  for (uint32_t i = 0; i < op.nterms; ++i) {
+    const PhaseTerm T = terms[op.term_off + i];
+    if ((base & T.bmask) != T.bval) continue;
+    any = true;
+    if (T.tmask == 0) { cang += T.angle; continue; }
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const uint32_t p = threadIdx.x + e * kThreads;
+      ang[e] += ((p & T.tmask) == T.tval) ? T.angle : 0.0;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const uint32_t p = threadIdx.x + e * kThreads;
+    if (NT < kThreads && p >= NT) break;
+    double2 v = tile[p];
+    if (any) {
+      double sn, cs;
+      sincos(ang[e] + cang, &sn, &cs);
+      v = make_double2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+    }
+    for (uint32_t d = 0; d < op.ndiag; ++d) {
+      const DiagTable& Dt = diags[op.diag_off + d];
+      uint32_t idx = 0;
+      for (uint32_t j = 0; j < Dt.m; ++j) {
+        const uint32_t bit = Dt.tile[j] ? ((p >> Dt.pos[j]) & 1u) : (uint32_t)((base >> Dt.pos[j]) & 1ull);
+        idx |= bit << j;
+      }
+      const double cr = coefs[2ull * (Dt.coef_off + idx)], ci = coefs[2ull * (Dt.coef_off + idx) + 1];
+      v = make_double2(v.x * cr - v.y * ci, v.x * ci + v.y * cr);
+    }
+    tile[p] = v;
+  }
+}
+
+template <int TB>
+__device__ void apply_ops(const PassArgs& a, double2* tile, uint64_t base) {
+  for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
+    const Op& op = a.ops[o];
+    if (op.kind == OP_DENSE) {
+      switch (op.k) {
+        case 1: if constexpr (TB >= 1) dense_op<1, TB>(op, tile, base, a.coefs); break;
+        case 2: if constexpr (TB >= 2) dense_op<2, TB>(op, tile, base, a.coefs); break;
+        case 3: if constexpr (TB >= 3) dense_op<3, TB>(op, tile, base, a.coefs); break;
+        case 4: if constexpr (TB >= 4) dense_op<4, TB>(op, tile, base, a.coefs); break;
+        case 5: if constexpr (TB >= 5) dense_op<5, TB>(op, tile, base, a.coefs); break;
+        default: break;
+      }
+    } else {
+      elem_op<TB>(op, tile, base, a.terms, a.diags, a.coefs);
+    }
+    __syncthreads();
+  }
+}
+
+// LDS layout beyond the tile (codec modes only)
+struct CodecScratch {
+  uint64_t desc[1u << (kMaxTileBits - kChunkAmpsLog2)];
+  uint32_t bid[1u << (kMaxTileBits - kChunkAmpsLog2)];
+  uint32_t ci[1u << (kMaxTileBits - kChunkAmpsLog2)];
+  uint32_t w[1u << (kMaxTileBits - kChunkAmpsLog2)];
+  int32_t e[1u << (kMaxTileBits - kChunkAmpsLog2)];
+  uint32_t pre[1u << (kMaxTileBits - kChunkAmpsLog2)];
+  uint32_t wave_tot[kThreads / 64];
+  unsigned long long tile_base;
+  uint32_t skip;
+  uint64_t stage[kThreads / 64][64];
+};
+
+template <int IN, int OUT, int TB>
+__global__ __launch_bounds__(kThreads) void pass_kernel(const PassArgs* __restrict__ pa, PassDyn dy) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* tile = reinterpret_cast<double2*>(smem);
+  constexpr uint32_t NT = 1u << TB;
+  const PassArgs& a = *pa;
+  const TileMapDev tm{a.t, a.c, a.nE, a.E};
+  const uint32_t lbits = a.L;
+  const uint64_t tau_all = dy.tile0 + blockIdx.x;
+  const uint64_t slot = tau_all >> (lbits - TB);
+  const uint64_t tau = tau_all & ((1ull << (lbits - TB)) - 1ull);
+  const uint64_t base = tm.base(tau);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  CodecScratch* cs = reinterpret_cast<CodecScratch*>(smem + sizeof(double2) * NT);
+  constexpr uint32_t NCH = (TB >= kChunkAmpsLog2) ? (NT >> kChunkAmpsLog2) : 1u;
+  const uint64_t bmask = (1ull << a.B) - 1ull;
+
+  // ---- chunk metadata (codec modes): one thread per chunk
+  if constexpr (IN == IN_DEC || OUT == OUT_ENC) {
+    for (uint32_t j = threadIdx.x; j < NCH; j += kThreads) {
+      const uint64_t loc = base | tm.off(j << kChunkAmpsLog2);
+      const uint32_t bid = dy.bmap[(slot << a.khi) + (loc >> a.B)];
+      const uint32_t ci = (uint32_t)((loc & bmask) >> kChunkAmpsLog2);
+      cs->bid[j] = bid;
+      cs->ci[j] = ci;
+      if constexpr (IN == IN_DEC) {
+        cs->desc[j] = dy.null_in[bid] ? 0ull : dy.desc_in[(uint64_t)bid * a.cpb + ci];
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- load
+  if constexpr (IN == IN_RAW) {
+    const double2* src = reinterpret_cast<const double2*>(dy.raw) + (slot << lbits) + base;
+#pragma unroll
+    for (uint32_t e = 0; e < (NT + kThreads - 1) / kThreads; ++e) {
+      const uint32_t p = threadIdx.x + e * kThreads;
+      if (NT >= kThreads || p < NT) tile[p] = src[tm.off(p)];
+    }
+  } else {
+    double* td = reinterpret_cast<double*>(tile);
+    constexpr int U = 4;
+    for (uint32_t j0 = wave; j0 < NCH; j0 += U * (kThreads / 64)) {
+      double xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t j = j0 + u * (kThreads / 64);
+        xv[u] = (j < NCH) ? decode_lane(cs->desc[j], dy.arena_in, lane) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t j = j0 + u * (kThreads / 64);
+        if (j < NCH) td[(j << (kChunkAmpsLog2 + 1)) + lane] = xv[u];
+      }
+    }
+  }
+  __syncthreads();
+
+  apply_ops<TB>(a, tile, base);
+
+  // ---- store
+  if constexpr (OUT == OUT_RAW) {
+    double2* dst = reinterpret_cast<double2*>(dy.raw) + (slot << lbits) + base;
+#pragma unroll
+    for (uint32_t e = 0; e < (NT + kThreads - 1) / kThreads; ++e) {
+      const uint32_t p = threadIdx.x + e * kThreads;
+      if (NT >= kThreads || p < NT) dst[tm.off(p)] = tile[p];
+    }
+  } else {
+    const double* td = reinterpret_cast<const double*>(tile);
+    const CodecParams cp{a.eps_rel, a.eps_abs, a.guard};
+    // phase 1: widths, exponents, zero check of blocks null at stage start
+    for (uint32_t j = wave; j < NCH; j += kThreads / 64) {
+      const double x = td[(j << (kChunkAmpsLog2 + 1)) + lane];
+      const uint32_t bid = cs->bid[j];
+      if (dy.null_in[bid]) {
+        const bool nz = (a.zero_mode == ZERO_EXACT) ? (x != 0.0) : (fabs(x) > a.zero_thr);
+        if (ballot64(nz) != 0 && lane == 0) dy.nonzero[bid] = 1;
+      }
+      uint32_t w; int e;
+      chunk_plan(x, cp, w, e);
+      if (lane == 0) { cs->w[j] = w; cs->e[j] = e; }
+    }
+    __syncthreads();
+    // exclusive prefix of chunk sizes (in 8-byte words)
+    {
+      uint32_t v = (threadIdx.x < NCH) ? cs->w[threadIdx.x] : 0u;
+      uint32_t incl = v;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+      }
+      if (lane == 63) cs->wave_tot[wave] = incl;
+      __syncthreads();
+      uint32_t woff = 0, tot = 0;
+#pragma unroll
+      for (int q = 0; q < kThreads / 64; ++q) {
+        if (q < wave) woff += cs->wave_tot[q];
+        tot += cs->wave_tot[q];
+      }
+      if (threadIdx.x < NCH) cs->pre[threadIdx.x] = woff + incl - v;
+      if (threadIdx.x == 0) {
+        const uint32_t sub = blockIdx.x & (kSubArenas - 1);
+        unsigned long long off = 0;
+        uint32_t skip = 0;
+        if (tot > 0) {
+          off = atomicAdd(&dy.cursors[sub * 16], (unsigned long long)tot);
+          if (off + tot > dy.sub_cap_words) { skip = 1; atomicOr(dy.overflow, 1u); }
+        }
+        cs->tile_base = off + (unsigned long long)sub * dy.sub_cap_words;
+        cs->skip = skip;
+      }
+      cs->stage[wave][lane] = 0ull;
+      __syncthreads();
+    }
+    // phase 2: pack and write
+    const unsigned long long tbase = cs->tile_base;
+    const uint32_t skip = cs->skip;
+    for (uint32_t j = wave; j < NCH; j += kThreads / 64) {
+      const uint32_t w = cs->w[j];
+      const int e = cs->e[j];
+      const uint64_t off = tbase + cs->pre[j];
+      if (lane == 0) dy.desc_out[(uint64_t)cs->bid[j] * a.cpb + cs->ci[j]] = desc_pack(off, w, e);
+      if (w == 0 || skip) continue;
+      const double x = td[(j << (kChunkAmpsLog2 + 1)) + lane];
+      const uint64_t u = chunk_code(x, w, e);
+      const uint32_t bit = (uint32_t)lane * w;
+      const uint32_t q = bit >> 6, s = bit & 63u;
+      atomicOr(&cs->stage[wave][q], (unsigned long long)(u << s));
+      if (s + w > 64u) atomicOr(&cs->stage[wave][q + 1], (unsigned long long)(u >> (64u - s)));
+      __builtin_amdgcn_wave_barrier();
+      if ((uint32_t)lane < w) {
+        dy.arena_out[off + lane] = cs->stage[wave][lane];
+        cs->stage[wave][lane] = 0ull;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
+// ------------------------------------------------------- state utilities
+__global__ void fill_basis_kernel(double2* v, uint64_t n, uint64_t idx) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = i; j < n; j += stride) v[j] = make_double2(j == idx ? 1.0 : 0.0, 0.0);
+}
+
+__global__ void fill_u8_kernel(uint8_t* p, uint64_t n, uint8_t val) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = i; j < n; j += stride) p[j] = val;
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, uint64_t n, unsigned long long val) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = i; j < n; j += stride) p[j] = val;
+}
+
+__global__ void scale_kernel(double2* v, uint64_t n, double s) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = i; j < n; j += stride) { double2 x = v[j]; v[j] = make_double2(x.x * s, x.y * s); }
+}
+
+// null <- null && !nonzero ; nonzero cleared
+__global__ void commit_null_kernel(uint8_t* null_flags, uint8_t* nonzero, uint64_t nb) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) {
+    if (nonzero[i]) null_flags[i] = 0;
+    nonzero[i] = 0;
+  }
+}
+
+// ------------------------------------------------ compensated reductions
+struct DD { double s, c; };
+__device__ __forceinline__ void dd_add(double& s, double& c, double x) {
+  const double t = s + x;
+  const double bp = t - s;
+  c += (s - (t - bp)) + (x - bp);
+  s = t;
+}
+__device__ __forceinline__ void dd_merge(double& s, double& c, double s2, double c2) {
+  dd_add(s, c, s2);
+  c += c2;
+}
+
+// Block reduce of NV compensated accumulators; writes partials[blockIdx][NV][2].
+template <int NV>
+__device__ void block_reduce_dd(double (&s)[NV], double (&c)[NV], double* partials) {
+  __shared__ double sh_s[NV][kThreads / 64];
+  __shared__ double sh_c[NV][kThreads / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    for (int d = 32; d >= 1; d >>= 1) {
+      const double s2 = __shfl_xor(s[v], d, 64);
+      const double c2 = __shfl_xor(c[v], d, 64);
+      dd_merge(s[v], c[v], s2, c2);
+    }
+    if (lane == 0) { sh_s[v][wave] = s[v]; sh_c[v][wave] = c[v]; }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double ss = sh_s[v][0], cc = sh_c[v][0];
+      for (int q = 1; q < kThreads / 64; ++q) dd_merge(ss, cc, sh_s[v][q], sh_c[v][q]);
+      partials[(blockIdx.x * NV + v) * 2] = ss;
+      partials[(blockIdx.x * NV + v) * 2 + 1] = cc;
+    }
+  }
+}
+
+// mode 0: sum |a|^2 ; mode 1: sum conj(a)*b (re, im)
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void sv_reduce_kernel(const double2* __restrict__ a, const double2* __restrict__ b,
+                                                             uint64_t n, double* partials) {
+  constexpr int NV = MODE == 0 ? 1 : 2;
+  double s[NV], c[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) { s[v] = 0.0; c[v] = 0.0; }
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    const double2 x = a[i];
+    if constexpr (MODE == 0) {
+      dd_add(s[0], c[0], fma(x.x, x.x, x.y * x.y));
+    } else {
+      const double2 y = b[i];
+      dd_add(s[0], c[0], fma(x.x, y.x, x.y * y.y));
+      dd_add(s[1], c[1], fma(x.x, y.y, -x.y * y.x));
+    }
+  }
+  block_reduce_dd<NV>(s, c, partials);
+}
+
+// <a| P |a> accumulation for one Pauli string: sum_i sign(i) conj(a[i^f]) a[i]
+// (the i^{nY} factor is applied by the host).
+__global__ __launch_bounds__(kThreads) void sv_pauli_kernel(const double2* __restrict__ a, uint64_t n, uint64_t fmask,
+                                                            uint64_t zmask, double* partials) {
+  double s[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    const double2 x = a[i];
+    const double2 y = a[i ^ fmask];
+    double re = fma(y.x, x.x, y.y * x.y);
+    double im = fma(y.x, x.y, -y.y * x.x);
+    if (__popcll(i & zmask) & 1) { re = -re; im = -im; }
+    dd_add(s[0], c[0], re);
+    dd_add(s[1], c[1], im);
+  }
+  block_reduce_dd<2>(s, c, partials);
+}
+
+// out[j] (+)= coef * P a[j]; P|i> = i^nY (-1)^{pc(i & zy)} |i ^ f>, so
+// (P a)[j] = i^nY (-1)^{pc((j^f) & zy)} a[j^f]; the complex factor coef*i^nY
+// is passed in (cr, ci).
+__global__ void sv_apply_pauli_kernel(const double2* __restrict__ a, double2* __restrict__ out, uint64_t n,
+                                      uint64_t fmask, uint64_t zmask, double cr, double ci, int accumulate) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    const uint64_t i = j ^ fmask;
+    double2 x = a[i];
+    double sg = (__popcll(i & zmask) & 1) ? -1.0 : 1.0;
+    const double re = sg * (cr * x.x - ci * x.y);
+    const double im = sg * (cr * x.y + ci * x.x);
+    if (accumulate) { double2 o = out[j]; out[j] = make_double2(o.x + re, o.y + im); }
+    else out[j] = make_double2(re, im);
+  }
+}
+
+// payload + descriptor bytes of every block (0 for null blocks)
+__global__ __launch_bounds__(256) void block_bytes_kernel(const uint64_t* __restrict__ desc, const uint8_t* __restrict__ nullf,
+                                                          uint64_t nb, uint64_t cpb, unsigned long long* out) {
+  __shared__ unsigned long long sh[256];
+  for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    unsigned long long s = 0;
+    if (!nullf[b])
+      for (uint64_t c = threadIdx.x; c < cpb; c += 256) s += desc_w(desc[b * cpb + c]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 128; d >= 1; d >>= 1) {
+      if (threadIdx.x < d) sh[threadIdx.x] += sh[threadIdx.x + d];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[b] = nullf[b] ? 0ull : 8ull * (sh[0] + cpb);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------- kernel launchers
+template <int IN, int OUT, int TB>
+static hipError_t launch_pass_tb(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
+  size_t lds = sizeof(double2) * (1u << TB);
+  if (IN == IN_DEC || OUT == OUT_ENC) lds += sizeof(CodecScratch);
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&pass_kernel<IN, OUT, TB>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr_set = true;
+  }
+  PassDyn d = dy;
+  const uint64_t maxg = 1ull << 30;
+  for (uint64_t t0 = 0; t0 < ntiles; t0 += maxg) {
+    const uint64_t cnt = (ntiles - t0 < maxg) ? (ntiles - t0) : maxg;
+    d.tile0 = dy.tile0 + t0;
+    hipLaunchKernelGGL((pass_kernel<IN, OUT, TB>), dim3((unsigned)cnt), dim3(kThreads), lds, st, d_args, d);
+  }
+  return hipGetLastError();
+}
+
+template <int IN, int OUT>
+static hipError_t launch_pass_mode(int tb, const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
+  switch (tb) {
+    case 1: if constexpr (IN == IN_RAW && OUT == OUT_RAW) return launch_pass_tb<IN, OUT, 1>(d_args, dy, ntiles, st); break;
+    case 2: if constexpr (IN == IN_RAW && OUT == OUT_RAW) return launch_pass_tb<IN, OUT, 2>(d_args, dy, ntiles, st); break;
+    case 3: if constexpr (IN == IN_RAW && OUT == OUT_RAW) return launch_pass_tb<IN, OUT, 3>(d_args, dy, ntiles, st); break;
+    case 4: if constexpr (IN == IN_RAW && OUT == OUT_RAW) return launch_pass_tb<IN, OUT, 4>(d_args, dy, ntiles, st); break;
+    case 5: return launch_pass_tb<IN, OUT, 5>(d_args, dy, ntiles, st);
+    case 6: return launch_pass_tb<IN, OUT, 6>(d_args, dy, ntiles, st);
+    case 7: return launch_pass_tb<IN, OUT, 7>(d_args, dy, ntiles, st);
+    case 8: return launch_pass_tb<IN, OUT, 8>(d_args, dy, ntiles, st);
+    case 9: return launch_pass_tb<IN, OUT, 9>(d_args, dy, ntiles, st);
+    case 10: return launch_pass_tb<IN, OUT, 10>(d_args, dy, ntiles, st);
+    case 11: return launch_pass_tb<IN, OUT, 11>(d_args, dy, ntiles, st);
+    case 12: return launch_pass_tb<IN, OUT, 12>(d_args, dy, ntiles, st);
+    case 13: return launch_pass_tb<IN, OUT, 13>(d_args, dy, ntiles, st);
+    default: break;
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_pass(int in_mode, int out_mode, int tb, const PassArgs* d_args, const PassDyn& dy,
+                       uint64_t ntiles, hipStream_t st) {
+  if (in_mode == IN_RAW && out_mode == OUT_RAW) return launch_pass_mode<IN_RAW, OUT_RAW>(tb, d_args, dy, ntiles, st);
+  if (in_mode == IN_DEC && out_mode == OUT_RAW) return launch_pass_mode<IN_DEC, OUT_RAW>(tb, d_args, dy, ntiles, st);
+  if (in_mode == IN_RAW && out_mode == OUT_ENC) return launch_pass_mode<IN_RAW, OUT_ENC>(tb, d_args, dy, ntiles, st);
+  return launch_pass_mode<IN_DEC, OUT_ENC>(tb, d_args, dy, ntiles, st);
+}
+
+static unsigned grid_for(uint64_t n, unsigned maxb = 65536u) {
+  uint64_t b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > maxb) b = maxb;
+  return (unsigned)b;
+}
+
+hipError_t launch_fill_basis(double2* v, uint64_t n, uint64_t idx, hipStream_t st) {
+  hipLaunchKernelGGL(fill_basis_kernel, dim3(grid_for(n)), dim3(256), 0, st, v, n, idx);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u8(uint8_t* p, uint64_t n, uint8_t val, hipStream_t st) {
+  hipLaunchKernelGGL(fill_u8_kernel, dim3(grid_for(n)), dim3(256), 0, st, p, n, val);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long val, hipStream_t st) {
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(n)), dim3(256), 0, st, p, n, val);
+  return hipGetLastError();
+}
+hipError_t launch_scale(double2* v, uint64_t n, double s, hipStream_t st) {
+  hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, st, v, n, s);
+  return hipGetLastError();
+}
+hipError_t launch_commit_null(uint8_t* nf, uint8_t* nz, uint64_t nb, hipStream_t st) {
+  hipLaunchKernelGGL(commit_null_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, nf, nz, nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_bytes(const uint64_t* desc, const uint8_t* nullf, uint64_t nb, uint64_t cpb,
+                              unsigned long long* out, hipStream_t st) {
+  hipLaunchKernelGGL(block_bytes_kernel, dim3(grid_for(nb, 4096u)), dim3(256), 0, st, desc, nullf, nb, cpb, out);
+  return hipGetLastError();
+}
+
+unsigned reduce_grid(uint64_t n) { return grid_for(n, 2048u); }
+
+hipError_t launch_sv_norm(const double2* a, uint64_t n, double* partials, unsigned grid, hipStream_t st) {
+  hipLaunchKernelGGL((sv_reduce_kernel<0>), dim3(grid), dim3(kThreads), 0, st, a, a, n, partials);
+  return hipGetLastError();
+}
+hipError_t launch_sv_inner(const double2* a, const double2* b, uint64_t n, double* partials, unsigned grid,
+                           hipStream_t st) {
+  hipLaunchKernelGGL((sv_reduce_kernel<1>), dim3(grid), dim3(kThreads), 0, st, a, b, n, partials);
+  return hipGetLastError();
+}
+hipError_t launch_sv_pauli(const double2* a, uint64_t n, uint64_t f, uint64_t z, double* partials, unsigned grid,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(sv_pauli_kernel, dim3(grid), dim3(kThreads), 0, st, a, n, f, z, partials);
+  return hipGetLastError();
+}
+hipError_t launch_sv_apply_pauli(const double2* a, double2* out, uint64_t n, uint64_t f, uint64_t z, double cr,
+                                 double ci, int acc, hipStream_t st) {
+  hipLaunchKernelGGL(sv_apply_pauli_kernel, dim3(grid_for(n)), dim3(256), 0, st, a, out, n, f, z, cr, ci, acc);
+  return hipGetLastError();
+}
+
+}  // namespace qsim

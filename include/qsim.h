@@ -1,0 +1,172 @@
+/*
+ * qsim.h — C ABI of libqsim_hip.so, the MI355X (gfx950) compressed
+ * state-vector stage loop.
+ *
+ * Every entry point returns 0 on success and a negative status on failure;
+ * qsim_last_error() then returns the message of the failing call on the
+ * calling thread.  Handles are opaque; the library owns every device
+ * allocation behind them.  Calls on one handle are not reentrant.  All
+ * amplitudes are complex128 stored as interleaved (re, im) doubles; qubit q is
+ * bit q of the amplitude index (Qulacs convention).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   QuantumCircuitOptimizer::optimize        src/cppsim/circuit_optimizer.cpp:113-168
+ *   QuantumState / update_quantum_state      src/cppsim/state.hpp:469-482,
+ *                                            src/cppsim/circuit.cpp:23-34
+ *   CompressedQuantumState                   src/cppsim/compressed_state.hpp:87-1707
+ *   pybind surface                           python/cppsim_wrapper.cpp:447-519
+ */
+#ifndef QSIM_H
+#define QSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSIM_MAX_GATE_QUBITS 8
+
+/* target commutation flags (Qulacs FLAG_X/Y/Z_COMMUTE) */
+#define QSIM_COMMUTE_X 1u
+#define QSIM_COMMUTE_Y 2u
+#define QSIM_COMMUTE_Z 4u
+
+/* gate flags */
+#define QSIM_GATE_PARAMETRIC 1u
+
+/*
+ * One gate in QuantumGateMatrix form (src/cppsim/gate_matrix.cpp:19-250):
+ * a 2^k x 2^k row-major complex matrix whose row/column index bit i belongs to
+ * targets[i], applied where every control qubit has its control value.  The
+ * matrix lives in a caller-owned pool of interleaved doubles at complex offset
+ * matrix_offset.  For parametric rotations (src/vqcsim/parametric_gate.hpp:
+ * 102-200) param_index names the parameter and pauli_ids[i] (1=X,2=Y,3=Z) the
+ * rotation axis on targets[i]; the matrix then holds exp(+i*theta/2*P).
+ */
+typedef struct qsim_gate {
+  uint32_t n_targets;
+  uint32_t n_controls;
+  uint32_t targets[QSIM_MAX_GATE_QUBITS];
+  uint32_t controls[QSIM_MAX_GATE_QUBITS];
+  uint32_t control_values[QSIM_MAX_GATE_QUBITS];
+  uint32_t target_commute[QSIM_MAX_GATE_QUBITS];
+  uint32_t flags;
+  int32_t param_index;
+  uint32_t pauli_ids[QSIM_MAX_GATE_QUBITS];
+  uint64_t matrix_offset;
+} qsim_gate;
+
+const char* qsim_last_error(void);
+int qsim_version(void);
+int qsim_device_count(int* count);
+
+/* ---- circuit optimizer ------------------------------------------------
+ * Greedy pairwise fusion into dense gates of at most max_block_size qubits
+ * (replaces QuantumCircuitOptimizer::optimize(circuit, block_size, 0),
+ * src/cppsim/circuit_optimizer.cpp:113-168, and gate::merge,
+ * src/cppsim/gate_merge.cpp:316-356).  Parametric gates are never merged.
+ * out must hold n_gates entries; pool_out must hold
+ * qsim_optimize_pool_bound(...) doubles.  Output matrix offsets index pool_out. */
+uint64_t qsim_optimize_pool_bound(uint64_t n_gates, uint64_t pool_len, uint32_t max_block_size);
+int qsim_optimize(const qsim_gate* gates, uint64_t n_gates, const double* pool, uint64_t pool_len,
+                  uint32_t max_block_size, qsim_gate* out, uint64_t* n_out, double* pool_out,
+                  uint64_t* pool_out_len);
+
+/* ---- uncompressed state in HBM ----------------------------------------
+ * QuantumState(n) (src/cppsim/state.hpp:469-482) with the gate sweep of
+ * QuantumCircuit::update_quantum_state (src/cppsim/circuit.cpp:23-34) fused
+ * into LDS-tiled passes. */
+typedef struct qsim_sv qsim_sv;
+int qsim_sv_create(uint32_t n_qubits, qsim_sv** out);
+int qsim_sv_destroy(qsim_sv* sv);
+int qsim_sv_set_basis(qsim_sv* sv, uint64_t index);
+int qsim_sv_load(qsim_sv* sv, const double* host_amplitudes);
+int qsim_sv_copy(qsim_sv* dst, const qsim_sv* src);
+int qsim_sv_read(const qsim_sv* sv, double* host_amplitudes);
+int qsim_sv_apply(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, const double* pool, uint64_t pool_len);
+int qsim_sv_norm_squared(const qsim_sv* sv, double* out);
+int qsim_sv_normalize(qsim_sv* sv, double squared_norm);
+int qsim_sv_scale(qsim_sv* sv, double factor);
+int qsim_sv_inner_product(const qsim_sv* bra, const qsim_sv* ket, double* re, double* im);
+/* sum_t coef_t <psi|P_t|psi>; P_t given by X-flip mask, Z-phase mask (Y sets
+ * both bits) and its Y count (src/cppsim/pauli_operator.cpp:132-183). */
+int qsim_sv_expectation_pauli(const qsim_sv* sv, uint64_t n_terms, const uint64_t* x_masks,
+                              const uint64_t* z_masks, const uint32_t* y_counts, const double* coef_re,
+                              const double* coef_im, double* out_re, double* out_im);
+/* out = sum_t coef_t P_t |psi> (GeneralQuantumOperator::apply_to_state) */
+int qsim_sv_apply_pauli_sum(const qsim_sv* sv, qsim_sv* out, uint64_t n_terms, const uint64_t* x_masks,
+                            const uint64_t* z_masks, const uint32_t* y_counts, const double* coef_re,
+                            const double* coef_im);
+void* qsim_sv_device_ptr(qsim_sv* sv);
+uint32_t qsim_sv_qubit_count(const qsim_sv* sv);
+
+/* ---- compressed block store in HBM ------------------------------------
+ * CompressedQuantumState(n, block_size) (src/cppsim/compressed_state.hpp:
+ * 87-239): 2^(n-B) blocks of 2^B amplitudes, each either null (all zero) or
+ * held by the error-bounded codec with |x^ - x| <= eps_rel*|x| + eps_abs per
+ * double (contract of src/cppsim/compand_bitcomp_pipeline.cuh:14-62). */
+#define QSIM_ZERO_EXACT 0     /* GPU semantics, cuda_complex_helper.cuh:96-106 */
+#define QSIM_ZERO_THRESHOLD 1 /* CPU semantics, compressed_state.hpp:939-951 */
+
+typedef struct qsim_cs_config {
+  double eps_rel;          /* BITCOMP_REL_EB (must be > 0) */
+  double eps_abs;          /* BITCOMP_ABS_EB */
+  int32_t guard_bits;      /* extra quantisation bits below the bound */
+  int32_t zero_mode;       /* QSIM_ZERO_EXACT or QSIM_ZERO_THRESHOLD */
+  double zero_threshold;   /* ALL_ZERO_THRESHOLD for QSIM_ZERO_THRESHOLD */
+  uint64_t scratch_bytes;  /* HBM staging for multi-pass stages (0 = default) */
+  uint32_t tile_bits;      /* LDS tile log2 amplitudes (0 = default) */
+  uint32_t reserved;
+} qsim_cs_config;
+
+typedef struct qsim_cs qsim_cs;
+int qsim_cs_default_config(qsim_cs_config* cfg);
+int qsim_cs_create(uint32_t n_qubits, uint32_t block_size, const qsim_cs_config* cfg, qsim_cs** out);
+int qsim_cs_destroy(qsim_cs* cs);
+/* partition_and_register_circuit (compressed_state.hpp:673-726) */
+int qsim_cs_partition_and_register(qsim_cs* cs, const qsim_gate* gates, uint64_t n_gates, const double* pool,
+                                   uint64_t pool_len, uint32_t inner_size);
+/* register_circuit_stages (compressed_state.hpp:464-507): stage s holds gate
+ * ids stage_gate_ids[stage_offsets[s] .. stage_offsets[s+1]); block_to_group
+ * holds n_stages x 2^(n-B) group ids; num_groups[s] groups per stage. */
+int qsim_cs_register_stages(qsim_cs* cs, const qsim_gate* gates, uint64_t n_gates, const double* pool,
+                            uint64_t pool_len, uint32_t n_stages, const uint32_t* stage_offsets,
+                            const uint32_t* stage_gate_ids, const uint32_t* block_to_group,
+                            const uint32_t* num_groups);
+int qsim_cs_num_stages(const qsim_cs* cs, uint32_t* out);
+int qsim_cs_stage_info(const qsim_cs* cs, uint32_t stage, uint32_t* n_gates, uint32_t* gate_ids,
+                       uint32_t* num_groups, uint32_t* block_to_group, uint32_t* n_passes);
+/* apply_circuit (compressed_state.hpp:840-937) */
+int qsim_cs_apply_circuit(qsim_cs* cs);
+int qsim_cs_run_stages(qsim_cs* cs, uint32_t stage_begin, uint32_t stage_end);
+/* get_decompressed_state_vector (compressed_state.hpp:1683-1707) */
+int qsim_cs_read_vector(qsim_cs* cs, double* host_amplitudes, int normalise);
+int qsim_cs_read_block(qsim_cs* cs, uint64_t block, double* host_amplitudes);
+int qsim_cs_set_basis(qsim_cs* cs, uint64_t index);
+int qsim_cs_load(qsim_cs* cs, const double* host_amplitudes);
+int qsim_cs_null_bitmap(const qsim_cs* cs, uint8_t* out);
+int qsim_cs_block_bytes(qsim_cs* cs, uint64_t* out);
+/* Total compressed bytes after the initial state and after every stage
+ * (append_total_SV_bytes, compressed_state.hpp:374-437); history[0] is the
+ * initial state.  *n receives the history length. */
+int qsim_cs_bytes_history(const qsim_cs* cs, uint64_t* out, uint64_t cap, uint64_t* n);
+int qsim_cs_max_sv_bytes(const qsim_cs* cs, uint64_t* out);
+/* Per-stage measurement of the last run: for each executed stage
+ * {gpu_seconds, bytes_in, bytes_out, decompressed_amplitudes, n_passes}. */
+int qsim_cs_stage_stats(const qsim_cs* cs, double* out5, uint64_t cap, uint64_t* n);
+#define QSIM_TIMER_COUNT 10 /* decomp gate comp fused_swap allgather hmalloc dmalloc h2d d2h dfree */
+int qsim_cs_times(const qsim_cs* cs, double* out);
+int qsim_cs_norm_squared(qsim_cs* cs, double* out);
+int qsim_cs_inner_product_sv(qsim_cs* cs, const qsim_sv* sv, double* re, double* im);
+uint32_t qsim_cs_qubit_count(const qsim_cs* cs);
+uint32_t qsim_cs_block_size(const qsim_cs* cs);
+
+/* ---- synchronisation / timing helpers ---------------------------------- */
+int qsim_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSIM_H */
