@@ -1,0 +1,180 @@
+"""Benchmark: the compressed state-vector stage loop on one MI355X.
+
+Workload (default): 32-qubit QFT-with-swaps on a seeded RY*RZ product state,
+608 gates, complex128, block_size 20, inner_size 4, eps_rel 1e-3,
+eps_abs 1e-7, after the QuantumCircuitOptimizer(block_size=2) pre-pass.
+A step resets the compressed store to |0...0> and runs every stage
+(decode -> gates in LDS -> zero check -> encode) with the state resident in
+HBM.  value = gates of the specified circuit / step time.
+
+Also reported: achieved HBM GB/s of the fused stage kernels against 8 TB/s
+(bytes per stage = C_in + C_out + 32 * A * (passes - 1), A = decompressed
+amplitudes of the active groups, timed with HIP events on the library stream),
+the compression ratio, the fidelity against the uncompressed GPU run of the
+same circuit, and the CPU oracle's compressed stage loop timed on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "compression-based-state-vector-simulator_amd"))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(stats):
+    tot = 0.0
+    for s in stats:
+        tot += s["bytes_in"] + s["bytes_out"] + 32.0 * s["amplitudes"] * max(0.0, s["passes"] - 1)
+    return tot
+
+
+def cpu_baseline(n_cpu, B_cpu, inner, gates_full, stages_full):
+    """The oracle's compressed stage loop (OpenMP C++) on QFT-n_cpu with the
+    same high-qubit layout (n - B = 12), scaled to 32 qubits by 2^(32-n)."""
+    import numpy as np  # noqa: F401
+
+    import pyoracle as O
+    from qompsim import QuantumCircuitOptimizer
+    from qompsim import workloads as W
+
+    c = W.qft_on_product(n_cpu, seed=2)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    oc = O.OracleCompressedState(n_cpu, B_cpu, 1e-3, 1e-7, 1)
+    nst = oc.partition_and_register_circuit(c, inner)
+    t0 = time.perf_counter()
+    oc.apply_circuit()
+    dt = time.perf_counter() - t0
+    t32 = dt * 2.0 ** (32 - n_cpu) * (stages_full / max(1, nst))
+    return {
+        "value": gates_full / t32,
+        "unit": "gates/s",
+        "cores": O.lib.orc_num_threads(),
+        "kind": "port",
+        "sample": (f"oracle C++/OpenMP compressed stage loop, full QFT-{n_cpu} (block_size {B_cpu}, inner 4, "
+                   f"{nst} stages) in {dt:.2f} s on the host, scaled by 2^{32 - n_cpu} amplitudes and "
+                   f"{stages_full}/{nst} stages to the 32-qubit run"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=32)
+    ap.add_argument("--block-size", type=int, default=None)
+    ap.add_argument("--inner-size", type=int, default=4)
+    ap.add_argument("--no-fidelity", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-qubits", type=int, default=25)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every block group lives on one "
+              "MI355X; the multi-GPU exchange is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from qompsim import CompressedQuantumState, QuantumCircuitOptimizer, QuantumState, _lib
+    from qompsim import workloads as W
+
+    n = args.qubits
+    B = args.block_size if args.block_size else n - 12
+    inner = args.inner_size
+    circ = W.qft_on_product(n, seed=2)
+    gates_spec = circ.get_gate_count()
+    t0 = time.perf_counter()
+    QuantumCircuitOptimizer().optimize(circ, 2)
+    t_opt = time.perf_counter() - t0
+    log(f"QFT-{n}: {gates_spec} gates -> {circ.get_gate_count()} after optimize ({t_opt:.2f} s)")
+
+    cs = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, verbose=False)
+    t0 = time.perf_counter()
+    cs.partition_and_register_circuit(circ, inner)
+    n_stages = cs.get_num_stages()
+    passes = cs.get_stage_passes()
+    log(f"{n_stages} stages, passes per stage {passes} ({time.perf_counter() - t0:.2f} s to plan)")
+
+    def step():
+        cs.set_zero_state()
+        cs.apply_circuit()
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        step()
+        log(f"warmup {i}: {time.perf_counter() - t0:.3f} s")
+    _lib.call("qsim_synchronize")
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        log(f"step {i} done")
+    _lib.call("qsim_synchronize")
+    dt = time.perf_counter() - t0
+    ms = dt / args.steps * 1e3
+    stats = cs.get_stage_stats()
+    gpu_s = sum(s["gpu_seconds"] for s in stats)
+    alg = algorithmic_bytes(stats)
+    achieved = alg / gpu_s / 1e9 if gpu_s > 0 else 0.0
+    hist = cs.get_bytes_history()
+    raw = 16.0 * (1 << n)
+    max_bytes = max(hist[-(n_stages + 1):])
+    survey_bytes = sum(s["bytes_in"] + s["bytes_out"] + 32.0 * s["amplitudes"] for s in stats)
+    log(f"{ms:.1f} ms/step, stage kernels {gpu_s * 1e3:.1f} ms, {achieved:.0f} GB/s, "
+        f"max compressed {max_bytes / 2**30:.2f} GiB ({raw / max_bytes:.2f}x)")
+
+    result = {
+        "metric": "gates/s & HBM GB/s (compressed, 32q) + adjoint grad/s vs plain, fidelity>=0.9999",
+        "value": gates_spec / (ms / 1e3),
+        "unit": "gates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {"workload": f"QFT-{n} compressed stage loop on a seeded RY*RZ product state (seed 2)",
+                   "qubits": n, "block_size": B, "inner_size": inner, "eps_rel": 1e-3, "eps_abs": 1e-7,
+                   "guard_bits": int(cs._cfg.guard_bits), "gates": gates_spec,
+                   "gates_after_optimize": circ.get_gate_count(), "stages": n_stages,
+                   "passes": int(sum(passes))},
+        "hbm_gbps": achieved,
+        "hbm_gbps_survey_model": survey_bytes / gpu_s / 1e9 if gpu_s > 0 else 0.0,
+        "stage_kernel_ms": gpu_s * 1e3,
+        "compression_ratio": raw / max_bytes,
+        "max_sv_GiB": max_bytes / 2**30,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
+    }
+
+    if not args.no_fidelity:
+        sv = QuantumState(n)
+        t0 = time.perf_counter()
+        circ.update_quantum_state(sv)
+        _lib.call("qsim_synchronize")
+        t_plain = time.perf_counter() - t0
+        result["fidelity"] = cs.fidelity_with(sv)
+        result["uncompressed_gates_per_s"] = gates_spec / t_plain
+        log(f"fidelity {result['fidelity']:.8f}; uncompressed run {t_plain * 1e3:.1f} ms")
+        del sv
+
+    if not args.no_cpu:
+        ncpu = min(args.cpu_qubits, n)
+        result["cpu_baseline"] = cpu_baseline(ncpu, ncpu - (n - B), inner, gates_spec, n_stages)
+        log("cpu baseline:", result["cpu_baseline"]["sample"])
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

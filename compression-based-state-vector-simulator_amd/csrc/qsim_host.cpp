@@ -97,6 +97,7 @@ struct HGate {
   std::vector<cplx> m;  // 2^k x 2^k row-major
   uint32_t flags = 0;
   int32_t param = -1;
+  int64_t origin = -1;  // input index for gates the optimizer left unmerged
   bool parametric() const { return flags & QSIM_GATE_PARAMETRIC; }
 };
 
@@ -114,6 +115,7 @@ static std::vector<HGate> parse_gates(const qsim_gate* g, uint64_t n, const doub
     h.cval.assign(q.control_values, q.control_values + q.n_controls);
     h.flags = q.flags;
     h.param = q.param_index;
+    h.origin = (int64_t)i;
     const uint64_t dim = 1ull << q.n_targets;
     if (q.matrix_offset + dim * dim > pool_len / 2)
       throw std::invalid_argument("gate " + std::to_string(i) + ": matrix outside the pool");
@@ -255,6 +257,7 @@ static HGate merge_gates(const HGate& g1, const HGate& g2) {
     }
   out.flags = g1.flags & g2.flags;
   out.param = -1;
+  out.origin = -1;
   return out;
 }
 
@@ -1070,7 +1073,8 @@ static void write_gates(const std::vector<HGate>& gs, qsim_gate* out, double* po
 }
 
 int qsim_optimize(const qsim_gate* gates, uint64_t n_gates, const double* pool, uint64_t pool_len,
-                  uint32_t max_block_size, qsim_gate* out, uint64_t* n_out, double* pool_out, uint64_t* pool_out_len) {
+                  uint32_t max_block_size, qsim_gate* out, uint64_t* n_out, double* pool_out, uint64_t* pool_out_len,
+                  int64_t* origin) {
   API_BEGIN
   auto gs = parse_gates(gates, n_gates, pool, pool_len);
   auto opt = optimize_circuit(std::move(gs), max_block_size);
@@ -1078,6 +1082,8 @@ int qsim_optimize(const qsim_gate* gates, uint64_t n_gates, const double* pool, 
   for (auto& g : opt) need += 2 * g.m.size();
   require(need <= *pool_out_len, "qsim_optimize: pool_out too small");
   write_gates(opt, out, pool_out, *pool_out_len);
+  if (origin)
+    for (size_t i = 0; i < opt.size(); ++i) origin[i] = opt[i].origin;
   *n_out = opt.size();
   *pool_out_len = need;
   API_END

@@ -68,11 +68,13 @@ int qsim_device_count(int* count);
  * src/cppsim/circuit_optimizer.cpp:113-168, and gate::merge,
  * src/cppsim/gate_merge.cpp:316-356).  Parametric gates are never merged.
  * out must hold n_gates entries; pool_out must hold
- * qsim_optimize_pool_bound(...) doubles.  Output matrix offsets index pool_out. */
+ * qsim_optimize_pool_bound(...) doubles.  Output matrix offsets index pool_out.
+ * origin (nullable, n_gates entries) receives the input index of every output
+ * gate that passed through unmerged, -1 for merged gates. */
 uint64_t qsim_optimize_pool_bound(uint64_t n_gates, uint64_t pool_len, uint32_t max_block_size);
 int qsim_optimize(const qsim_gate* gates, uint64_t n_gates, const double* pool, uint64_t pool_len,
                   uint32_t max_block_size, qsim_gate* out, uint64_t* n_out, double* pool_out,
-                  uint64_t* pool_out_len);
+                  uint64_t* pool_out_len, int64_t* origin);
 
 /* ---- uncompressed state in HBM ----------------------------------------
  * QuantumState(n) (src/cppsim/state.hpp:469-482) with the gate sweep of

@@ -1,0 +1,125 @@
+"""GPU parity of the compressed stage loop: codec bit-exact against the
+oracle's restatement, the contract bound on every double, the GHZ-20 stage
+and null-bitmap fixture bit-exact, and compressed runs against the oracle's
+stage loop and the exact state."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpu_util import rand_state, random_circuit
+from qompsim import CompressedQuantumState, QuantumCircuitOptimizer, QuantumState
+from qompsim import workloads as W
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.mark.parametrize("er,ea,guard", [(1e-3, 1e-7, 1), (1e-3, 1e-7, 0), (1e-2, 0.0, 0), (1e-6, 1e-12, 2)])
+def test_codec_bit_exact_vs_oracle(rng, er, ea, guard):
+    n, B = 14, 9
+    x = rng.normal(scale=10 ** rng.uniform(-6, 0, size=1 << n), size=1 << n) + 1j * rng.normal(scale=1e-4, size=1 << n)
+    x[: 1 << B] = 0  # first block exactly zero -> null
+    x[5000:5100] = 0
+    x[7000] = 1.0
+    cs = CompressedQuantumState(n, B, eps_rel=er, eps_abs=ea, guard_bits=guard, verbose=False)
+    cs.load(x)
+    got = cs.get_raw_vector()
+    ref, words, _ = O.codec_roundtrip(x.view(np.float64), er, ea, guard)
+    ref = ref.view(np.complex128)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)) or np.array_equal(got, ref)
+    xd = x.view(np.float64)
+    assert np.all(np.abs(got.view(np.float64) - xd) <= er * np.abs(xd) + ea)
+    null = cs.get_null_bitmap()
+    assert null[0] and not null[1:].any()
+    nonnull_chunks = (1 << (B - 5)) * int((~null).sum())
+    assert int(cs.get_block_bytes().sum()) == 8 * (words + nonnull_chunks)
+
+
+@pytest.mark.parametrize("mode", ["exact", "threshold"])
+def test_ghz20_fixture(mode):
+    fx = json.load(open(os.path.join(GOLD, "ghz20_b12_i4.json")))
+    c = W.ghz(20)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    cs = CompressedQuantumState(20, 12, eps_rel=1e-3, eps_abs=1e-7, zero_mode=mode, zero_threshold=1e-6,
+                                verbose=False)
+    cs.partition_and_register_circuit(c, 4, verbose=1)
+    assert [len(s) for s in cs.get_stages()] == fx["stage_sizes"]
+    groups = cs.get_block_groups()
+    assert [len(g) for g in groups] == fx["stage_num_groups"]
+    cs.apply_circuit()
+    assert cs.get_nonnull_blocks() == fx["final_nonnull_blocks"]
+    oc = O.OracleCompressedState(20, 12, 1e-3, 1e-7, 1, mode, 1e-6)
+    oc.partition_and_register_circuit(c, 4)
+    oc.apply_circuit()
+    assert np.array_equal(cs.get_null_bitmap(), oc.null_bitmap())
+    v = cs.get_vector()
+    for k, (re, im) in fx["final_amplitudes"].items():
+        assert abs(v[int(k)] - complex(re, im)) < 1e-6
+    assert np.max(np.abs(v - oc.get_vector())) < 1e-12
+    assert cs.get_bytes_history() == oc.bytes_history()
+
+
+@pytest.mark.parametrize("n,B,I,tb", [(16, 8, 4, 0), (16, 8, 2, 6), (18, 10, 4, 7), (14, 6, 3, 5)])
+def test_compressed_qft_vs_oracle(n, B, I, tb):
+    c = W.qft_on_product(n, seed=2)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    kw = dict(eps_rel=1e-3, eps_abs=1e-7, guard_bits=1, verbose=False)
+    if tb:
+        kw["tile_bits"] = tb
+    cs = CompressedQuantumState(n, B, **kw)
+    cs.partition_and_register_circuit(c, I)
+    cs.apply_circuit()
+    oc = O.OracleCompressedState(n, B, 1e-3, 1e-7, 1)
+    oc.partition_and_register_circuit(c, I)
+    assert cs.get_stages() == [s[0] for s in oc.stages()]
+    oc.apply_circuit()
+    exact = O.apply_circuit(O.zero_state(n), c.gate_list)
+    v = cs.get_vector()
+    assert abs(np.vdot(exact, v)) > 0.99999
+    # same algorithm, rounding differences only: within a few quantisation steps
+    assert np.max(np.abs(v - oc.get_vector())) < 1e-6
+    assert np.array_equal(cs.get_null_bitmap(), oc.null_bitmap())
+    s = QuantumState(n)
+    s.load(exact)
+    assert abs(cs.fidelity_with(s) - abs(np.vdot(exact, v))) < 1e-12
+    assert len(cs.get_stage_stats()) == cs.get_num_stages()
+
+
+def test_multipass_random_circuit(rng):
+    n, B = 13, 7
+    c = random_circuit(n, 120, rng, max_k=2)
+    cs = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-9, guard_bits=2, tile_bits=6, verbose=False)
+    cs.partition_and_register_circuit(c, 3)
+    assert max(cs.get_stage_passes()) > 1
+    cs.apply_circuit()
+    exact = O.apply_circuit(O.zero_state(n), c.gate_list)
+    assert abs(np.vdot(exact, cs.get_vector())) > 1 - 1e-9
+
+
+def test_register_circuit_stages_explicit(rng):
+    n, B = 12, 7
+    c = W.qft_on_product(n, seed=3)
+    cs = CompressedQuantumState(n, B, verbose=False)
+    cs.partition_and_register_circuit(c, 2)
+    stages = cs.get_stages()
+    groups = cs.get_block_groups()
+    cs2 = CompressedQuantumState(n, B, verbose=False)
+    cs2.register_circuit_stages(c, stages, groups)
+    cs.apply_circuit()
+    cs2.apply_circuit()
+    assert np.array_equal(cs.get_raw_vector(), cs2.get_raw_vector())
+    bad = [[[b] for b in range(1 << (n - B))] for _ in stages]  # one block per group splits gate pairs
+    cs3 = CompressedQuantumState(n, B, verbose=False)
+    with pytest.raises(IndexError):
+        cs3.register_circuit_stages(c, stages, bad)
+
+
+def test_basis_and_empty_stage():
+    cs = CompressedQuantumState(10, 6, verbose=False)
+    cs.set_computational_basis(777)
+    v = cs.get_raw_vector()
+    assert v[777] == 1.0 and np.count_nonzero(v) == 1
+    assert cs.get_nonnull_blocks() == [777 >> 6]

@@ -1,0 +1,104 @@
+"""GPU parity of the uncompressed LDS-pass path against the oracle
+(tolerance 1e-12 on complex128 amplitudes)."""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpu_util import rand_state, random_circuit
+from qompsim import Observable, QuantumState, inner_product
+from qompsim import workloads as W
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.fixture
+def tile_bits():
+    old = os.environ.get("QOMPSIM_TILE_BITS")
+    yield lambda v: os.environ.__setitem__("QOMPSIM_TILE_BITS", str(v))
+    if old is None:
+        os.environ.pop("QOMPSIM_TILE_BITS", None)
+    else:
+        os.environ["QOMPSIM_TILE_BITS"] = old
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 11, 14])
+def test_random_circuits_match_oracle(n, rng):
+    c = random_circuit(n, 60, rng, max_k=min(3, n))
+    psi = rand_state(n, rng)
+    s = QuantumState(n)
+    s.load(psi)
+    c.update_quantum_state(s)
+    ref = O.apply_circuit(psi, c.gate_list)
+    assert np.max(np.abs(s.get_vector() - ref)) < TOL
+
+
+@pytest.mark.parametrize("tb", [5, 7, 10, 12, 13])
+def test_tile_sizes(tb, tile_bits, rng):
+    tile_bits(tb)
+    n = 15
+    c = random_circuit(n, 80, rng, max_k=3)
+    psi = rand_state(n, rng)
+    s = QuantumState(n)
+    s.load(psi)
+    c.update_quantum_state(s)
+    assert np.max(np.abs(s.get_vector() - O.apply_circuit(psi, c.gate_list))) < TOL
+
+
+def test_five_qubit_dense_gate(rng):
+    from gpu_util import rand_unitary
+    from qompsim import QuantumCircuit, gate
+
+    n = 12
+    c = QuantumCircuit(n)
+    c.add_gate(gate.DenseMatrix([9, 0, 4, 11, 2], rand_unitary(32, rng)))
+    g = gate.DenseMatrix([3, 7, 1, 10], rand_unitary(16, rng))
+    g.add_control_qubit(5, 1)
+    c.add_gate(g)
+    psi = rand_state(n, rng)
+    s = QuantumState(n)
+    s.load(psi)
+    c.update_quantum_state(s)
+    assert np.max(np.abs(s.get_vector() - O.apply_circuit(psi, c.gate_list))) < TOL
+
+
+def test_qft20_is_dft(rng):
+    n = 20
+    psi = rand_state(n, rng)
+    s = QuantumState(n)
+    s.load(psi)
+    c = W.qft_on_product(n)
+    c.gate_list = W.qft_gates(n)
+    c.update_quantum_state(s)
+    assert np.max(np.abs(s.get_vector() - np.fft.ifft(psi, norm="ortho"))) < TOL
+
+
+def test_ghz_and_basis(rng):
+    s = QuantumState(16)
+    W.ghz(16).update_quantum_state(s)
+    v = s.get_vector()
+    assert abs(v[0] - 2 ** -0.5) < 1e-15 and abs(v[-1] - 2 ** -0.5) < 1e-15
+    s.set_computational_basis(12345)
+    v = s.get_vector()
+    assert v[12345] == 1 and np.count_nonzero(v) == 1
+
+
+def test_reductions_match_oracle(rng):
+    n = 16
+    a, b = rand_state(n, rng) * 1.3, rand_state(n, rng)
+    sa, sb = QuantumState(n), QuantumState(n)
+    sa.load(a)
+    sb.load(b)
+    assert abs(sa.get_squared_norm() - O.norm(a)) < 1e-13
+    assert abs(inner_product(sa, sb) - O.inner(a, b)) < 1e-13
+    obs = Observable(n)
+    obs.add_operator(0.3, "X 0 Y 9 Z 15")
+    obs.add_operator(-1.1, "Z 1 Z 3")
+    obs.add_operator(0.7, "X 12")
+    obs.add_operator(0.2, "")
+    assert abs(obs.get_expectation_value(sa) - O.expectation(a, obs).real) < 1e-13
+    out = QuantumState(n)
+    obs.apply_to_state(None, sa, out)
+    assert np.max(np.abs(out.get_vector() - O.apply_pauli_sum(a, obs))) < TOL
