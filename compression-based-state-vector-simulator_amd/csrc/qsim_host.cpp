@@ -23,8 +23,8 @@
 namespace qsim {
 
 
-hipError_t launch_pass(int in_mode, int out_mode, int tb, const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles,
-                       hipStream_t st);
+hipError_t launch_pass(int in_mode, int out_mode, int tb, bool big, const PassArgs* d_args, const PassDyn& dy,
+                       uint64_t ntiles, hipStream_t st);
 hipError_t launch_fill_basis(double2* v, uint64_t n, uint64_t idx, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, uint64_t n, uint8_t val, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long val, hipStream_t st);
@@ -521,6 +521,7 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
     op.mat_off = (uint32_t)(P.coefs.size() / 2);
     for (auto& v : g.mats) { P.coefs.push_back(v.real()); P.coefs.push_back(v.imag()); }
     P.ops.push_back(op);
+    a.maxk = std::max(a.maxk, op.k);
   }
   a.op_end = (uint32_t)P.ops.size();
   P.passes.push_back(a);
@@ -528,10 +529,13 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
 
 // Greedy split of a gate list into LDS passes: a pass takes gates in order
 // while the union of their non-diagonal targets fits into a tile of 2^t
-// amplitudes whose low c >= cmin qubits are contiguous.  Diagonal gates fit
-// every pass.
+// amplitudes whose low c >= cmin qubits are contiguous.  A gate that cannot
+// meet cmin on its own gets a pass with a shorter contiguous run.  Diagonal
+// gates fit every pass.  With codec_ends the first and last pass keep
+// c >= kChunkAmpsLog2 (codec chunks are 32 contiguous amplitudes); an
+// op-free pass is added at either end when needed.
 static void plan_program(Program& P, const std::vector<LGate>& lg, uint32_t L, uint32_t tmax, uint32_t cmin,
-                         bool at_least_one_pass) {
+                         bool at_least_one_pass, bool codec_ends = false) {
   P.L = L;
   const uint32_t t = std::min(tmax, L);
   cmin = std::min(cmin, t);
@@ -541,6 +545,7 @@ static void plan_program(Program& P, const std::vector<LGate>& lg, uint32_t L, u
   bool open = false;
   auto close = [&]() {
     int c = choose_c(R, t, cmin);
+    if (c < 0) c = choose_c(R, t, 0);
     if (c < 0) throw std::logic_error("planner: tile overflow");
     cur.c = (uint32_t)c;
     cur.E.clear();
@@ -554,14 +559,16 @@ static void plan_program(Program& P, const std::vector<LGate>& lg, uint32_t L, u
   for (size_t i = 0; i < lg.size(); ++i) {
     const LGate& g = lg[i];
     if (!g.diag) {
-      std::set<uint32_t> R2 = R;
-      for (auto q : g.N) R2.insert(q);
-      if (choose_c(R2, t, cmin) < 0) {
-        if (!open) throw std::invalid_argument("gate does not fit in one LDS tile");
+      std::set<uint32_t> R2 = R, Rg;
+      for (auto q : g.N) { R2.insert(q); Rg.insert(q); }
+      if (choose_c(Rg, t, 0) < 0) throw std::invalid_argument("gate does not fit in one LDS tile");
+      bool add;
+      if (choose_c(R2, t, cmin) >= 0) add = true;
+      else if (choose_c(Rg, t, cmin) >= 0) add = false;          // g alone keeps cmin: start a new pass
+      else add = choose_c(R2, t, 0) >= 0;                       // g needs a short run anyway
+      if (!add && open) {
         close();
-        R2.clear();
-        for (auto q : g.N) R2.insert(q);
-        if (choose_c(R2, t, cmin) < 0) throw std::invalid_argument("gate does not fit in one LDS tile");
+        R2 = Rg;
       }
       R = R2;
     }
@@ -569,6 +576,11 @@ static void plan_program(Program& P, const std::vector<LGate>& lg, uint32_t L, u
     open = true;
   }
   if (open || (at_least_one_pass && plans.empty())) close();
+  if (codec_ends) {
+    const uint32_t need = std::min<uint32_t>((uint32_t)kChunkAmpsLog2, t);
+    if (plans.front().c < need) plans.insert(plans.begin(), PassPlan{t, t, {}, {}});
+    if (plans.back().c < need) plans.push_back(PassPlan{t, t, {}, {}});
+  }
   for (auto& pp : plans) emit_pass(P, pp, lg);
 }
 
@@ -640,7 +652,7 @@ static void sv_apply_gates(qsim_sv* sv, const std::vector<HGate>& gs) {
     PassDyn dy{};
     dy.raw = reinterpret_cast<double*>(sv->d.p);
     const uint32_t t = P.passes[i].t;
-    HIPCHECK(launch_pass(IN_RAW, OUT_RAW, (int)t, P.dargs(i), dy, 1ull << (sv->n - t), stream()));
+    HIPCHECK(launch_pass(IN_RAW, OUT_RAW, (int)t, P.passes[i].maxk > 3, P.dargs(i), dy, 1ull << (sv->n - t), stream()));
   }
   HIPCHECK(hipStreamSynchronize(stream()));
 }
@@ -746,6 +758,13 @@ static void cs_fill_codec_fields(qsim_cs* cs, Program& P, uint32_t khi, int zero
     a.guard = cs->cfg.guard_bits;
     a.zero_mode = zero_mode;
     a.zero_thr = cs->cfg.zero_threshold;
+    if (cs->cfg.eps_abs > 0.0) {
+      a.e_abs = std::ilogb(2.0 * cs->cfg.eps_abs);
+      a.p_abs = std::ldexp(1.0, a.e_abs);
+    } else {
+      a.e_abs = INT32_MIN;
+      a.p_abs = 0.0;
+    }
   }
 }
 
@@ -771,7 +790,7 @@ static void cs_encode_scratch(qsim_cs* cs, int out, const std::vector<uint32_t>&
   dy.raw = cs->scratch.as<double>();
   dy.bmap = cs->bmap.as<uint32_t>();
   const uint32_t t = cs->io.passes[0].t;
-  HIPCHECK(launch_pass(IN_RAW, OUT_ENC, (int)t, cs->io.dargs(0), dy, (uint64_t)ids.size() << (cs->B - t), stream()));
+  HIPCHECK(launch_pass(IN_RAW, OUT_ENC, (int)t, false, cs->io.dargs(0), dy, (uint64_t)ids.size() << (cs->B - t), stream()));
 }
 
 static void cs_decode_to_scratch(qsim_cs* cs, const std::vector<uint32_t>& ids) {
@@ -780,7 +799,7 @@ static void cs_decode_to_scratch(qsim_cs* cs, const std::vector<uint32_t>& ids) 
   dy.raw = cs->scratch.as<double>();
   dy.bmap = cs->bmap.as<uint32_t>();
   const uint32_t t = cs->io.passes[0].t;
-  HIPCHECK(launch_pass(IN_DEC, OUT_RAW, (int)t, cs->io.dargs(0), dy, (uint64_t)ids.size() << (cs->B - t), stream()));
+  HIPCHECK(launch_pass(IN_DEC, OUT_RAW, (int)t, false, cs->io.dargs(0), dy, (uint64_t)ids.size() << (cs->B - t), stream()));
 }
 
 static uint64_t cs_blocks_per_batch(qsim_cs* cs) {
@@ -853,7 +872,7 @@ static void build_stage_program(qsim_cs* cs, CStage& S) {
   for (auto gid : S.gate_ids) lg.push_back(lower_gate(cs->gates[gid], qmap));
   S.prog.reset(new Program());
   const uint32_t tb = cs->cfg.tile_bits ? cs->cfg.tile_bits : default_tile_bits();
-  plan_program(*S.prog, lg, L, std::min<uint32_t>(tb, L), kChunkAmpsLog2, true);
+  plan_program(*S.prog, lg, L, std::min<uint32_t>(tb, L), kChunkAmpsLog2, true, true);
   cs_fill_codec_fields(cs, *S.prog, k, cs->cfg.zero_mode);
   S.prog->upload_all();
   const uint64_t need = (1ull << L);
@@ -933,7 +952,7 @@ static void cs_run_stage(qsim_cs* cs, uint32_t si) {
     if (P.passes.size() == 1) {
       dy.bmap = cs->bmap.as<uint32_t>();
       const uint32_t t = P.passes[0].t;
-      HIPCHECK(launch_pass(IN_DEC, OUT_ENC, (int)t, P.dargs(0), dy, nact << (L - t), stream()));
+      HIPCHECK(launch_pass(IN_DEC, OUT_ENC, (int)t, P.passes[0].maxk > 3, P.dargs(0), dy, nact << (L - t), stream()));
     } else {
       const uint64_t gpb = std::max<uint64_t>(1, cs->scratch_amps >> L);
       dy.raw = cs->scratch.as<double>();
@@ -944,7 +963,7 @@ static void cs_run_stage(qsim_cs* cs, uint32_t si) {
           const int in = p == 0 ? IN_DEC : IN_RAW;
           const int om = p + 1 == P.passes.size() ? OUT_ENC : OUT_RAW;
           const uint32_t t = P.passes[p].t;
-          HIPCHECK(launch_pass(in, om, (int)t, P.dargs(p), dy, cnt << (L - t), stream()));
+          HIPCHECK(launch_pass(in, om, (int)t, P.passes[p].maxk > 3, P.dargs(p), dy, cnt << (L - t), stream()));
         }
       }
     }

@@ -71,6 +71,8 @@ struct PassArgs {
   uint32_t t, c, nE, L;
   uint32_t E[32];
   uint32_t op_begin, op_end;
+  uint32_t maxk;           // largest dense op of the pass (5 selects the BIG kernel)
+  uint32_t pad0_;
   const Op* ops;
   const PhaseTerm* terms;
   const DiagTable* diags;
@@ -83,6 +85,9 @@ struct PassArgs {
   int guard;
   int zero_mode;
   double zero_thr;
+  int e_abs;               // ilogb(2*eps_abs), INT_MIN when eps_abs == 0
+  int pad_;
+  double p_abs;            // 2^e_abs
 };
 
 // Per-launch buffers of a pass (passed by value).

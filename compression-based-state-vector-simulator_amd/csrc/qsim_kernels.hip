@@ -74,6 +74,8 @@ __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 struct CodecParams {
   double eps_rel, eps_abs;
   int guard;
+  int e_abs;      // ilogb(2*eps_abs), or INT_MIN when eps_abs == 0
+  double p_abs;   // 2^e_abs
 };
 
 // Per-lane exponent candidate; INT_MAX for zero lanes, INT_MIN requests the
@@ -87,16 +89,30 @@ __device__ __forceinline__ int lane_exponent(double x, const CodecParams& cp) {
 }
 
 // Quantise one chunk (wave-uniform call).  Returns w (0, 1..52, or 64) and e.
+// e is the minimum over nonzero lanes of ilogb(2*fma(eps_rel,|x|,eps_abs))
+// minus the guard bits.  Every lane has y >= 2*eps_abs, so when one nonzero
+// lane has y < 2^(e_abs+1) the minimum is e_abs itself and the wave-wide
+// reduction is skipped (same result).
 __device__ __forceinline__ void chunk_plan(double x, const CodecParams& cp, uint32_t& w, int& e) {
-  const uint64_t bal = ballot64(x != 0.0);
-  if (bal == 0) { w = 0; e = 0; return; }
-  const int emin = wave_min_i32(lane_exponent(x, cp));
+  const bool nz = x != 0.0;
+  if (ballot64(nz) == 0) { w = 0; e = 0; return; }
+  int emin;
+  const bool small = nz && (fma(cp.eps_rel, fabs(x), cp.eps_abs) < cp.p_abs);
+  if (cp.e_abs != (int)0x80000000 && ballot64(small) != 0) emin = cp.e_abs;
+  else emin = wave_min_i32(lane_exponent(x, cp));
   if (emin == (int)0x80000000 || emin < -1000) { w = 64; e = 0; return; }
   e = emin - cp.guard;
   const double s = ldexp(x, -e);
-  const bool bad = !(fabs(s) < 2251799813685248.0);  // 2^51
+  const bool bad = !(fabs(s) < 2251799813685248.0);  // 2^51 (also NaN / inf)
   if (ballot64(bad) != 0) { w = 64; e = 0; return; }
   const double k = rint(s);
+  if (ballot64(!(fabs(k) < 1073741824.0)) == 0) {    // |k| < 2^30: 32-bit codes
+    const int ki = (int)k;
+    const uint32_t u = ((uint32_t)ki << 1) ^ (uint32_t)(ki >> 31);
+    const int bl = (u == 0) ? 0 : 32 - __clz((int)u);
+    w = (uint32_t)wave_max_i32(bl);
+    return;
+  }
   const double uz = (k >= 0.0) ? 2.0 * k : -2.0 * k - 1.0;
   const uint64_t u = (uint64_t)uz;
   const int bl = (u == 0) ? 0 : 64 - __clzll(u);
@@ -111,11 +127,28 @@ __device__ __forceinline__ uint64_t chunk_code(double x, uint32_t w, int e) {
   return (uint64_t)uz;
 }
 
+// zig-zag code of rint(x * 2^-e) for chunks whose codes fit 32 bits
+__device__ __forceinline__ uint32_t chunk_code32(double x, int e) {
+  const int ki = (int)rint(ldexp(x, -e));
+  return ((uint32_t)ki << 1) ^ (uint32_t)(ki >> 31);
+}
+
 __device__ __forceinline__ double decode_lane(uint64_t d, const uint64_t* __restrict__ arena, int lane) {
   const uint32_t w = desc_w(d);
   if (w == 0) return 0.0;
   const uint64_t* P = arena + desc_off(d);
   if (w == 64) return __longlong_as_double((long long)P[lane]);
+  if (w <= 32) {
+    const uint32_t* P32 = reinterpret_cast<const uint32_t*>(P);
+    const uint32_t bit = (uint32_t)lane * w;
+    const uint32_t q = bit >> 5, s = bit & 31u;
+    const uint32_t lo = P32[q];
+    const uint32_t hi = (s + w > 32u) ? P32[q + 1] : 0u;
+    uint32_t v = (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
+    if (w < 32) v &= (1u << w) - 1u;
+    const int k = (int)(v >> 1) ^ -(int)(v & 1u);
+    return ldexp((double)k, desc_e(d));
+  }
   const uint32_t bit = (uint32_t)lane * w;
   const uint32_t q = bit >> 6, s = bit & 63u;
   uint64_t v = P[q] >> s;
@@ -181,21 +214,56 @@ __device__ void dense_op(const Op& op, double2* tile, uint64_t base, const doubl
   }
 }
 
+// Phase table entries per tile: 64 for the low six tile bits, up to 128 for
+// the rest.
+constexpr int kPhaseTab = 192;
+
+// A run of diagonal gates.  Phase terms whose tile condition involves only
+// the low six tile bits (or none) are folded into tlo[64], those involving only
+// the high tile bits into thi[]; both tables are built once per tile (one
+// sincos per entry), so an amplitude costs two LDS reads and two complex
+// multiplies.  Terms spanning both halves are summed per amplitude.
 template <int TB>
 __device__ void elem_op(const Op& op, double2* tile, uint64_t base, const PhaseTerm* __restrict__ terms,
-                        const DiagTable* __restrict__ diags, const double* __restrict__ coefs) {
+                        const DiagTable* __restrict__ diags, const double* __restrict__ coefs, double2* ptab) {
   constexpr uint32_t NT = 1u << TB;
   constexpr int EPT = (NT + kThreads - 1) / kThreads;
+  constexpr uint32_t LOB = TB < 6 ? TB : 6;
+  constexpr uint32_t NLO = 1u << LOB, NHI = 1u << (TB - LOB);
+  constexpr uint32_t LOM = NLO - 1u;
+  const uint32_t nterms = op.nterms;
+  if (nterms) {
+    const uint32_t tid = threadIdx.x;
+    if (tid < NLO + NHI) {
+      const bool lo = tid < NLO;
+      const uint32_t idx = lo ? tid : tid - NLO;
+      double ang = 0.0;
+      for (uint32_t i = 0; i < nterms; ++i) {
+        const PhaseTerm& T = terms[op.term_off + i];
+        if ((base & T.bmask) != T.bval) continue;
+        const uint32_t tl = T.tmask & LOM, th = T.tmask >> LOB;
+        if (tl && th) continue;
+        if (lo) {
+          if (!th) ang += ((idx & tl) == (T.tval & LOM)) ? T.angle : 0.0;
+        } else {
+          if (th) ang += ((idx & th) == (T.tval >> LOB)) ? T.angle : 0.0;
+        }
+      }
+      double sn, cs;
+      sincos(ang, &sn, &cs);
+      ptab[lo ? idx : 64 + idx] = make_double2(cs, sn);
+    }
+    __syncthreads();
+  }
   double ang[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) ang[e] = 0.0;
-  double cang = 0.0;
-  bool any = false;
-  for (uint32_t i = 0; i < op.nterms; ++i) {
-    const PhaseTerm T = terms[op.term_off + i];
+  bool cross = false;
+  for (uint32_t i = 0; i < nterms; ++i) {
+    const PhaseTerm& T = terms[op.term_off + i];
+    if (!((T.tmask & LOM) && (T.tmask >> LOB))) continue;
     if ((base & T.bmask) != T.bval) continue;
-    any = true;
-    if (T.tmask == 0) { cang += T.angle; continue; }
+    cross = true;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const uint32_t p = threadIdx.x + e * kThreads;
@@ -207,9 +275,15 @@ __device__ void elem_op(const Op& op, double2* tile, uint64_t base, const PhaseT
     const uint32_t p = threadIdx.x + e * kThreads;
     if (NT < kThreads && p >= NT) break;
     double2 v = tile[p];
-    if (any) {
+    if (nterms) {
+      const double2 a = ptab[p & LOM];
+      const double2 b = ptab[64 + (p >> LOB)];
+      const double2 ab = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+      v = make_double2(v.x * ab.x - v.y * ab.y, v.x * ab.y + v.y * ab.x);
+    }
+    if (cross) {
       double sn, cs;
-      sincos(ang[e] + cang, &sn, &cs);
+      sincos(ang[e], &sn, &cs);
       v = make_double2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
     }
     for (uint32_t d = 0; d < op.ndiag; ++d) {
@@ -226,8 +300,10 @@ __device__ void elem_op(const Op& op, double2* tile, uint64_t base, const PhaseT
   }
 }
 
-template <int TB>
-__device__ void apply_ops(const PassArgs& a, double2* tile, uint64_t base) {
+// BIG instantiations carry the 4- and 5-target dense ops; the common kernel
+// stops at 3 targets so its register budget stays small.
+template <int TB, bool BIG>
+__device__ void apply_ops(const PassArgs& a, double2* tile, uint64_t base, double2* ptab) {
   for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
     const Op& op = a.ops[o];
     if (op.kind == OP_DENSE) {
@@ -235,12 +311,12 @@ __device__ void apply_ops(const PassArgs& a, double2* tile, uint64_t base) {
         case 1: if constexpr (TB >= 1) dense_op<1, TB>(op, tile, base, a.coefs); break;
         case 2: if constexpr (TB >= 2) dense_op<2, TB>(op, tile, base, a.coefs); break;
         case 3: if constexpr (TB >= 3) dense_op<3, TB>(op, tile, base, a.coefs); break;
-        case 4: if constexpr (TB >= 4) dense_op<4, TB>(op, tile, base, a.coefs); break;
-        case 5: if constexpr (TB >= 5) dense_op<5, TB>(op, tile, base, a.coefs); break;
+        case 4: if constexpr (BIG && TB >= 4) dense_op<4, TB>(op, tile, base, a.coefs); break;
+        case 5: if constexpr (BIG && TB >= 5) dense_op<5, TB>(op, tile, base, a.coefs); break;
         default: break;
       }
     } else {
-      elem_op<TB>(op, tile, base, a.terms, a.diags, a.coefs);
+      elem_op<TB>(op, tile, base, a.terms, a.diags, a.coefs, ptab);
     }
     __syncthreads();
   }
@@ -260,7 +336,7 @@ struct CodecScratch {
   uint64_t stage[kThreads / 64][64];
 };
 
-template <int IN, int OUT, int TB>
+template <int IN, int OUT, int TB, bool BIG>
 __global__ __launch_bounds__(kThreads) void pass_kernel(const PassArgs* __restrict__ pa, PassDyn dy) {
   extern __shared__ __align__(16) unsigned char smem[];
   double2* tile = reinterpret_cast<double2*>(smem);
@@ -273,7 +349,8 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(const PassArgs* __restri
   const uint64_t tau = tau_all & ((1ull << (lbits - TB)) - 1ull);
   const uint64_t base = tm.base(tau);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  CodecScratch* cs = reinterpret_cast<CodecScratch*>(smem + sizeof(double2) * NT);
+  double2* ptab = reinterpret_cast<double2*>(smem + sizeof(double2) * NT);
+  CodecScratch* cs = reinterpret_cast<CodecScratch*>(smem + sizeof(double2) * (NT + kPhaseTab));
   constexpr uint32_t NCH = (TB >= kChunkAmpsLog2) ? (NT >> kChunkAmpsLog2) : 1u;
   const uint64_t bmask = (1ull << a.B) - 1ull;
 
@@ -319,7 +396,7 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(const PassArgs* __restri
   }
   __syncthreads();
 
-  apply_ops<TB>(a, tile, base);
+  apply_ops<TB, BIG>(a, tile, base, ptab);
 
   // ---- store
   if constexpr (OUT == OUT_RAW) {
@@ -331,7 +408,7 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(const PassArgs* __restri
     }
   } else {
     const double* td = reinterpret_cast<const double*>(tile);
-    const CodecParams cp{a.eps_rel, a.eps_abs, a.guard};
+    const CodecParams cp{a.eps_rel, a.eps_abs, a.guard, a.e_abs, a.p_abs};
     // phase 1: widths, exponents, zero check of blocks null at stage start
     for (uint32_t j = wave; j < NCH; j += kThreads / 64) {
       const double x = td[(j << (kChunkAmpsLog2 + 1)) + lane];
@@ -387,6 +464,22 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(const PassArgs* __restri
       if (lane == 0) dy.desc_out[(uint64_t)cs->bid[j] * a.cpb + cs->ci[j]] = desc_pack(off, w, e);
       if (w == 0 || skip) continue;
       const double x = td[(j << (kChunkAmpsLog2 + 1)) + lane];
+      if (w <= 32) {
+        uint32_t* st32 = reinterpret_cast<uint32_t*>(cs->stage[wave]);
+        const uint32_t u = chunk_code32(x, e);
+        const uint32_t bit = (uint32_t)lane * w;
+        const uint32_t q = bit >> 5, s = bit & 31u;
+        atomicOr(&st32[q], u << s);
+        if (s + w > 32u) atomicOr(&st32[q + 1], u >> (32u - s));
+        __builtin_amdgcn_wave_barrier();
+        uint32_t* out32 = reinterpret_cast<uint32_t*>(dy.arena_out + off);
+        if ((uint32_t)lane < 2u * w) {
+          out32[lane] = st32[lane];
+          st32[lane] = 0u;
+        }
+        __builtin_amdgcn_wave_barrier();
+        continue;
+      }
       const uint64_t u = chunk_code(x, w, e);
       const uint32_t bit = (uint32_t)lane * w;
       const uint32_t q = bit >> 6, s = bit & 63u;
@@ -553,13 +646,13 @@ __global__ __launch_bounds__(256) void block_bytes_kernel(const uint64_t* __rest
 }
 
 // ------------------------------------------------------- kernel launchers
-template <int IN, int OUT, int TB>
+template <int IN, int OUT, int TB, bool BIG = false>
 static hipError_t launch_pass_tb(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
-  size_t lds = sizeof(double2) * (1u << TB);
+  size_t lds = sizeof(double2) * ((1u << TB) + kPhaseTab);
   if (IN == IN_DEC || OUT == OUT_ENC) lds += sizeof(CodecScratch);
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&pass_kernel<IN, OUT, TB>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&pass_kernel<IN, OUT, TB, BIG>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
@@ -568,38 +661,47 @@ static hipError_t launch_pass_tb(const PassArgs* d_args, const PassDyn& dy, uint
   for (uint64_t t0 = 0; t0 < ntiles; t0 += maxg) {
     const uint64_t cnt = (ntiles - t0 < maxg) ? (ntiles - t0) : maxg;
     d.tile0 = dy.tile0 + t0;
-    hipLaunchKernelGGL((pass_kernel<IN, OUT, TB>), dim3((unsigned)cnt), dim3(kThreads), lds, st, d_args, d);
+    hipLaunchKernelGGL((pass_kernel<IN, OUT, TB, BIG>), dim3((unsigned)cnt), dim3(kThreads), lds, st, d_args, d);
   }
   return hipGetLastError();
 }
 
-template <int IN, int OUT>
+template <int IN, int OUT, bool BIG>
 static hipError_t launch_pass_mode(int tb, const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
+  constexpr bool RR = IN == IN_RAW && OUT == OUT_RAW;
   switch (tb) {
-    case 1: if constexpr (IN == IN_RAW && OUT == OUT_RAW) return launch_pass_tb<IN, OUT, 1>(d_args, dy, ntiles, st); break;
-    case 2: if constexpr (IN == IN_RAW && OUT == OUT_RAW) return launch_pass_tb<IN, OUT, 2>(d_args, dy, ntiles, st); break;
-    case 3: if constexpr (IN == IN_RAW && OUT == OUT_RAW) return launch_pass_tb<IN, OUT, 3>(d_args, dy, ntiles, st); break;
-    case 4: if constexpr (IN == IN_RAW && OUT == OUT_RAW) return launch_pass_tb<IN, OUT, 4>(d_args, dy, ntiles, st); break;
-    case 5: return launch_pass_tb<IN, OUT, 5>(d_args, dy, ntiles, st);
-    case 6: return launch_pass_tb<IN, OUT, 6>(d_args, dy, ntiles, st);
-    case 7: return launch_pass_tb<IN, OUT, 7>(d_args, dy, ntiles, st);
-    case 8: return launch_pass_tb<IN, OUT, 8>(d_args, dy, ntiles, st);
-    case 9: return launch_pass_tb<IN, OUT, 9>(d_args, dy, ntiles, st);
-    case 10: return launch_pass_tb<IN, OUT, 10>(d_args, dy, ntiles, st);
-    case 11: return launch_pass_tb<IN, OUT, 11>(d_args, dy, ntiles, st);
-    case 12: return launch_pass_tb<IN, OUT, 12>(d_args, dy, ntiles, st);
-    case 13: return launch_pass_tb<IN, OUT, 13>(d_args, dy, ntiles, st);
+    case 1: if constexpr (RR && !BIG) return launch_pass_tb<IN, OUT, 1, BIG>(d_args, dy, ntiles, st); break;
+    case 2: if constexpr (RR && !BIG) return launch_pass_tb<IN, OUT, 2, BIG>(d_args, dy, ntiles, st); break;
+    case 3: if constexpr (RR && !BIG) return launch_pass_tb<IN, OUT, 3, BIG>(d_args, dy, ntiles, st); break;
+    case 4: if constexpr (RR) return launch_pass_tb<IN, OUT, 4, BIG>(d_args, dy, ntiles, st); break;
+    case 5: return launch_pass_tb<IN, OUT, 5, BIG>(d_args, dy, ntiles, st);
+    case 6: return launch_pass_tb<IN, OUT, 6, BIG>(d_args, dy, ntiles, st);
+    case 7: return launch_pass_tb<IN, OUT, 7, BIG>(d_args, dy, ntiles, st);
+    case 8: return launch_pass_tb<IN, OUT, 8, BIG>(d_args, dy, ntiles, st);
+    case 9: return launch_pass_tb<IN, OUT, 9, BIG>(d_args, dy, ntiles, st);
+    case 10: return launch_pass_tb<IN, OUT, 10, BIG>(d_args, dy, ntiles, st);
+    case 11: return launch_pass_tb<IN, OUT, 11, BIG>(d_args, dy, ntiles, st);
+    case 12: return launch_pass_tb<IN, OUT, 12, BIG>(d_args, dy, ntiles, st);
+    case 13: return launch_pass_tb<IN, OUT, 13, BIG>(d_args, dy, ntiles, st);
     default: break;
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_pass(int in_mode, int out_mode, int tb, const PassArgs* d_args, const PassDyn& dy,
+template <bool BIG>
+static hipError_t launch_pass_b(int in_mode, int out_mode, int tb, const PassArgs* d_args, const PassDyn& dy,
+                                uint64_t ntiles, hipStream_t st) {
+  if (in_mode == IN_RAW && out_mode == OUT_RAW) return launch_pass_mode<IN_RAW, OUT_RAW, BIG>(tb, d_args, dy, ntiles, st);
+  if (in_mode == IN_DEC && out_mode == OUT_RAW) return launch_pass_mode<IN_DEC, OUT_RAW, BIG>(tb, d_args, dy, ntiles, st);
+  if (in_mode == IN_RAW && out_mode == OUT_ENC) return launch_pass_mode<IN_RAW, OUT_ENC, BIG>(tb, d_args, dy, ntiles, st);
+  return launch_pass_mode<IN_DEC, OUT_ENC, BIG>(tb, d_args, dy, ntiles, st);
+}
+
+// big selects the instantiation that also carries 4- and 5-target dense ops
+hipError_t launch_pass(int in_mode, int out_mode, int tb, bool big, const PassArgs* d_args, const PassDyn& dy,
                        uint64_t ntiles, hipStream_t st) {
-  if (in_mode == IN_RAW && out_mode == OUT_RAW) return launch_pass_mode<IN_RAW, OUT_RAW>(tb, d_args, dy, ntiles, st);
-  if (in_mode == IN_DEC && out_mode == OUT_RAW) return launch_pass_mode<IN_DEC, OUT_RAW>(tb, d_args, dy, ntiles, st);
-  if (in_mode == IN_RAW && out_mode == OUT_ENC) return launch_pass_mode<IN_RAW, OUT_ENC>(tb, d_args, dy, ntiles, st);
-  return launch_pass_mode<IN_DEC, OUT_ENC>(tb, d_args, dy, ntiles, st);
+  return big ? launch_pass_b<true>(in_mode, out_mode, tb, d_args, dy, ntiles, st)
+             : launch_pass_b<false>(in_mode, out_mode, tb, d_args, dy, ntiles, st);
 }
 
 static unsigned grid_for(uint64_t n, unsigned maxb = 65536u) {
