@@ -91,12 +91,20 @@ def test_compressed_qft_vs_oracle(n, B, I, tb):
 def test_multipass_random_circuit(rng):
     n, B = 13, 7
     c = random_circuit(n, 120, rng, max_k=2)
-    cs = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-9, guard_bits=2, tile_bits=6, verbose=False)
+    # near-lossless codec settings: the relative budget would otherwise allow
+    # errors of eps_rel * |x| per double
+    cs = CompressedQuantumState(n, B, eps_rel=1e-12, eps_abs=1e-10, guard_bits=2, tile_bits=6, verbose=False)
     cs.partition_and_register_circuit(c, 3)
     assert max(cs.get_stage_passes()) > 1
     cs.apply_circuit()
     exact = O.apply_circuit(O.zero_state(n), c.gate_list)
-    assert abs(np.vdot(exact, cs.get_vector())) > 1 - 1e-9
+    v = cs.get_vector()
+    assert abs(np.vdot(exact, v)) > 1 - 1e-12
+    oc = O.OracleCompressedState(n, B, 1e-12, 1e-10, 2)
+    oc.partition_and_register_circuit(c, 3)
+    oc.apply_circuit()
+    assert np.max(np.abs(v - oc.get_vector())) < 1e-9
+    assert np.array_equal(cs.get_null_bitmap(), oc.null_bitmap())
 
 
 def test_register_circuit_stages_explicit(rng):
