@@ -16,7 +16,7 @@ INCLUDE = os.path.join(REPO, "include")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(PKG, "libqsim_hip.so")
 
-SOURCES = ["qsim_kernels.hip", "qsim_host.cpp"]
+SOURCES = ["qsim_kernels.hip", "qsim_wave.hip", "qsim_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 COMMON = [

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -25,6 +26,8 @@ namespace qsim {
 
 hipError_t launch_pass(int in_mode, int out_mode, int tb, bool big, const PassArgs* d_args, const PassDyn& dy,
                        uint64_t ntiles, hipStream_t st);
+hipError_t launch_wave(int in_mode, int out_mode, const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles,
+                       hipStream_t st);
 hipError_t launch_fill_basis(double2* v, uint64_t n, uint64_t idx, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, uint64_t n, uint8_t val, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long val, hipStream_t st);
@@ -38,6 +41,7 @@ hipError_t launch_sv_pauli(const double2* a, uint64_t n, uint64_t f, uint64_t z,
                            hipStream_t st);
 hipError_t launch_sv_apply_pauli(const double2* a, double2* out, uint64_t n, uint64_t f, uint64_t z, double cr,
                                  double ci, int acc, hipStream_t st);
+hipError_t launch_cursor_stats(const unsigned long long* cur, uint32_t n, unsigned long long* out, hipStream_t st);
 hipError_t launch_block_bytes(const uint64_t* desc, const uint8_t* nullf, uint64_t nb, uint64_t cpb,
                               unsigned long long* out, hipStream_t st);
 
@@ -534,54 +538,132 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
 // gates fit every pass.  With codec_ends the first and last pass keep
 // c >= kChunkAmpsLog2 (codec chunks are 32 contiguous amplitudes); an
 // op-free pass is added at either end when needed.
-static void plan_program(Program& P, const std::vector<LGate>& lg, uint32_t L, uint32_t tmax, uint32_t cmin,
-                         bool at_least_one_pass, bool codec_ends = false) {
+static void plan_program_engine(Program& P, const std::vector<LGate>& lg, uint32_t L, uint32_t tmax, uint32_t cmin,
+                                bool at_least_one_pass, bool codec_ends, bool wave) {
   P.L = L;
-  const uint32_t t = std::min(tmax, L);
+  const uint32_t t = wave ? kWaveTile : std::min(tmax, L);
+  if (wave) cmin = 6;
   cmin = std::min(cmin, t);
+  // Commutation-aware greedy scheduling: a pass scans the remaining gates in
+  // order; a gate joins when it commutes with every gate deferred before it
+  // (no shared qubit on which either acts non-diagonally) and its
+  // non-diagonal targets fit the tile; otherwise it is deferred to a later
+  // pass.  Applied gates keep their relative order.
   std::vector<PassPlan> plans;
-  std::set<uint32_t> R;
-  PassPlan cur{t, t, {}, {}};
-  bool open = false;
-  auto close = [&]() {
+  auto qmask = [](const std::vector<uint32_t>& v) {
+    uint64_t m = 0;
+    for (auto q : v) m |= 1ull << q;
+    return m;
+  };
+  std::vector<uint64_t> ndm(lg.size()), allm(lg.size());
+  for (size_t i = 0; i < lg.size(); ++i) {
+    const LGate& g = lg[i];
+    ndm[i] = g.diag ? 0ull : qmask(g.N);
+    allm[i] = g.diag ? qmask(g.Q) : (qmask(g.N) | qmask(g.S));
+    if (!g.diag) {
+      std::set<uint32_t> Rg(g.N.begin(), g.N.end());
+      if (choose_c(Rg, t, 0) < 0) throw std::invalid_argument("gate does not fit in one LDS tile");
+    }
+  }
+  std::vector<size_t> remaining(lg.size());
+  std::iota(remaining.begin(), remaining.end(), 0);
+  while (!remaining.empty()) {
+    std::set<uint32_t> R;
+    PassPlan cur{t, t, {}, {}};
+    std::vector<size_t> deferred;
+    uint64_t d_all = 0, d_nd = 0;
+    for (size_t i : remaining) {
+      const LGate& g = lg[i];
+      bool take = false;
+      if (!((ndm[i] & d_all) || (allm[i] & d_nd))) {
+        if (g.diag) {
+          take = true;
+        } else {
+          std::set<uint32_t> R2 = R, Rg(g.N.begin(), g.N.end());
+          R2.insert(g.N.begin(), g.N.end());
+          if (choose_c(R2, t, cmin) >= 0) take = true;
+          else if (choose_c(Rg, t, cmin) < 0 && choose_c(R2, t, 0) >= 0) take = true;  // needs a short run anyway
+          if (take) R = R2;
+        }
+      }
+      if (take) {
+        cur.gates.push_back(i);
+      } else {
+        deferred.push_back(i);
+        d_all |= allm[i];
+        d_nd |= ndm[i];
+      }
+    }
+    if (cur.gates.empty()) throw std::logic_error("planner: no progress");
     int c = choose_c(R, t, cmin);
     if (c < 0) c = choose_c(R, t, 0);
     if (c < 0) throw std::logic_error("planner: tile overflow");
     cur.c = (uint32_t)c;
-    cur.E.clear();
     for (auto q : R)
       if (q >= (uint32_t)c) cur.E.push_back(q);
     plans.push_back(cur);
-    cur = PassPlan{t, t, {}, {}};
-    R.clear();
-    open = false;
-  };
-  for (size_t i = 0; i < lg.size(); ++i) {
-    const LGate& g = lg[i];
-    if (!g.diag) {
-      std::set<uint32_t> R2 = R, Rg;
-      for (auto q : g.N) { R2.insert(q); Rg.insert(q); }
-      if (choose_c(Rg, t, 0) < 0) throw std::invalid_argument("gate does not fit in one LDS tile");
-      bool add;
-      if (choose_c(R2, t, cmin) >= 0) add = true;
-      else if (choose_c(Rg, t, cmin) >= 0) add = false;          // g alone keeps cmin: start a new pass
-      else add = choose_c(R2, t, 0) >= 0;                       // g needs a short run anyway
-      if (!add && open) {
-        close();
-        R2 = Rg;
-      }
-      R = R2;
-    }
-    cur.gates.push_back(i);
-    open = true;
+    remaining.swap(deferred);
   }
-  if (open || (at_least_one_pass && plans.empty())) close();
+  if (plans.empty() && at_least_one_pass) plans.push_back(PassPlan{t, t, {}, {}});
+  if (wave) {
+    // lane bits are local qubits 0..5; every other tile qubit is a register slot
+    for (auto& pp : plans) {
+      std::vector<uint32_t> E;
+      for (uint32_t q = 6; q < pp.c; ++q) E.push_back(q);
+      E.insert(E.end(), pp.E.begin(), pp.E.end());
+      std::sort(E.begin(), E.end());
+      if (E.size() != kWaveTile - 6) throw std::logic_error("planner: wave tile needs four register qubits");
+      pp.c = 6;
+      pp.E = E;
+    }
+  }
   if (codec_ends) {
     const uint32_t need = std::min<uint32_t>((uint32_t)kChunkAmpsLog2, t);
     if (plans.front().c < need) plans.insert(plans.begin(), PassPlan{t, t, {}, {}});
     if (plans.back().c < need) plans.push_back(PassPlan{t, t, {}, {}});
   }
-  for (auto& pp : plans) emit_pass(P, pp, lg);
+  for (auto& pp : plans) {
+    emit_pass(P, pp, lg);
+    P.passes.back().engine = wave ? ENGINE_WAVE : ENGINE_LDS;
+  }
+}
+
+// The wave-resident engine takes programs on at least 10 local qubits whose
+// gates have at most two non-diagonal targets; the LDS-tile engine takes the
+// rest.  QOMPSIM_ENGINE=lds forces the LDS-tile engine.
+static bool wave_engine_ok(const std::vector<LGate>& lg, uint32_t L) {
+  const char* e = std::getenv("QOMPSIM_ENGINE");
+  if (e && std::string(e) == "lds") return false;
+  if (L < kWaveTile) return false;
+  for (auto& g : lg)
+    if (!g.diag && g.N.size() > 2) return false;
+  return true;
+}
+
+static void plan_program(Program& P, const std::vector<LGate>& lg, uint32_t L, uint32_t tmax, uint32_t cmin,
+                         bool at_least_one_pass, bool codec_ends = false) {
+  plan_program_engine(P, lg, L, tmax, cmin, at_least_one_pass, codec_ends, wave_engine_ok(lg, L));
+  if (std::getenv("QOMPSIM_PLAN_LOG")) {
+    fprintf(stderr, "[plan] L=%u gates=%zu passes=%zu\n", L, lg.size(), P.passes.size());
+    for (auto& a : P.passes) {
+      uint32_t nd = 0, ne = 0, nt = 0;
+      for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
+        if (P.ops[o].kind == OP_DENSE) ++nd;
+        else { ++ne; nt += P.ops[o].nterms; }
+      }
+      fprintf(stderr, "[plan]   engine=%s t=%u c=%u E=", a.engine == ENGINE_WAVE ? "wave" : "lds", a.t, a.c);
+      for (uint32_t i = 0; i < a.nE; ++i) fprintf(stderr, "%u%s", a.E[i], i + 1 < a.nE ? "," : "");
+      fprintf(stderr, " dense=%u elem=%u terms=%u maxk=%u\n", nd, ne, nt, a.maxk);
+    }
+  }
+}
+
+// Launch pass i of a program over `groups` group slots of 2^L amplitudes.
+static hipError_t launch_program_pass(const Program& P, size_t i, int in_mode, int out_mode, const PassDyn& dy,
+                                      uint64_t groups, hipStream_t st) {
+  const PassArgs& a = P.passes[i];
+  if (a.engine == ENGINE_WAVE) return launch_wave(in_mode, out_mode, P.dargs(i), dy, groups << (P.L - kWaveTile), st);
+  return launch_pass(in_mode, out_mode, (int)a.t, a.maxk > 3, P.dargs(i), dy, groups << (P.L - a.t), st);
 }
 
 // --------------------------------------------------------- reductions (DD)
@@ -651,8 +733,7 @@ static void sv_apply_gates(qsim_sv* sv, const std::vector<HGate>& gs) {
   for (size_t i = 0; i < P.passes.size(); ++i) {
     PassDyn dy{};
     dy.raw = reinterpret_cast<double*>(sv->d.p);
-    const uint32_t t = P.passes[i].t;
-    HIPCHECK(launch_pass(IN_RAW, OUT_RAW, (int)t, P.passes[i].maxk > 3, P.dargs(i), dy, 1ull << (sv->n - t), stream()));
+    HIPCHECK(launch_program_pass(P, i, IN_RAW, OUT_RAW, dy, 1, stream()));
   }
   HIPCHECK(hipStreamSynchronize(stream()));
 }
@@ -675,8 +756,9 @@ struct qsim_cs {
   qsim_cs_config cfg{};
   std::vector<HGate> gates;
   std::vector<CStage> stages;
-  DBuf desc[2], arena[2], cursors[2];
+  DBuf desc[2], arena[2], cursors[2], cstats;
   uint64_t cap_words[2] = {0, 0};
+  uint32_t nsub[2] = {kSubArenas, kSubArenas};
   DBuf null_flags, nonzero, overflow, bmap, scratch, bytes_tmp;
   uint64_t scratch_amps = 0;
   int cur = 0;
@@ -697,27 +779,35 @@ static uint64_t words_for_cap(uint64_t bytes) {
   return w;
 }
 
+// The decoder reads whole dwords around a code field and may touch up to 8
+// bytes past a chunk's payload, so every arena keeps 64 bytes of padding.
 static void cs_alloc_arena(qsim_cs* cs, int which, uint64_t words) {
-  cs->arena[which].alloc(words * 8);
+  cs->arena[which].alloc((words + 8) * 8);
   cs->cap_words[which] = words;
 }
 
-static void cs_reset_out(qsim_cs* cs, int out) {
-  HIPCHECK(launch_fill_u64(cs->cursors[out].as<unsigned long long>(), (uint64_t)kSubArenas * 16, 0ull, stream()));
+// Number of allocation cursors for an encode of `tiles` tiles: about 64
+// tiles per cursor keeps the device-scope atomics on one address few.
+static uint32_t pick_nsub(uint64_t tiles) {
+  uint32_t S = kSubArenas;
+  while (S < (uint32_t)kMaxSubArenas && (uint64_t)S * 64 < tiles) S <<= 1;
+  return S;
+}
+
+static void cs_reset_out(qsim_cs* cs, int out, uint32_t nsub) {
+  cs->nsub[out] = nsub;
+  HIPCHECK(launch_fill_u64(cs->cursors[out].as<unsigned long long>(), (uint64_t)nsub * 16, 0ull, stream()));
   HIPCHECK(launch_fill_u8(cs->overflow.as<uint8_t>(), sizeof(uint32_t), 0, stream()));
 }
 
 static uint64_t cs_read_cursor_words(qsim_cs* cs, int which, uint64_t* maxsub) {
-  std::vector<unsigned long long> h((size_t)kSubArenas * 16);
-  HIPCHECK(hipMemcpyAsync(h.data(), cs->cursors[which].p, h.size() * 8, hipMemcpyDeviceToHost, stream()));
+  unsigned long long h[2];
+  HIPCHECK(launch_cursor_stats(cs->cursors[which].as<unsigned long long>(), cs->nsub[which],
+                               cs->cstats.as<unsigned long long>(), stream()));
+  HIPCHECK(hipMemcpyAsync(h, cs->cstats.p, sizeof(h), hipMemcpyDeviceToHost, stream()));
   HIPCHECK(hipStreamSynchronize(stream()));
-  uint64_t tot = 0, mx = 0;
-  for (int s = 0; s < kSubArenas; ++s) {
-    tot += h[(size_t)s * 16];
-    mx = std::max<uint64_t>(mx, h[(size_t)s * 16]);
-  }
-  if (maxsub) *maxsub = mx;
-  return tot;
+  if (maxsub) *maxsub = h[1];
+  return h[0];
 }
 
 static bool cs_check_overflow(qsim_cs* cs, int out) {
@@ -727,7 +817,7 @@ static bool cs_check_overflow(qsim_cs* cs, int out) {
   if (!ov) return false;
   uint64_t mx = 0;
   cs_read_cursor_words(cs, out, &mx);
-  const uint64_t need = mx * kSubArenas;
+  const uint64_t need = mx * cs->nsub[out];
   uint64_t nw = std::max<uint64_t>(cs->cap_words[out] * 2, need + need / 4);
   nw = words_for_cap(nw * 8);
   cs_alloc_arena(cs, out, nw);
@@ -741,8 +831,9 @@ static PassDyn cs_dyn(qsim_cs* cs, int out) {
   dy.arena_in = cs->arena[cs->cur].as<uint64_t>();
   dy.desc_out = cs->desc[out].as<uint64_t>();
   dy.arena_out = cs->arena[out].as<uint64_t>();
-  dy.sub_cap_words = cs->cap_words[out] / kSubArenas;
+  dy.sub_cap_words = cs->cap_words[out] / cs->nsub[out];
   dy.cursors = cs->cursors[out].as<unsigned long long>();
+  dy.sub_mask = cs->nsub[out] - 1;
   dy.overflow = cs->overflow.as<uint32_t>();
   dy.nonzero = cs->nonzero.as<uint8_t>();
   return dy;
@@ -789,8 +880,7 @@ static void cs_encode_scratch(qsim_cs* cs, int out, const std::vector<uint32_t>&
   PassDyn dy = cs_dyn(cs, out);
   dy.raw = cs->scratch.as<double>();
   dy.bmap = cs->bmap.as<uint32_t>();
-  const uint32_t t = cs->io.passes[0].t;
-  HIPCHECK(launch_pass(IN_RAW, OUT_ENC, (int)t, false, cs->io.dargs(0), dy, (uint64_t)ids.size() << (cs->B - t), stream()));
+  HIPCHECK(launch_program_pass(cs->io, 0, IN_RAW, OUT_ENC, dy, ids.size(), stream()));
 }
 
 static void cs_decode_to_scratch(qsim_cs* cs, const std::vector<uint32_t>& ids) {
@@ -798,8 +888,7 @@ static void cs_decode_to_scratch(qsim_cs* cs, const std::vector<uint32_t>& ids) 
   PassDyn dy = cs_dyn(cs, 1 - cs->cur);
   dy.raw = cs->scratch.as<double>();
   dy.bmap = cs->bmap.as<uint32_t>();
-  const uint32_t t = cs->io.passes[0].t;
-  HIPCHECK(launch_pass(IN_DEC, OUT_RAW, (int)t, false, cs->io.dargs(0), dy, (uint64_t)ids.size() << (cs->B - t), stream()));
+  HIPCHECK(launch_program_pass(cs->io, 0, IN_DEC, OUT_RAW, dy, ids.size(), stream()));
 }
 
 static uint64_t cs_blocks_per_batch(qsim_cs* cs) {
@@ -812,7 +901,11 @@ static void cs_load(qsim_cs* cs, const double* host, uint64_t basis) {
   for (int attempt = 0;; ++attempt) {
     HIPCHECK(launch_fill_u8(cs->null_flags.as<uint8_t>(), cs->nb, 1, stream()));
     HIPCHECK(launch_fill_u8(cs->nonzero.as<uint8_t>(), cs->nb, 0, stream()));
-    cs_reset_out(cs, out);
+    {
+      const PassArgs& io = cs->io.passes[0];
+      const uint32_t tb = io.engine == ENGINE_WAVE ? kWaveTile : io.t;
+      cs_reset_out(cs, out, pick_nsub((host ? cs->nb : 1ull) << (cs->B - tb)));
+    }
     if (host) {
       const uint64_t bpb = cs_blocks_per_batch(cs);
       for (uint64_t b0 = 0; b0 < cs->nb; b0 += bpb) {
@@ -946,13 +1039,16 @@ static void cs_run_stage(qsim_cs* cs, uint32_t si) {
   float ms = 0.f;
   for (int attempt = 0;; ++attempt) {
     HIPCHECK(launch_fill_u8(cs->nonzero.as<uint8_t>(), cs->nb, 0, stream()));
-    cs_reset_out(cs, out);
+    {
+      const PassArgs& last = P.passes.back();
+      const uint32_t tb = last.engine == ENGINE_WAVE ? kWaveTile : last.t;
+      cs_reset_out(cs, out, pick_nsub(nact << (L - tb)));
+    }
     HIPCHECK(hipEventRecord(cs->ev0, stream()));
     PassDyn dy = cs_dyn(cs, out);
     if (P.passes.size() == 1) {
       dy.bmap = cs->bmap.as<uint32_t>();
-      const uint32_t t = P.passes[0].t;
-      HIPCHECK(launch_pass(IN_DEC, OUT_ENC, (int)t, P.passes[0].maxk > 3, P.dargs(0), dy, nact << (L - t), stream()));
+      HIPCHECK(launch_program_pass(P, 0, IN_DEC, OUT_ENC, dy, nact, stream()));
     } else {
       const uint64_t gpb = std::max<uint64_t>(1, cs->scratch_amps >> L);
       dy.raw = cs->scratch.as<double>();
@@ -962,8 +1058,7 @@ static void cs_run_stage(qsim_cs* cs, uint32_t si) {
         for (size_t p = 0; p < P.passes.size(); ++p) {
           const int in = p == 0 ? IN_DEC : IN_RAW;
           const int om = p + 1 == P.passes.size() ? OUT_ENC : OUT_RAW;
-          const uint32_t t = P.passes[p].t;
-          HIPCHECK(launch_pass(in, om, (int)t, P.passes[p].maxk > 3, P.dargs(p), dy, cnt << (L - t), stream()));
+          HIPCHECK(launch_program_pass(P, p, in, om, dy, cnt, stream()));
         }
       }
     }
@@ -1296,13 +1391,14 @@ int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** 
   const uint64_t raw = cs->dim * 16;
   for (int w = 0; w < 2; ++w) {
     cs->desc[w].alloc(cs->nb * cs->cpb * 8);
-    cs->cursors[w].alloc((size_t)kSubArenas * 16 * 8);
-    const uint64_t init = std::min<uint64_t>(std::max<uint64_t>(raw / 8, 8ull << 20), 16ull << 30);
+    cs->cursors[w].alloc((size_t)kMaxSubArenas * 16 * 8);
+    const uint64_t init = std::min<uint64_t>(std::max<uint64_t>(raw / 6, 8ull << 20), 24ull << 30);
     cs_alloc_arena(cs.get(), w, words_for_cap(init));
   }
   cs->null_flags.alloc(cs->nb);
   cs->nonzero.alloc(cs->nb);
   cs->overflow.alloc(16);
+  cs->cstats.alloc(16);
   cs->h_null.assign(cs->nb, 1);
   uint64_t sbytes = c.scratch_bytes;
   if (sbytes == 0) {

@@ -25,13 +25,16 @@ constexpr int kMaxSel = 6;         // selector qubits (diagonal targets + contro
 constexpr int kMaxTileBits = 13;   // 2^13 complex128 = 128 KiB of LDS
 constexpr int kThreads = 256;      // work-group size of the pass kernels
 constexpr int kChunkAmpsLog2 = 5;  // 32 amplitudes = 64 doubles per codec chunk
-constexpr int kSubArenas = 64;     // independent allocation cursors per arena
+constexpr int kSubArenas = 64;         // allocation granularity of arena capacities
+constexpr int kMaxSubArenas = 65536;   // allocation cursors per arena (the launch uses a power of two)
 constexpr int kMaxLocalBits = 48;
 
 enum OpKind : uint32_t { OP_DENSE = 0, OP_ELEM = 1 };
 enum InMode : int { IN_RAW = 0, IN_DEC = 1 };
 enum OutMode : int { OUT_RAW = 0, OUT_ENC = 1 };
 enum ZeroMode : int { ZERO_EXACT = 0, ZERO_THRESHOLD = 1 };
+enum Engine : uint32_t { ENGINE_LDS = 0, ENGINE_WAVE = 1 };
+constexpr uint32_t kWaveTile = 10;  // wave tile bits: 6 lane bits + 4 register slots
 
 // One op of a pass.  Dense: 2^nsel matrices of 2^k x 2^k (row-major, matrix
 // index bit i <-> tile bit tpos[i]); the matrix is selected by the selector
@@ -71,8 +74,8 @@ struct PassArgs {
   uint32_t t, c, nE, L;
   uint32_t E[32];
   uint32_t op_begin, op_end;
-  uint32_t maxk;           // largest dense op of the pass (5 selects the BIG kernel)
-  uint32_t pad0_;
+  uint32_t maxk;           // largest dense op of the pass (4 or 5 select the BIG kernel)
+  uint32_t engine;         // ENGINE_LDS: LDS tile kernel; ENGINE_WAVE: wave-resident kernel
   const Op* ops;
   const PhaseTerm* terms;
   const DiagTable* diags;
@@ -100,7 +103,9 @@ struct PassDyn {
   uint64_t* desc_out;
   uint64_t* arena_out;
   uint64_t sub_cap_words;   // capacity of each sub-arena (words)
-  unsigned long long* cursors;  // [kSubArenas] (stride 16 words)
+  unsigned long long* cursors;  // [sub_mask + 1] (stride 16 words)
+  uint32_t sub_mask;        // number of sub-arenas - 1 (power of two)
+  uint32_t pad_;
   uint32_t* overflow;
   uint8_t* nonzero;         // [nb]: set when a block null at start holds data
   uint64_t tile0;           // first tile of this launch

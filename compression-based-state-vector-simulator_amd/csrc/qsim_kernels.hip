@@ -17,155 +17,12 @@
 #include <stdint.h>
 #include <math.h>
 #include "qsim_internal.h"
+#include "qsim_device.h"
 
 namespace qsim {
 
-// ---------------------------------------------------------------- helpers
-__device__ __forceinline__ uint32_t ins0(uint32_t x, uint32_t pos) {
-  uint32_t lo = x & ((1u << pos) - 1u);
-  return lo | ((x >> pos) << (pos + 1));
-}
-__device__ __forceinline__ uint64_t ins0_64(uint64_t x, uint32_t pos) {
-  uint64_t lo = x & ((1ull << pos) - 1ull);
-  return lo | ((x >> pos) << (pos + 1));
-}
-
-struct TileMapDev {
-  uint32_t t, c, nE;
-  const uint32_t* E;
-  __device__ uint64_t base(uint64_t tau) const {
-    uint64_t x = tau << c;
-    for (uint32_t i = 0; i < nE; ++i) x = ins0_64(x, E[i]);
-    return x;
-  }
-  __device__ uint64_t off(uint32_t p) const {
-    uint64_t o = p & ((1u << c) - 1u);
-    for (uint32_t i = 0; i < nE; ++i) o |= (uint64_t)((p >> (c + i)) & 1u) << E[i];
-    return o;
-  }
-};
-
-// wave64 reductions through DPP (quad perms, row mirrors, row broadcasts);
-// result is read from lane 63.  All 64 lanes must be active.
-__device__ __forceinline__ int wave_min_i32(int v) {
-  const int id = 0x7fffffff;
-  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0xb1, 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x4e, 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xa, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xc, 0xf, false));
-  return __builtin_amdgcn_readlane(v, 63);
-}
-__device__ __forceinline__ int wave_max_i32(int v) {
-  const int id = (int)0x80000000;
-  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0xb1, 0xf, 0xf, false));
-  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x4e, 0xf, 0xf, false));
-  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xf, 0xf, false));
-  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xf, 0xf, false));
-  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xa, 0xf, false));
-  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xc, 0xf, false));
-  return __builtin_amdgcn_readlane(v, 63);
-}
-
-__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
-
-// ------------------------------------------------------------------ codec
-struct CodecParams {
-  double eps_rel, eps_abs;
-  int guard;
-  int e_abs;      // ilogb(2*eps_abs), or INT_MIN when eps_abs == 0
-  double p_abs;   // 2^e_abs
-};
-
-// Per-lane exponent candidate; INT_MAX for zero lanes, INT_MIN requests the
-// verbatim mode (non-finite value or a zero error budget).
-__device__ __forceinline__ int lane_exponent(double x, const CodecParams& cp) {
-  if (x == 0.0) return 0x7fffffff;
-  if (!isfinite(x)) return (int)0x80000000;
-  const double y = 2.0 * fma(cp.eps_rel, fabs(x), cp.eps_abs);
-  if (!(y > 0.0) || !isfinite(y)) return (int)0x80000000;
-  return __builtin_amdgcn_frexp_exp(y) - 1;  // == ilogb(y)
-}
-
-// Quantise one chunk (wave-uniform call).  Returns w (0, 1..52, or 64) and e.
-// e is the minimum over nonzero lanes of ilogb(2*fma(eps_rel,|x|,eps_abs))
-// minus the guard bits.  Every lane has y >= 2*eps_abs, so when one nonzero
-// lane has y < 2^(e_abs+1) the minimum is e_abs itself and the wave-wide
-// reduction is skipped (same result).
-__device__ __forceinline__ void chunk_plan(double x, const CodecParams& cp, uint32_t& w, int& e) {
-  const bool nz = x != 0.0;
-  if (ballot64(nz) == 0) { w = 0; e = 0; return; }
-  int emin;
-  const bool small = nz && (fma(cp.eps_rel, fabs(x), cp.eps_abs) < cp.p_abs);
-  if (cp.e_abs != (int)0x80000000 && ballot64(small) != 0) emin = cp.e_abs;
-  else emin = wave_min_i32(lane_exponent(x, cp));
-  if (emin == (int)0x80000000 || emin < -1000) { w = 64; e = 0; return; }
-  e = emin - cp.guard;
-  const double s = ldexp(x, -e);
-  const bool bad = !(fabs(s) < 2251799813685248.0);  // 2^51 (also NaN / inf)
-  if (ballot64(bad) != 0) { w = 64; e = 0; return; }
-  const double k = rint(s);
-  if (ballot64(!(fabs(k) < 1073741824.0)) == 0) {    // |k| < 2^30: 32-bit codes
-    const int ki = (int)k;
-    const uint32_t u = ((uint32_t)ki << 1) ^ (uint32_t)(ki >> 31);
-    const int bl = (u == 0) ? 0 : 32 - __clz((int)u);
-    w = (uint32_t)wave_max_i32(bl);
-    return;
-  }
-  const double uz = (k >= 0.0) ? 2.0 * k : -2.0 * k - 1.0;
-  const uint64_t u = (uint64_t)uz;
-  const int bl = (u == 0) ? 0 : 64 - __clzll(u);
-  w = (uint32_t)wave_max_i32(bl);
-}
-
-__device__ __forceinline__ uint64_t chunk_code(double x, uint32_t w, int e) {
-  if (w == 64) return (uint64_t)__double_as_longlong(x);
-  if (w == 0) return 0;
-  const double k = rint(ldexp(x, -e));
-  const double uz = (k >= 0.0) ? 2.0 * k : -2.0 * k - 1.0;
-  return (uint64_t)uz;
-}
-
-// zig-zag code of rint(x * 2^-e) for chunks whose codes fit 32 bits
-__device__ __forceinline__ uint32_t chunk_code32(double x, int e) {
-  const int ki = (int)rint(ldexp(x, -e));
-  return ((uint32_t)ki << 1) ^ (uint32_t)(ki >> 31);
-}
-
-__device__ __forceinline__ double decode_lane(uint64_t d, const uint64_t* __restrict__ arena, int lane) {
-  const uint32_t w = desc_w(d);
-  if (w == 0) return 0.0;
-  const uint64_t* P = arena + desc_off(d);
-  if (w == 64) return __longlong_as_double((long long)P[lane]);
-  if (w <= 32) {
-    const uint32_t* P32 = reinterpret_cast<const uint32_t*>(P);
-    const uint32_t bit = (uint32_t)lane * w;
-    const uint32_t q = bit >> 5, s = bit & 31u;
-    const uint32_t lo = P32[q];
-    const uint32_t hi = (s + w > 32u) ? P32[q + 1] : 0u;
-    uint32_t v = (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
-    if (w < 32) v &= (1u << w) - 1u;
-    const int k = (int)(v >> 1) ^ -(int)(v & 1u);
-    return ldexp((double)k, desc_e(d));
-  }
-  const uint32_t bit = (uint32_t)lane * w;
-  const uint32_t q = bit >> 6, s = bit & 63u;
-  uint64_t v = P[q] >> s;
-  if (s + w > 64u) v |= P[q + 1] << (64u - s);
-  v &= (1ull << w) - 1ull;
-  const int64_t k = (int64_t)(v >> 1) ^ -(int64_t)(v & 1ull);
-  return ldexp((double)k, desc_e(d));
-}
-
 // ------------------------------------------------------------ pass kernel
 
-__device__ __forceinline__ void cmad(double& re, double& im, double mr, double mi, double vr, double vi) {
-  re = fma(mr, vr, re);
-  re = fma(-mi, vi, re);
-  im = fma(mr, vi, im);
-  im = fma(mi, vr, im);
-}
 
 template <int K, int TB>
 __device__ void dense_op(const Op& op, double2* tile, uint64_t base, const double* __restrict__ coefs) {
@@ -441,7 +298,7 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(const PassArgs* __restri
       }
       if (threadIdx.x < NCH) cs->pre[threadIdx.x] = woff + incl - v;
       if (threadIdx.x == 0) {
-        const uint32_t sub = blockIdx.x & (kSubArenas - 1);
+        const uint32_t sub = blockIdx.x & dy.sub_mask;
         unsigned long long off = 0;
         uint32_t skip = 0;
         if (tot > 0) {
@@ -645,6 +502,32 @@ __global__ __launch_bounds__(256) void block_bytes_kernel(const uint64_t* __rest
   }
 }
 
+// sum and maximum of the allocation cursors (stride 16 words)
+__global__ __launch_bounds__(256) void cursor_stats_kernel(const unsigned long long* __restrict__ cur, uint32_t n,
+                                                           unsigned long long* out) {
+  __shared__ unsigned long long ss[256], sm[256];
+  unsigned long long s = 0, m = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    const unsigned long long v = cur[(uint64_t)i * 16];
+    s += v;
+    m = v > m ? v : m;
+  }
+  ss[threadIdx.x] = s;
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int d = 128; d >= 1; d >>= 1) {
+    if (threadIdx.x < d) {
+      ss[threadIdx.x] += ss[threadIdx.x + d];
+      sm[threadIdx.x] = sm[threadIdx.x] > sm[threadIdx.x + d] ? sm[threadIdx.x] : sm[threadIdx.x + d];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = ss[0];
+    out[1] = sm[0];
+  }
+}
+
 // ------------------------------------------------------- kernel launchers
 template <int IN, int OUT, int TB, bool BIG = false>
 static hipError_t launch_pass_tb(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
@@ -735,6 +618,11 @@ hipError_t launch_commit_null(uint8_t* nf, uint8_t* nz, uint64_t nb, hipStream_t
 hipError_t launch_block_bytes(const uint64_t* desc, const uint8_t* nullf, uint64_t nb, uint64_t cpb,
                               unsigned long long* out, hipStream_t st) {
   hipLaunchKernelGGL(block_bytes_kernel, dim3(grid_for(nb, 4096u)), dim3(256), 0, st, desc, nullf, nb, cpb, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_cursor_stats(const unsigned long long* cur, uint32_t n, unsigned long long* out, hipStream_t st) {
+  hipLaunchKernelGGL(cursor_stats_kernel, dim3(1), dim3(256), 0, st, cur, n, out);
   return hipGetLastError();
 }
 

@@ -131,3 +131,21 @@ def test_basis_and_empty_stage():
     v = cs.get_raw_vector()
     assert v[777] == 1.0 and np.count_nonzero(v) == 1
     assert cs.get_nonnull_blocks() == [777 >> 6]
+
+
+@pytest.mark.parametrize("n,B,I", [(16, 10, 4), (18, 11, 3)])
+def test_compressed_engines_agree(n, B, I, monkeypatch):
+    c = W.qft_on_product(n, seed=5)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    res = []
+    for eng in ("wave", "lds"):
+        if eng == "lds":
+            monkeypatch.setenv("QOMPSIM_ENGINE", "lds")
+        cs = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, verbose=False)
+        cs.partition_and_register_circuit(c, I)
+        cs.apply_circuit()
+        res.append((cs.get_vector(), cs.get_null_bitmap()))
+    assert np.array_equal(res[0][1], res[1][1])
+    assert np.max(np.abs(res[0][0] - res[1][0])) < 1e-6
+    exact = O.apply_circuit(O.zero_state(n), c.gate_list)
+    assert abs(np.vdot(exact, res[0][0])) > 0.99999

@@ -24,9 +24,11 @@ def tile_bits():
         os.environ["QOMPSIM_TILE_BITS"] = old
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 11, 14])
-def test_random_circuits_match_oracle(n, rng):
-    c = random_circuit(n, 60, rng, max_k=min(3, n))
+@pytest.mark.parametrize("n,max_k", [(1, 1), (2, 2), (3, 3), (5, 3), (8, 3), (11, 3), (14, 3), (10, 2), (13, 2),
+                                     (17, 2)])
+def test_random_circuits_match_oracle(n, max_k, rng):
+    # max_k <= 2 on >= 10 qubits runs the wave-resident engine, the rest the LDS-tile engine
+    c = random_circuit(n, 60, rng, max_k=min(max_k, n))
     psi = rand_state(n, rng)
     s = QuantumState(n)
     s.load(psi)
@@ -45,6 +47,19 @@ def test_tile_sizes(tb, tile_bits, rng):
     s.load(psi)
     c.update_quantum_state(s)
     assert np.max(np.abs(s.get_vector() - O.apply_circuit(psi, c.gate_list))) < TOL
+
+
+@pytest.mark.parametrize("n", [10, 12, 16])
+def test_engines_agree(n, rng, monkeypatch):
+    c = random_circuit(n, 120, rng, max_k=2)
+    psi = rand_state(n, rng)
+    s1, s2 = QuantumState(n), QuantumState(n)
+    s1.load(psi)
+    s2.load(psi)
+    c.update_quantum_state(s1)
+    monkeypatch.setenv("QOMPSIM_ENGINE", "lds")
+    c.update_quantum_state(s2)
+    assert np.max(np.abs(s1.get_vector() - s2.get_vector())) < TOL
 
 
 def test_five_qubit_dense_gate(rng):
