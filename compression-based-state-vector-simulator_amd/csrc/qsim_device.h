@@ -43,6 +43,42 @@ struct TileMapT {
 using TileMapDev = TileMapT<const uint32_t*>;
 using TileMapConst = TileMapT<const QC uint32_t*>;
 
+// (cos a, sin a) for |a| < 2^20: Cody-Waite reduction modulo pi/2 (pi/2 split
+// into three parts; the first two carry 33 significant bits, so k*P1 and k*P2
+// are exact for |k| < 2^20) and the Taylor series of sin and cos on
+// [-pi/4, pi/4] through degree 17 and 18 (truncation below 1e-17).  The
+// coefficients are 1/n! rounded to double.  Phase angles reaching the kernels
+// are sums of terms in [-pi, pi], far inside the range.
+__device__ __forceinline__ double2 cis_reduced(double a) {
+  const double k = rint(a * 0.6366197723675814);  // 2/pi
+  double r = fma(-k, 1.5707963267341256, a);
+  r = fma(-k, 6.077100506303966e-11, r);
+  r = fma(-k, 2.0222662487959506e-21, r);
+  const double r2 = r * r;
+  double q = 2.8114572543455206e-15;        //  1/17!
+  q = fma(q, r2, -7.647163731819816e-13);   // -1/15!
+  q = fma(q, r2, 1.6059043836821613e-10);   //  1/13!
+  q = fma(q, r2, -2.505210838544172e-08);   // -1/11!
+  q = fma(q, r2, 2.7557319223985893e-06);   //  1/9!
+  q = fma(q, r2, -0.0001984126984126984);   // -1/7!
+  q = fma(q, r2, 0.008333333333333333);     //  1/5!
+  q = fma(q, r2, -0.16666666666666666);     // -1/3!
+  const double sr = fma(q * r2, r, r);
+  double p = -1.5619206968586225e-16;       // -1/18!
+  p = fma(p, r2, 4.779477332387385e-14);    //  1/16!
+  p = fma(p, r2, -1.1470745597729725e-11);  // -1/14!
+  p = fma(p, r2, 2.08767569878681e-09);     //  1/12!
+  p = fma(p, r2, -2.755731922398589e-07);   // -1/10!
+  p = fma(p, r2, 2.48015873015873e-05);     //  1/8!
+  p = fma(p, r2, -0.001388888888888889);    // -1/6!
+  p = fma(p, r2, 0.041666666666666664);     //  1/4!
+  p = fma(p, r2, -0.5);                     // -1/2!
+  const double cr = fma(p, r2, 1.0);
+  const int n = (int)k & 3;
+  const double c0 = (n & 1) ? sr : cr, s0 = (n & 1) ? cr : sr;
+  return make_double2(((n + 1) & 2) ? -c0 : c0, (n & 2) ? -s0 : s0);
+}
+
 // wave64 reductions through DPP (quad perms, row mirrors, row broadcasts);
 // result is read from lane 63.  All 64 lanes must be active.
 __device__ __forceinline__ int wave_min_i32(int v) {
@@ -78,12 +114,13 @@ struct CodecParams {
 
 // Per-lane exponent candidate; INT_MAX for zero lanes, INT_MIN requests the
 // verbatim mode (non-finite value or a zero error budget).
+// A non-finite x gives a non-finite y, so one class test (y positive normal
+// or subnormal) covers every verbatim case.
 __device__ __forceinline__ int lane_exponent(double x, const CodecParams& cp) {
-  if (x == 0.0) return 0x7fffffff;
-  if (!isfinite(x)) return (int)0x80000000;
   const double y = 2.0 * fma(cp.eps_rel, fabs(x), cp.eps_abs);
-  if (!(y > 0.0) || !isfinite(y)) return (int)0x80000000;
-  return __builtin_amdgcn_frexp_exp(y) - 1;  // == ilogb(y)
+  const int ey = __builtin_amdgcn_frexp_exp(y) - 1;  // == ilogb(y)
+  const bool ok = __builtin_amdgcn_class(y, 0x180);  // +normal | +subnormal
+  return x == 0.0 ? 0x7fffffff : (ok ? ey : (int)0x80000000);
 }
 
 // Quantise one chunk (wave-uniform call).  Returns w (0, 1..52, or 64) and e.

@@ -839,8 +839,28 @@ static PassDyn cs_dyn(qsim_cs* cs, int out) {
   return dy;
 }
 
+// High word of the smallest |x| for which the quantiser's 2*fma(eps_rel,|x|,eps_abs)
+// is not finite (the predicate is monotone in the bit pattern of |x|).
+static uint32_t codec_overflow_hi(double eps_rel, double eps_abs) {
+  auto ovf = [&](uint64_t bits) {
+    double x;
+    std::memcpy(&x, &bits, sizeof x);
+    return !std::isfinite(2.0 * std::fma(eps_rel, x, eps_abs));
+  };
+  if (ovf(0)) return 0;
+  uint64_t lo = 0, hi = 0x7ff0000000000000ull;
+  while (hi - lo > 1) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (ovf(mid)) hi = mid;
+    else lo = mid;
+  }
+  return (uint32_t)(hi >> 32);
+}
+
 static void cs_fill_codec_fields(qsim_cs* cs, Program& P, uint32_t khi, int zero_mode) {
+  const uint32_t ovf_hi = codec_overflow_hi(cs->cfg.eps_rel, cs->cfg.eps_abs);
   for (auto& a : P.passes) {
+    a.ovf_hi = ovf_hi;
     a.B = cs->B;
     a.khi = khi;
     a.cpb = cs->cpb;

@@ -89,7 +89,8 @@ struct PassArgs {
   int zero_mode;
   double zero_thr;
   int e_abs;               // ilogb(2*eps_abs), INT_MIN when eps_abs == 0
-  int pad_;
+  uint32_t ovf_hi;         // high word of the smallest |x| whose 2*fma(eps_rel,|x|,eps_abs)
+                           // overflows (0x7ff00000 when none is finite)
   double p_abs;            // 2^e_abs
 };
 

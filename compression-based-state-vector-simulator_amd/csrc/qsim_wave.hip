@@ -1,17 +1,23 @@
 // Wave-resident pass kernel for gfx950 (CDNA4, wave64).
 //
-// Each wave owns a "wave tile" of 1024 amplitudes of the group-local state:
-// tile bits 0..5 are the lane (one complex128 per lane, a 1 KiB coalesced row;
-// each half-wave holds one 32-amplitude codec chunk) and tile bits 6..9 are
+// Each wave owns a "wave tile" of 1024 amplitudes of the group-local state.
+// For the gates the tile is held in the wave layout: tile bits 0..5 are the
+// lane (one complex128 per lane, a 1 KiB coalesced row) and tile bits 6..9 are
 // four register slots (16 complex128 per lane in VGPRs) placed on the pass's
 // register qubits E[0..3].  Gates on lane bits exchange partners across lanes,
 // gates on register slots are in-lane, and runs of diagonal gates become
-// per-lane and per-slot phase factors.  There is no LDS tile and no
-// work-group barrier: waves progress independently, so occupancy is bounded by
-// VGPRs only.  The codec works per wave tile: half-wave DPP reductions for the
-// chunk exponent and width, one allocation atomic per wave tile, and wave-local
-// LDS staging for the bit packing.  Results are bit-identical to the LDS-tile
-// kernel (same chunk format and quantiser).
+// per-lane and per-slot phase factors.
+//
+// The codec runs in the chunk layout: one LDS transpose gives lane L the tile
+// positions 16L..16L+15, i.e. one half of codec chunk L>>1 (32 consecutive
+// amplitudes, 64 doubles).  Exponent and width reductions are then in-register
+// plus one partner-lane DPP step, each lane's codes form one contiguous bit
+// range of its chunk's payload, and the tile's payload is staged in LDS and
+// written with one allocation atomic and a coalesced copy.  There is no
+// work-group barrier: waves progress independently.  Results are bit-identical
+// to the LDS-tile kernel (same chunk format and quantiser).
+#include <utility>
+
 #include "qsim_device.h"
 
 namespace qsim {
@@ -19,33 +25,6 @@ namespace qsim {
 constexpr int kRegBits = 4;
 constexpr int kV = 1 << kRegBits;  // complex values per lane
 constexpr int kWaveTileBits = 6 + kRegBits;
-
-__device__ __forceinline__ uint32_t half_bits(uint64_t bal, int lane) {
-  return lane < 32 ? (uint32_t)bal : (uint32_t)(bal >> 32);
-}
-
-// 32-lane reductions: lanes 0..31 get the reduction of lanes 0..31, lanes
-// 32..63 that of lanes 32..63.  All 64 lanes must be active.
-__device__ __forceinline__ int half_min_i32(int v, int lane) {
-  const int id = 0x7fffffff;
-  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0xb1, 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x4e, 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xa, 0xf, false));
-  const int a = __builtin_amdgcn_readlane(v, 31), b = __builtin_amdgcn_readlane(v, 63);
-  return lane < 32 ? a : b;
-}
-__device__ __forceinline__ int half_max_i32(int v, int lane) {
-  const int id = (int)0x80000000;
-  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0xb1, 0xf, 0xf, false));
-  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x4e, 0xf, 0xf, false));
-  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xf, 0xf, false));
-  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xf, 0xf, false));
-  v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xa, 0xf, false));
-  const int a = __builtin_amdgcn_readlane(v, 31), b = __builtin_amdgcn_readlane(v, 63);
-  return lane < 32 ? a : b;
-}
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
   const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, l);
@@ -75,6 +54,64 @@ __device__ __forceinline__ double2 mac2(double2 m0, double2 a, double2 m1, doubl
 }
 
 // ------------------------------------------------------------------ codec
+// value of the partner lane (lane ^ 1)
+__device__ __forceinline__ uint32_t swap1_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xb1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+}
+__device__ __forceinline__ double swap1_f64(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint64_t r = ((uint64_t)swap1_u32((uint32_t)(b >> 32)) << 32) | swap1_u32((uint32_t)b);
+  return __longlong_as_double((long long)r);
+}
+__device__ __forceinline__ int swap1_i32(int v) { return (int)swap1_u32((uint32_t)v); }
+
+// min(a, |b|) with the IEEE NaN rule (a NaN operand yields the other one)
+__device__ __forceinline__ double vmin_abs(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, |%2|" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// wave-wide inclusive prefix sum (gfx9 DPP row shifts and row broadcasts)
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
+// LDS slot of tile position p for the layout transposes: one pad slot per 16
+// positions.  Wave-layout accesses (64v + l) become slot 68v + l + l/16 and
+// chunk-layout accesses (16L + q) slot 17L + q: a per-lane base plus an
+// immediate offset, and free of bank conflicts in both directions.
+constexpr uint32_t kXSlots = 1024 + 64;
+
+// wave layout (lane l, row v: tile position 64v + l) -> chunk layout
+// (lane L, row q: tile position 16L + q), through the wave's LDS buffer
+__device__ __forceinline__ void to_chunk_layout(double2 (&amp)[kV], double2* xb, uint32_t lane) {
+  double2* wa = xb + lane + (lane >> 4);
+#pragma unroll
+  for (int v = 0; v < kV; ++v) wa[68 * v] = amp[v];
+  __builtin_amdgcn_wave_barrier();
+  const double2* ca = xb + 17u * lane;
+#pragma unroll
+  for (int q = 0; q < kV; ++q) amp[q] = ca[q];
+  __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ void to_wave_layout(double2 (&amp)[kV], double2* xb, uint32_t lane) {
+  double2* ca = xb + 17u * lane;
+#pragma unroll
+  for (int q = 0; q < kV; ++q) ca[q] = amp[q];
+  __builtin_amdgcn_wave_barrier();
+  const double2* wa = xb + lane + (lane >> 4);
+#pragma unroll
+  for (int v = 0; v < kV; ++v) amp[v] = wa[68 * v];
+  __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ double unzig_ldexp(uint64_t u, int e) {
   const int64_t k = (int64_t)(u >> 1) ^ -(int64_t)(u & 1ull);
   return ldexp((double)k, e);
@@ -86,94 +123,106 @@ __device__ __forceinline__ uint64_t extract64(const QG uint64_t* __restrict__ P,
   return v & ((1ull << w) - 1ull);
 }
 
-// Doubles 2i and 2i+1 of a chunk (the lane's re and im).  For w <= 32 the
-// 2w-bit field of lane i starts at bit 2iw; three dwords from dword
-// floor(2iw/32) always cover it (the arena keeps 8 bytes of padding past its
-// last payload, so the third dword is always mapped).
-__device__ __forceinline__ double2 decode_pair(uint64_t d, const QG uint64_t* __restrict__ arena, uint32_t i) {
+// Double di of a chunk, any width 0..64 (the general path, branch-free):
+// the code is read from the two payload words that can hold it (the arena
+// keeps 64 bytes of padding past its last payload, so both are mapped).
+__device__ __forceinline__ double decode_at(uint64_t d, const QG uint64_t* __restrict__ arena, uint32_t di) {
   const uint32_t w = desc_w(d);
-  const int e = desc_e(d);
   const QG uint64_t* P = arena + desc_off(d);
-  double2 r = make_double2(0.0, 0.0);
-  if (w - 1u < 32u) {
-    const QG uint32_t* P32 = (const QG uint32_t*)(P);
-    const uint32_t bit = 2u * i * w, q = bit >> 5, s = bit & 31u;
-    const uint32_t d0 = P32[q], d1 = P32[q + 1], d2 = P32[q + 2];
-    const uint32_t lo = __builtin_amdgcn_alignbit(d1, d0, s), hi = __builtin_amdgcn_alignbit(d2, d1, s);
-    const uint64_t x = ((uint64_t)hi << 32) | lo;
-    const uint64_t m = (2ull << (w - 1u)) - 1ull;
-    const uint32_t u0 = (uint32_t)(x & m), u1 = (uint32_t)((x >> w) & m);
-    const int k0 = (int)(u0 >> 1) ^ -(int)(u0 & 1u), k1 = (int)(u1 >> 1) ^ -(int)(u1 & 1u);
-    r = make_double2(ldexp((double)k0, e), ldexp((double)k1, e));
-  } else if (w == 64u) {
-    r = make_double2(__longlong_as_double((long long)P[2 * i]), __longlong_as_double((long long)P[2 * i + 1]));
-  } else if (w != 0u) {
-    r = make_double2(unzig_ldexp(extract64(P, 2u * i * w, w), e), unzig_ldexp(extract64(P, (2u * i + 1u) * w, w), e));
+  const uint32_t bit = di * w, qw = bit >> 6, sh = bit & 63u;
+  const uint64_t v = (P[qw] >> sh) | ((P[qw + 1] << 1) << (63u - sh));
+  const uint64_t u = w >= 64u ? v : v & ((1ull << w) - 1ull);
+  const double x = w >= 64u ? __longlong_as_double((long long)u) : unzig_ldexp(u, desc_e(d));
+  return w == 0u ? 0.0 : x;
+}
+
+// OR a code of w <= 64 bits at bit offset `bit` into an LDS staging row (the
+// second OR is zero when the field does not cross a word boundary)
+__device__ __forceinline__ void stage_or(uint64_t* stg, uint32_t bit, uint64_t c) {
+  const uint32_t qw = bit >> 6, sh = bit & 63u;
+  atomicOr(reinterpret_cast<unsigned long long*>(&stg[qw]), (unsigned long long)(c << sh));
+  atomicOr(reinterpret_cast<unsigned long long*>(&stg[qw + 1]), (unsigned long long)((c >> 1) >> (63u - sh)));
+}
+
+// Quantise the half chunk held by this lane (chunk layout; the partner lane
+// lane^1 holds the other half) and replace amp by the codes (zig-zag integers
+// of rint(x * 2^-e), or the raw bit patterns when the chunk is stored
+// verbatim).  Returns the chunk's w | (e & 0xffff) << 8, the same in both
+// lanes of the pair.
+//
+// e is the minimum over the chunk's nonzero doubles of
+// ilogb(2*fma(eps_rel,|x|,eps_abs)) minus the guard bits.  That function is
+// non-decreasing in |x|, so it is evaluated once, at the smallest nonzero |x|
+// (found as min over |bits(x) - 1| read as doubles: zeros become NaN patterns,
+// which v_min_f64 skips).  The largest high word bounds |x| exactly against
+// powers of two, which decides the 2^51 verbatim bound and the 32-bit code path.
+// A high word at or above ovf_hi (non-finite values, or values whose error
+// budget overflows) takes the exact per-double exponent path.
+// rint is the 1.5*2^52 shift (round-to-nearest-even for |s| < 2^51).
+// Every lane of the wave must call it.
+__device__ __forceinline__ uint32_t lane_quantise(double2 (&amp)[kV], const CodecParams& cp, uint32_t ovf_hi) {
+  double mn = __longlong_as_double(0x7ff0000000000000ll);  // +inf
+  uint32_t hmax = 0;
+#pragma unroll
+  for (int q = 0; q < kV; ++q) {
+    const uint64_t bx = (uint64_t)__double_as_longlong(amp[q].x), by = (uint64_t)__double_as_longlong(amp[q].y);
+    mn = vmin_abs(mn, __longlong_as_double((long long)(bx - 1ull)));
+    mn = vmin_abs(mn, __longlong_as_double((long long)(by - 1ull)));
+    hmax = max(hmax, max((uint32_t)(bx >> 32) & 0x7fffffffu, (uint32_t)(by >> 32) & 0x7fffffffu));
   }
-  return r;
-}
-
-// OR over each half-wave: lane 31 ends with lanes 0..31, lane 63 with 32..63
-__device__ __forceinline__ uint32_t half_or_u32(uint32_t v) {
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xb1, 0xf, 0xf, false);
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4e, 0xf, 0xf, false);
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xf, 0xf, false);
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xf, 0xf, false);
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
-  return v;
-}
-
-struct HalfPlan {
-  uint32_t a, b;  // w | (e & 0xffff) << 8 of the low and the high half-wave
-};
-
-// Quantise the two chunks held by the wave (half-wave h holds the doubles
-// x0, x1 of its 32 lanes) and replace x0, x1 by their codes: zig-zag integers
-// of rint(x * 2^-e), or the raw bit patterns when a chunk is stored verbatim
-// (w = 64).  e is the minimum over nonzero doubles of
-// ilogb(2*fma(eps_rel,|x|,eps_abs)) minus the guard bits; w the largest code
-// bit length; an all-zero chunk gets w = e = 0.  rint uses the 1.5*2^52 shift
-// (exact round-to-nearest-even for |s| < 2^51, the bound that selects the
-// quantised form).  Every lane of the wave must call it.
-__device__ __forceinline__ HalfPlan half_quantise(double& x0, double& x1, const CodecParams& cp, int lane) {
-  const bool nz = x0 != 0.0 || x1 != 0.0;
-  const uint64_t bnz = ballot64(nz);
-  const int le = min(lane_exponent(x0, cp), lane_exponent(x1, cp));
-  const int r = half_min_i32(le, lane);
-  const int ra = __builtin_amdgcn_readlane(r, 0), rb = __builtin_amdgcn_readlane(r, 32);
-  const bool rawA = ra == (int)0x80000000 || ra < -1000, rawB = rb == (int)0x80000000 || rb < -1000;
-  const int eA = rawA ? 0 : ra - cp.guard, eB = rawB ? 0 : rb - cp.guard;
-  const int e = lane < 32 ? eA : eB;
-  const double s0 = ldexp(x0, -e), s1 = ldexp(x1, -e);
-  const uint64_t bbad = ballot64(!(fabs(s0) < 2251799813685248.0) || !(fabs(s1) < 2251799813685248.0));
-  const bool vA = !rawA && (uint32_t)bbad == 0u, vB = !rawB && (uint32_t)(bbad >> 32) == 0u;
+  mn = fmin(mn, swap1_f64(mn));
+  hmax = max(hmax, swap1_u32(hmax));
+  int le;
+  if (hmax >= ovf_hi) {
+    int l2 = 0x7fffffff;
+#pragma unroll
+    for (int q = 0; q < kV; ++q) l2 = min(l2, min(lane_exponent(amp[q].x, cp), lane_exponent(amp[q].y, cp)));
+    le = min(l2, swap1_i32(l2));
+  } else {
+    const bool nzc = (uint64_t)__double_as_longlong(mn) != 0x7ff0000000000000ull;
+    le = nzc ? lane_exponent(__longlong_as_double(__double_as_longlong(mn) + 1ll), cp) : 0x7fffffff;
+  }
+  if (le == 0x7fffffff) {
+    // all-zero chunk: no payload; its codes must be 0 (a -0.0 bit pattern
+    // would otherwise be OR-ed into the next chunk's first word)
+#pragma unroll
+    for (int q = 0; q < kV; ++q) amp[q] = make_double2(0.0, 0.0);
+    return 0u;
+  }
+  bool raw = le == (int)0x80000000 || le < -1000;
+  const int e = raw ? 0 : le - cp.guard;
+  raw = raw || hmax >= ((uint32_t)(51 + 1023 + e) << 20);  // some |x * 2^-e| >= 2^51 (or non-finite)
+  if (raw) return 64u;
   const double M = 6755399441055744.0;  // 1.5 * 2^52
-  const int64_t k0 = __double_as_longlong(s0 + M) - 0x4338000000000000ll;
-  const int64_t k1 = __double_as_longlong(s1 + M) - 0x4338000000000000ll;
-  const uint64_t u0 = ((uint64_t)k0 << 1) ^ (uint64_t)(k0 >> 63);
-  const uint64_t u1 = ((uint64_t)k1 << 1) ^ (uint64_t)(k1 >> 63);
-  const uint64_t m = u0 | u1;
-  const uint32_t olo = half_or_u32((uint32_t)m), ohi = half_or_u32((uint32_t)(m >> 32));
-  const uint64_t orA = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)ohi, 31) << 32) |
-                       (uint32_t)__builtin_amdgcn_readlane((int)olo, 31);
-  const uint64_t orB = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)ohi, 63) << 32) |
-                       (uint32_t)__builtin_amdgcn_readlane((int)olo, 63);
-  const uint32_t wA = orA ? 64u - (uint32_t)__clzll(orA) : 0u, wB = orB ? 64u - (uint32_t)__clzll(orB) : 0u;
-  HalfPlan hp;
-  hp.a = (uint32_t)bnz == 0u ? 0u : (!vA ? 64u : (wA | ((uint32_t)(eA & 0xffff) << 8)));
-  hp.b = (uint32_t)(bnz >> 32) == 0u ? 0u : (!vB ? 64u : (wB | ((uint32_t)(eB & 0xffff) << 8)));
-  const bool verbatim = lane < 32 ? !vA : !vB;
-  x0 = verbatim ? x0 : __longlong_as_double((long long)u0);
-  x1 = verbatim ? x1 : __longlong_as_double((long long)u1);
-  return hp;
-}
-
-// OR a code of nbits (<= 64) at bit offset `bit` into the wave's staging row
-__device__ __forceinline__ void stage_or(uint64_t* stg, uint32_t bit, uint32_t nbits, uint64_t c) {
-  if (c == 0) return;
-  const uint32_t q = bit >> 6, s = bit & 63u;
-  atomicOr(reinterpret_cast<unsigned long long*>(&stg[q]), (unsigned long long)(c << s));
-  if (s + nbits > 64u) atomicOr(reinterpret_cast<unsigned long long*>(&stg[q + 1]), (unsigned long long)(c >> (64u - s)));
+  uint32_t w;
+  if (hmax < ((uint32_t)(30 + 1023 + e) << 20)) {
+    // every |x * 2^-e| < 2^30: 32-bit codes; the low dword of the shifted
+    // double is the integer in two's complement
+    uint32_t orr = 0;
+#pragma unroll
+    for (int q = 0; q < kV; ++q) {
+      const int k0 = (int)(uint32_t)__double_as_longlong(ldexp(amp[q].x, -e) + M);
+      const int k1 = (int)(uint32_t)__double_as_longlong(ldexp(amp[q].y, -e) + M);
+      const uint32_t u0 = ((uint32_t)k0 << 1) ^ (uint32_t)(k0 >> 31), u1 = ((uint32_t)k1 << 1) ^ (uint32_t)(k1 >> 31);
+      orr |= u0 | u1;
+      amp[q] = make_double2(__longlong_as_double((long long)u0), __longlong_as_double((long long)u1));
+    }
+    orr |= swap1_u32(orr);
+    w = orr ? 32u - (uint32_t)__clz((int)orr) : 0u;
+  } else {
+    uint64_t orr = 0;
+#pragma unroll
+    for (int q = 0; q < kV; ++q) {
+      const int64_t k0 = __double_as_longlong(ldexp(amp[q].x, -e) + M) - 0x4338000000000000ll;
+      const int64_t k1 = __double_as_longlong(ldexp(amp[q].y, -e) + M) - 0x4338000000000000ll;
+      const uint64_t u0 = ((uint64_t)k0 << 1) ^ (uint64_t)(k0 >> 63), u1 = ((uint64_t)k1 << 1) ^ (uint64_t)(k1 >> 63);
+      orr |= u0 | u1;
+      amp[q] = make_double2(__longlong_as_double((long long)u0), __longlong_as_double((long long)u1));
+    }
+    orr |= ((uint64_t)swap1_u32((uint32_t)(orr >> 32)) << 32) | swap1_u32((uint32_t)orr);
+    w = orr ? 64u - (uint32_t)__clzll(orr) : 0u;
+  }
+  return w | ((uint32_t)(e & 0xffff) << 8);
 }
 
 // ------------------------------------------------------------ dense ops
@@ -403,23 +452,40 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[kV], c
       aF3 += j == 3 ? f : 0.0;
       used |= 1u << j;
     }
-    double2 L = make_double2(1.0, 0.0), F0 = L, F1 = L, F2 = L, F3 = L;
-    if (used & 16u) sincos(aL, &L.y, &L.x);
-    if (used & 1u) sincos(aF0, &F0.y, &F0.x);
-    if (used & 2u) sincos(aF1, &F1.y, &F1.x);
-    if (used & 4u) sincos(aF2, &F2.y, &F2.x);
-    if (used & 8u) sincos(aF3, &F3.y, &F3.x);
+    // each factor is formed right before it is applied; the scheduling
+    // barriers keep the compiler from evaluating all five at once
     if (used & 16u) {
+      const double2 F = cis_reduced(aL);
 #pragma unroll
-      for (int v = 0; v < kV; ++v) amp[v] = cmul(amp[v], L);
+      for (int v = 0; v < kV; ++v) amp[v] = cmul(amp[v], F);
     }
+    __builtin_amdgcn_sched_barrier(0);
+    if (used & 1u) {
+      const double2 F = cis_reduced(aF0);
 #pragma unroll
-    for (int v = 0; v < kV; ++v) {
-      if ((v & 1) && (used & 1u)) amp[v] = cmul(amp[v], F0);
-      if ((v & 2) && (used & 2u)) amp[v] = cmul(amp[v], F1);
-      if ((v & 4) && (used & 4u)) amp[v] = cmul(amp[v], F2);
-      if ((v & 8) && (used & 8u)) amp[v] = cmul(amp[v], F3);
+      for (int v = 1; v < kV; v += 2) amp[v] = cmul(amp[v], F);
     }
+    __builtin_amdgcn_sched_barrier(0);
+    if (used & 2u) {
+      const double2 F = cis_reduced(aF1);
+#pragma unroll
+      for (int v = 0; v < kV; ++v)
+        if (v & 2) amp[v] = cmul(amp[v], F);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (used & 4u) {
+      const double2 F = cis_reduced(aF2);
+#pragma unroll
+      for (int v = 0; v < kV; ++v)
+        if (v & 4) amp[v] = cmul(amp[v], F);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (used & 8u) {
+      const double2 F = cis_reduced(aF3);
+#pragma unroll
+      for (int v = 8; v < kV; ++v) amp[v] = cmul(amp[v], F);
+    }
+    __builtin_amdgcn_sched_barrier(0);
     if (generic) {
       // terms on two or more slots: one uniform phase per term, applied to
       // the register elements whose slot bits match and the lanes whose lane
@@ -430,9 +496,7 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[kV], c
         if (!(rm & (rm - 1u))) continue;
         if ((cx.base & T.bmask) != T.bval) continue;
         const uint32_t lm = T.tmask & 63u, lv = T.tval & 63u;
-        double sn, cs;
-        sincos(T.angle, &sn, &cs);
-        const double2 f = ((lane & lm) == lv) ? make_double2(cs, sn) : make_double2(1.0, 0.0);
+        const double2 f = ((lane & lm) == lv) ? cis_reduced(T.angle) : make_double2(1.0, 0.0);
 #pragma unroll
         for (int v = 0; v < kV; ++v)
           if (((uint32_t)v & rm) == rv) amp[v] = cmul(amp[v], f);
@@ -441,18 +505,28 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[kV], c
   }
   for (uint32_t d = 0; d < op.ndiag; ++d) {
     const QC DiagTable& Dt = diags[op.diag_off + d];
+    // index bits from base and lane are fixed for the wave; slot bits vary by row
+    uint32_t idx0 = 0, slot_pos = 0, slot_bit[kRegBits] = {0u, 0u, 0u, 0u};
+    for (uint32_t j = 0; j < Dt.m; ++j) {
+      const uint32_t pos = Dt.pos[j];
+      if (!Dt.tile[j]) idx0 |= (uint32_t)((cx.base >> pos) & 1ull) << j;
+      else if (pos < 6) idx0 |= ((lane >> pos) & 1u) << j;
+      else {
+        slot_pos |= 1u << (pos - 6);
+        slot_bit[0] = pos == 6 ? 1u << j : slot_bit[0];
+        slot_bit[1] = pos == 7 ? 1u << j : slot_bit[1];
+        slot_bit[2] = pos == 8 ? 1u << j : slot_bit[2];
+        slot_bit[3] = pos == 9 ? 1u << j : slot_bit[3];
+      }
+    }
+    (void)slot_pos;
+    const uint64_t cbase = Dt.coef_off + idx0;
 #pragma unroll
     for (int v = 0; v < kV; ++v) {
-      uint32_t idx = 0;
-      for (uint32_t j = 0; j < Dt.m; ++j) {
-        const uint32_t pos = Dt.pos[j];
-        uint32_t bit;
-        if (!Dt.tile[j]) bit = (uint32_t)((cx.base >> pos) & 1ull);
-        else if (pos < 6) bit = (lane >> pos) & 1u;
-        else bit = ((uint32_t)v >> (pos - 6)) & 1u;
-        idx |= bit << j;
-      }
-      amp[v] = cmul(amp[v], ldc(C, Dt.coef_off + idx));
+      QSIM_ITER_FENCE();
+      const uint32_t sidx = ((v & 1) ? slot_bit[0] : 0u) | ((v & 2) ? slot_bit[1] : 0u) |
+                            ((v & 4) ? slot_bit[2] : 0u) | ((v & 8) ? slot_bit[3] : 0u);
+      amp[v] = cmul(amp[v], ldc(C, cbase + sidx));
     }
   }
 }
@@ -460,10 +534,11 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[kV], c
 // ------------------------------------------------------------- the kernel
 template <int IN, int OUT>
 __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __restrict__ pa, PassDyn dy, uint64_t ntiles) {
-  // per-wave staging for the packed payload of half a wave tile (8 register
-  // rows = 16 chunks of at most 64 words), plus 8 words of slack for the
-  // zero-valued spill OR past the last field
-  __shared__ uint64_t stg[kThreads / 64][kV / 2 * 2 * 64 + 8];
+  // per wave: the layout transposes (kXSlots complex128) and, reused after the
+  // encoder's transpose, the packed payload of the tile (at most 2048 words)
+  // plus slack for the zero-valued spill OR past the last field
+  static_assert(2 * kXSlots >= 2 * 1024 + 8, "staging must fit the transpose buffer");
+  __shared__ __align__(16) uint64_t wbuf[kThreads / 64][2 * kXSlots];
   const QC PassArgs& a = *(const QC PassArgs*)(pa);
   const QC Op* ops = (const QC Op*)(a.ops);
   const QC PhaseTerm* terms = (const QC PhaseTerm*)(a.terms);
@@ -490,19 +565,19 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
   };
   const WaveCtx cx{lane, base};
   double2 amp[kV];
+  double2* xb = reinterpret_cast<double2*>(wbuf[wave]);
+  const uint32_t g = (uint32_t)lane & 1u;  // half of the chunk (chunk layout)
 
-  // chunk metadata: lane i < 2*kV describes chunk i = (register index i>>1, half i&1)
+  // chunk metadata (chunk layout): lanes 2c and 2c+1 both describe chunk c,
+  // whose first amplitude is tile position 32c
   uint64_t mdesc = 0;
   uint32_t mbid = 0, mci = 0, mnull = 1;
   if constexpr (IN == IN_DEC || OUT == OUT_ENC) {
-    if (lane < 2 * kV) {
-      const uint32_t p0 = ((uint32_t)(lane >> 1) << 6) | ((uint32_t)(lane & 1) << 5);
-      const uint64_t loc = base | tm.off(p0);
-      mbid = bmap[(slot << a.khi) + (loc >> a.B)];
-      mci = (uint32_t)((loc & ((1ull << a.B) - 1ull)) >> kChunkAmpsLog2);
-      mnull = null_in[mbid];
-      if constexpr (IN == IN_DEC) mdesc = mnull ? 0ull : desc_in[(uint64_t)mbid * a.cpb + mci];
-    }
+    const uint64_t loc = base | tm.off(32u * ((uint32_t)lane >> 1));
+    mbid = bmap[(slot << a.khi) + (loc >> a.B)];
+    mci = (uint32_t)((loc & ((1ull << a.B) - 1ull)) >> kChunkAmpsLog2);
+    mnull = null_in[mbid];
+    if constexpr (IN == IN_DEC) mdesc = mnull ? 0ull : desc_in[(uint64_t)mbid * a.cpb + mci];
   }
 
   // ---- load
@@ -511,11 +586,39 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
 #pragma unroll
     for (int v = 0; v < kV; ++v) amp[v] = make_double2(src[2 * roff(v)], src[2 * roff(v) + 1]);
   } else {
+    // this lane's codes are doubles 32g .. 32g+31 of its chunk
+    const uint32_t w = desc_w(mdesc);
+    if (ballot64(w > 32u) == 0) {
+      // every code of the tile has at most 32 bits: all payload loads are
+      // issued before the first decode (one memory round trip per tile)
+      const int e = desc_e(mdesc);
+      const QG uint32_t* P32 = (const QG uint32_t*)(arena_in + desc_off(mdesc));
+      const uint32_t m = (uint32_t)((1ull << w) - 1ull);
+      uint32_t dw[kV][3];
 #pragma unroll
-    for (int v = 0; v < kV; ++v) {
-      const uint64_t dA = readlane64(mdesc, 2 * v), dB = readlane64(mdesc, 2 * v + 1);
-      amp[v] = decode_pair(lane < 32 ? dA : dB, arena_in, (uint32_t)lane & 31u);
+      for (int q = 0; q < kV; ++q) {
+        const uint32_t bit = (32u * g + 2u * q) * w;
+        const QG uint32_t* P = P32 + (bit >> 5);
+        dw[q][0] = P[0];
+        dw[q][1] = P[1];
+        dw[q][2] = P[2];
+      }
+#pragma unroll
+      for (int q = 0; q < kV; ++q) {
+        const uint32_t sh = ((32u * g + 2u * q) * w) & 31u;
+        const uint32_t lo = __builtin_amdgcn_alignbit(dw[q][1], dw[q][0], sh);
+        const uint32_t hi = __builtin_amdgcn_alignbit(dw[q][2], dw[q][1], sh);
+        const uint64_t x = ((uint64_t)hi << 32) | lo;
+        const uint32_t u0 = lo & m, u1 = (uint32_t)(x >> w) & m;
+        const int k0 = (int)(u0 >> 1) ^ -(int)(u0 & 1u), k1 = (int)(u1 >> 1) ^ -(int)(u1 & 1u);
+        amp[q] = make_double2(ldexp((double)k0, e), ldexp((double)k1, e));
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < kV; ++q)
+        amp[q] = make_double2(decode_at(mdesc, arena_in, 32u * g + 2u * q), decode_at(mdesc, arena_in, 32u * g + 2u * q + 1u));
     }
+    to_wave_layout(amp, xb, (uint32_t)lane);
   }
 
   // ---- ops
@@ -534,85 +637,58 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
       dst[2 * roff(v) + 1] = amp[v].y;
     }
   } else {
-    const CodecParams cp{a.eps_rel, a.eps_abs, a.guard, a.e_abs, a.p_abs};
-    // quantise every register row; chunk j = 2v + h is described by lane j
-    uint32_t planA[kV], planB[kV], loffA[kV];
-    uint32_t tot = 0, tot_half = 0;
-    uint32_t dplan = 0, doff = 0;
+    to_chunk_layout(amp, xb, (uint32_t)lane);
+    // zero check of blocks that were null at stage start
+    if (ballot64(mnull != 0u) != 0) {
+      bool nz = false;
 #pragma unroll
-    for (int v = 0; v < kV; ++v) {
-      // zero check of blocks that were null at stage start
-      const uint32_t nullA = (uint32_t)__builtin_amdgcn_readlane((int)mnull, 2 * v);
-      const uint32_t nullB = (uint32_t)__builtin_amdgcn_readlane((int)mnull, 2 * v + 1);
-      if (nullA | nullB) {
-        const bool nzv = a.zero_mode == ZERO_EXACT ? (amp[v].x != 0.0 || amp[v].y != 0.0)
-                                                   : (fabs(amp[v].x) > a.zero_thr || fabs(amp[v].y) > a.zero_thr);
-        const uint64_t bz = ballot64(nzv);
-        const uint32_t bidA = (uint32_t)__builtin_amdgcn_readlane((int)mbid, 2 * v);
-        const uint32_t bidB = (uint32_t)__builtin_amdgcn_readlane((int)mbid, 2 * v + 1);
-        if (lane == 0 && nullA && (uint32_t)bz) nonzero[bidA] = 1;
-        if (lane == 32 && nullB && (uint32_t)(bz >> 32)) nonzero[bidB] = 1;
-      }
-      const HalfPlan hp = half_quantise(amp[v].x, amp[v].y, cp, lane);
-      planA[v] = hp.a;
-      planB[v] = hp.b;
-      if (v == kV / 2) tot_half = tot;
-      loffA[v] = tot;
-      if ((lane >> 1) == v) {
-        dplan = (lane & 1) ? hp.b : hp.a;
-        doff = (lane & 1) ? tot + (hp.a & 0xffu) : tot;
-      }
-      tot += (hp.a & 0xffu) + (hp.b & 0xffu);
+      for (int q = 0; q < kV; ++q)
+        nz = nz || (a.zero_mode == ZERO_EXACT ? (amp[q].x != 0.0 || amp[q].y != 0.0)
+                                              : (fabs(amp[q].x) > a.zero_thr || fabs(amp[q].y) > a.zero_thr));
+      if (mnull && nz) nonzero[mbid] = 1;
     }
+    const CodecParams cp{a.eps_rel, a.eps_abs, a.guard, a.e_abs, a.p_abs};
+    const uint32_t plan = lane_quantise(amp, cp, a.ovf_hi);
+    const uint32_t w = plan & 0xffu;
+    // payload offsets: chunks in order, each w words
+    const uint32_t incl = wave_incl_scan(g == 0 ? w : 0u);
+    const uint32_t loff = incl - w;
+    const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     // one allocation per wave tile; its latency is hidden behind the packing
-    // of the first half of the rows
     const uint32_t sub = (uint32_t)(wt & dy.sub_mask);
     unsigned long long aoff = 0;
     if (lane == 0 && tot > 0) aoff = atomicAdd(&dy.cursors[sub * 16], (unsigned long long)tot);
-    uint64_t* st = stg[wave];
-    const uint32_t i = (uint32_t)lane & 31u;
-    uint64_t tbase = 0;
-    bool skip = false;
+    uint64_t* st = wbuf[wave];
+    for (uint32_t q = lane; q < tot; q += 64) st[q] = 0ull;
+    __builtin_amdgcn_wave_barrier();
+    uint64_t* sc = st + loff;
+    if (ballot64(w > 32u) == 0) {
+      // two codes of at most 32 bits form one field of at most 64 bits
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
-      const uint32_t g0 = g == 0 ? 0u : tot_half, g1 = g == 0 ? tot_half : tot;
-      for (uint32_t q = lane; q < g1 - g0; q += 64) st[q] = 0ull;
-      __builtin_amdgcn_wave_barrier();
+      for (int q = 0; q < kV; ++q)
+        stage_or(sc, (32u * g + 2u * q) * w,
+                 (uint64_t)__double_as_longlong(amp[q].x) | ((uint64_t)__double_as_longlong(amp[q].y) << w));
+    } else {
 #pragma unroll
-      for (int v = g * (kV / 2); v < (g + 1) * (kV / 2); ++v) {
-        const uint32_t pl = lane < 32 ? planA[v] : planB[v];
-        const uint32_t w = pl & 0xffu;
-        const uint32_t cb = (lane < 32 ? loffA[v] : loffA[v] + (planA[v] & 0xffu)) - g0;
-        const uint64_t u0 = (uint64_t)__double_as_longlong(amp[v].x), u1 = (uint64_t)__double_as_longlong(amp[v].y);
-        uint64_t* sc = st + cb;
-        if (w - 1u < 32u) {
-          const uint64_t c = u0 | (u1 << w);
-          const uint32_t bit = 2u * i * w, q = bit >> 6, sh = bit & 63u;
-          atomicOr(reinterpret_cast<unsigned long long*>(&sc[q]), (unsigned long long)(c << sh));
-          atomicOr(reinterpret_cast<unsigned long long*>(&sc[q + 1]), (unsigned long long)((c >> 1) >> (63u - sh)));
-        } else if (w == 64u) {
-          sc[2 * i] = u0;
-          sc[2 * i + 1] = u1;
-        } else if (w != 0u) {
-          stage_or(sc, 2u * i * w, w, u0);
-          stage_or(sc, (2u * i + 1u) * w, w, u1);
-        }
+      for (int q = 0; q < kV; ++q) {
+        const uint32_t di = 32u * g + 2u * q;
+        stage_or(sc, di * w, (uint64_t)__double_as_longlong(amp[q].x));
+        stage_or(sc, (di + 1u) * w, (uint64_t)__double_as_longlong(amp[q].y));
       }
-      __builtin_amdgcn_wave_barrier();
-      if (g == 0 && tot > 0) {
-        const uint64_t off = readlane64(aoff, 0);
-        if (off + tot > dy.sub_cap_words) {
-          skip = true;
-          if (lane == 0) atomicOr(dy.overflow, 1u);
-        }
-        tbase = off + (uint64_t)sub * dy.sub_cap_words;
-      }
-      if (!skip)
-        for (uint32_t q = lane; q < g1 - g0; q += 64) arena_out[tbase + g0 + q] = st[q];
-      __builtin_amdgcn_wave_barrier();
     }
-    if (lane < 2 * kV && !skip)
-      desc_out[(uint64_t)mbid * a.cpb + mci] = desc_pack(tbase + doff, dplan & 0xffu, (int)(int16_t)(uint16_t)(dplan >> 8));
+    __builtin_amdgcn_wave_barrier();
+    if (tot > 0) {
+      const uint64_t off = readlane64(aoff, 0);
+      if (off + tot > dy.sub_cap_words) {
+        if (lane == 0) atomicOr(dy.overflow, 1u);
+      } else {
+        const uint64_t tbase = off + (uint64_t)sub * dy.sub_cap_words;
+        for (uint32_t q = lane; q < tot; q += 64) arena_out[tbase + q] = st[q];
+        if (g == 0) desc_out[(uint64_t)mbid * a.cpb + mci] = desc_pack(tbase + loff, w, (int)(int16_t)(uint16_t)(plan >> 8));
+      }
+    } else if (g == 0) {
+      desc_out[(uint64_t)mbid * a.cpb + mci] = desc_pack(0, 0, 0);
+    }
   }
 }
 

@@ -10,7 +10,7 @@ import pytest
 
 import pyoracle as O
 from gpu_util import rand_state, random_circuit
-from qompsim import CompressedQuantumState, QuantumCircuitOptimizer, QuantumState
+from qompsim import CompressedQuantumState, QuantumCircuit, QuantumCircuitOptimizer, QuantumState, gate
 from qompsim import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -36,6 +36,29 @@ def test_codec_bit_exact_vs_oracle(rng, er, ea, guard):
     assert null[0] and not null[1:].any()
     nonnull_chunks = (1 << (B - 5)) * int((~null).sum())
     assert int(cs.get_block_bytes().sum()) == 8 * (words + nonnull_chunks)
+
+
+def test_codec_negative_zero_chunks(rng):
+    """All-zero chunks made of -0.0 carry no payload and must not disturb the
+    chunk packed after them (regression: their sign bits were once OR-ed into
+    the next chunk's first payload word)."""
+    n, B = 14, 10  # one block per group: L = 10 runs the wave-tile engine
+    x = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    x *= 2.0 ** -7
+    for c0 in (64, 96, 1000, 4096):  # whole 32-amplitude chunks of -0.0
+        x[32 * c0: 32 * c0 + 32] = complex(-0.0, -0.0)
+    x[32 * 300 + 5] = complex(-0.0, 0.25)
+    cs = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, guard_bits=1, verbose=False)
+    cs.load(x)
+    got = cs.get_raw_vector()
+    ref, _, _ = O.codec_roundtrip(x.view(np.float64), 1e-3, 1e-7, 1)
+    assert np.array_equal(got.view(np.float64), ref)
+    # the same through a DEC -> ENC stage that leaves the state unchanged
+    c = QuantumCircuit(n)
+    c.add_gate(gate.DenseMatrix([0], np.eye(2)))
+    cs.register_circuit_stages(c, [[0]], [[[b] for b in range(1 << (n - B))]])
+    cs.apply_circuit()
+    assert np.array_equal(cs.get_raw_vector().view(np.float64), ref)
 
 
 @pytest.mark.parametrize("mode", ["exact", "threshold"])
