@@ -5,11 +5,11 @@
 // as the checker and the timed CPU baseline.  The product (libqsim_hip.so)
 // never links, loads or calls it.  Built with g++ -O3 -fopenmp (oracle/build.py).
 //
-// Parity pins (see tests/test_oracle_*.py): closed forms (GHZ, basis
-// permutations, QFT = unitary DFT checked by numpy FFT), kron-built unitaries
-// for small n, the reference's own csim kernels compiled from
-// /root/reference/src/csim into oracle/_ref when that tree is present, the
-// derived GHZ-20 stage/bitmap fixtures, and codec contract properties.
+// Parity pins (tests/test_oracle.py): closed forms (GHZ, basis permutations,
+// QFT = unitary DFT checked by numpy FFT), kron-built unitaries for small n,
+// the reference's own src/csim kernels compiled in place into oracle/_ref by
+// oracle/ref_csim.mk when the reference tree is present, the derived GHZ-20
+// stage/bitmap fixtures, and codec contract properties.
 #include <omp.h>
 
 #include <algorithm>

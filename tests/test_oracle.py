@@ -57,7 +57,7 @@ def kron_unitary(n, g):
 
 
 # ------------------------------------------------------------- gate kernels
-@pytest.mark.skipif(not os.path.exists(O.REF_SO), reason="reference csim not built (no /root/reference)")
+@pytest.mark.skipif(not os.path.exists(O.REF_SO), reason="reference csim library (oracle/_ref) not built")
 def test_oracle_matches_reference_csim(rng):
     ref = O.RefCsim()
     for n in (3, 7, 14):
@@ -70,7 +70,7 @@ def test_oracle_matches_reference_csim(rng):
         assert np.max(np.abs(a - b)) < 1e-13
 
 
-@pytest.mark.skipif(not os.path.exists(O.REF_SO), reason="reference csim not built (no /root/reference)")
+@pytest.mark.skipif(not os.path.exists(O.REF_SO), reason="reference csim library (oracle/_ref) not built")
 def test_named_rotations_match_reference_csim(rng):
     ref = O.RefCsim()
     n = 5
