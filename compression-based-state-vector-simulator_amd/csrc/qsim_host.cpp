@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <set>
@@ -42,6 +43,11 @@ hipError_t launch_sv_pauli(const double2* a, uint64_t n, uint64_t f, uint64_t z,
 hipError_t launch_sv_apply_pauli(const double2* a, double2* out, uint64_t n, uint64_t f, uint64_t z, double cr,
                                  double ci, int acc, hipStream_t st);
 hipError_t launch_cursor_stats(const unsigned long long* cur, uint32_t n, unsigned long long* out, hipStream_t st);
+hipError_t launch_adjoint_pass(int tb, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
+                               uint64_t ntiles, hipStream_t st);
+hipError_t launch_grad_reduce(const double* partials, uint64_t nblk, uint32_t npoints, double* out, hipStream_t st);
+hipError_t launch_pauli_sum(const double2* a, double2* out, uint64_t n, const PauliGroup* groups, uint32_t ng,
+                            const PauliTermZ* terms, double* partials, unsigned grid, hipStream_t st);
 hipError_t launch_block_bytes(const uint64_t* desc, const uint8_t* nullf, uint64_t nb, uint64_t cpb,
                               unsigned long long* out, hipStream_t st);
 
@@ -312,6 +318,11 @@ static std::vector<HGate> optimize_circuit(std::vector<HGate> gl, uint32_t max_b
 // ---------------------------------------------------------------- planner
 // A gate rewritten in the local qubit numbering of one pass program.
 struct LGate {
+  // gradient point of the adjoint reverse sweep (parameter index), or -1; its
+  // Pauli string has X/Y on gx and Z/Y on gz (qubit masks), nY Y factors
+  int32_t grad = -1;
+  uint64_t gx = 0, gz = 0;
+  uint32_t nY = 0;
   bool diag = false;
   std::vector<uint32_t> N;      // non-diagonal targets (matrix bit order)
   std::vector<uint32_t> S;      // selector qubits
@@ -452,6 +463,28 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
   bool elem_open = false;
   for (size_t gi : pp.gates) {
     const LGate& g = lg[gi];
+    if (g.grad >= 0) {
+      elem_open = false;
+      Op op{};
+      op.kind = OP_GRAD;
+      op.gidx = (uint32_t)g.grad;  // parameter index; renumbered per pass by the caller
+      op.nY = g.nY;
+      for (uint32_t q = 0; q < 64; ++q) {
+        const uint64_t bit = 1ull << q;
+        if (!((g.gx | g.gz) & bit)) continue;
+        const int p = tpos(q);
+        if (g.gx & bit) {
+          if (p < 0) throw std::logic_error("planner: gradient flip outside the tile");
+          op.gfx |= 1ull << p;
+        }
+        if (g.gz & bit) {
+          if (p >= 0) op.gzt |= 1ull << p;
+          else op.gzb |= bit;
+        }
+      }
+      P.ops.push_back(op);
+      continue;
+    }
     if (g.diag) {
       if (!elem_open) {
         Op op{};
@@ -706,6 +739,9 @@ struct qsim_sv {
   uint64_t dim = 0;
   DBuf d;
   mutable Partials part;
+  // adjoint work buffers (kept between calls): lambda, the uncomputed psi,
+  // per-tile gradient partials, reduced points, Pauli-sum tables
+  DBuf adj_lam, adj_psi, adj_part, adj_out, adj_groups, adj_terms;
   double2* ptr() const { return reinterpret_cast<double2*>(d.p); }
 };
 
@@ -1351,6 +1387,172 @@ int qsim_sv_expectation_pauli(const qsim_sv* sv, uint64_t n_terms, const uint64_
   }
   *ore = tr.value();
   *oim = ti.value();
+  API_END
+}
+
+namespace qsim {
+// Pauli-sum tables: terms grouped by flip mask (X/Y qubits); the coefficient
+// of a term is scale * c * i^ny.
+static uint32_t build_pauli_groups(uint64_t n_terms, const uint64_t* xm, const uint64_t* zm, const uint32_t* ny,
+                                   const double* cre, const double* cim, double scale, std::vector<PauliGroup>& G,
+                                   std::vector<PauliTermZ>& T) {
+  std::map<uint64_t, std::vector<uint64_t>> by_flip;
+  for (uint64_t t = 0; t < n_terms; ++t) by_flip[xm[t]].push_back(t);
+  G.clear();
+  T.clear();
+  for (auto& kv : by_flip) {
+    PauliGroup g{};
+    g.flip = kv.first;
+    g.t0 = (uint32_t)T.size();
+    for (auto t : kv.second) {
+      double pr, pi;
+      ipow(ny[t], pr, pi);
+      PauliTermZ z{};
+      z.z = zm[t];
+      z.cr = scale * (cre[t] * pr - cim[t] * pi);
+      z.ci = scale * (cre[t] * pi + cim[t] * pr);
+      T.push_back(z);
+    }
+    g.nt = (uint32_t)(T.size() - g.t0);
+    G.push_back(g);
+  }
+  return (uint32_t)G.size();
+}
+
+static HGate inverse_gate(const HGate& g) {
+  HGate h = g;
+  const size_t D = 1ull << g.t.size();
+  for (size_t r = 0; r < D; ++r)
+    for (size_t c = 0; c < D; ++c) h.m[c * D + r] = std::conj(g.m[r * D + c]);
+  h.flags &= ~QSIM_GATE_PARAMETRIC;
+  return h;
+}
+
+// Gradient point of a parametric Pauli rotation R_P(theta) = exp(+i theta P/2).
+static LGate grad_point(const HGate& g) {
+  LGate L;
+  L.grad = g.param;
+  for (size_t i = 0; i < g.t.size(); ++i) {
+    const uint64_t bit = 1ull << g.t[i];
+    const uint32_t pid = g.pauli[i];
+    if (pid == 1 || pid == 2) L.gx |= bit;
+    if (pid == 2 || pid == 3) L.gz |= bit;
+    if (pid == 2) L.nY++;
+  }
+  for (uint32_t q = 0; q < 64; ++q) {
+    const uint64_t bit = 1ull << q;
+    if (L.gx & bit) L.N.push_back(q);
+    else if (L.gz & bit) (L.gx ? L.S : L.Q).push_back(q);
+  }
+  L.diag = L.gx == 0;
+  return L;
+}
+
+static uint32_t adjoint_tile_bits(uint32_t n) {
+  uint32_t tb = 10;
+  if (const char* e = std::getenv("QOMPSIM_ADJ_TILE_BITS")) tb = (uint32_t)std::atoi(e);
+  tb = std::min<uint32_t>(std::max<uint32_t>(tb, 4), 11);
+  return std::min(tb, n);
+}
+}  // namespace qsim
+
+int qsim_sv_backprop(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, const double* pool, uint64_t pool_len,
+                     uint64_t n_terms, const uint64_t* xm, const uint64_t* zm, const uint32_t* ny, const double* cre,
+                     const double* cim, uint64_t n_params, double* energy, double* grads) {
+  API_BEGIN
+  require(sv != nullptr && energy != nullptr && (grads != nullptr || n_params == 0), "null argument");
+  require(sv->n >= 4, "backprop needs at least 4 qubits");
+  auto gs = parse_gates(gates, n_gates, pool, pool_len);
+  check_qubits(gs, sv->n);
+  for (size_t i = 0; i < gs.size(); ++i) {
+    const HGate& g = gs[i];
+    if (!g.parametric()) continue;
+    if (g.param < 0 || (uint64_t)g.param >= n_params)
+      throw std::out_of_range("gate " + std::to_string(i) + ": parameter index out of range");
+    bool pauli_ok = !g.c.empty() ? false : !g.pauli.empty();
+    for (auto pid : g.pauli) pauli_ok = pauli_ok && pid >= 1 && pid <= 3;
+    if (!pauli_ok)
+      throw std::invalid_argument("gate " + std::to_string(i) + " does not support backprop (not a Pauli rotation)");
+    if (g.t.size() > 3) {
+      uint32_t nxy = 0;
+      for (auto pid : g.pauli) nxy += pid != 3;
+      if (nxy > 3) throw std::invalid_argument("backprop: Pauli rotation with more than 3 X/Y factors");
+    }
+  }
+  for (auto& g : gs) {
+    uint32_t nd = 0;
+    const size_t K = g.t.size(), D = 1ull << K;
+    for (size_t i = 0; i < K; ++i) {
+      bool dq = true;
+      for (size_t r = 0; r < D && dq; ++r)
+        for (size_t c = 0; c < D; ++c)
+          if ((((r ^ c) >> i) & 1u) && g.m[r * D + c] != cplx(0, 0)) { dq = false; break; }
+      nd += !dq;
+    }
+    if (nd > 3) throw std::invalid_argument("backprop: gates with more than 3 non-diagonal targets");
+  }
+  const uint64_t bytes = sv->dim * sizeof(double2);
+  // forward: psi = U psi_in
+  sv_apply_gates(sv, gs);
+  // lambda = 2 H psi; <H> = Re<psi|lambda> / 2
+  std::vector<PauliGroup> G;
+  std::vector<PauliTermZ> T;
+  const uint32_t ng = build_pauli_groups(n_terms, xm, zm, ny, cre, cim, 2.0, G, T);
+  sv->adj_lam.ensure(bytes);
+  sv->adj_psi.ensure(bytes);
+  upload(sv->adj_groups, G.data(), G.size() * sizeof(PauliGroup));
+  upload(sv->adj_terms, T.data(), T.size() * sizeof(PauliTermZ));
+  const unsigned grid = reduce_grid(sv->dim);
+  sv->part.ensure(1, grid);
+  HIPCHECK(launch_pauli_sum(sv->ptr(), sv->adj_lam.as<double2>(), sv->dim, sv->adj_groups.as<PauliGroup>(), ng,
+                            sv->adj_terms.as<PauliTermZ>(), sv->part.d.as<double>(), grid, stream()));
+  sv->part.fetch(1, grid);
+  DDAcc eacc;
+  sv->part.accumulate(1, grid, &eacc);
+  *energy = eacc.value() / 2.0;
+  HIPCHECK(hipMemcpyAsync(sv->adj_psi.p, sv->d.p, bytes, hipMemcpyDeviceToDevice, stream()));
+  // reverse sweep: gradient point, then the inverse, gate by gate from the end
+  std::vector<int> qmap(sv->n);
+  std::iota(qmap.begin(), qmap.end(), 0);
+  std::vector<LGate> lg;
+  for (size_t i = gs.size(); i-- > 0;) {
+    if (gs[i].parametric()) lg.push_back(grad_point(gs[i]));
+    lg.push_back(lower_gate(inverse_gate(gs[i]), qmap));
+  }
+  std::vector<DDAcc> gacc(n_params);
+  if (!lg.empty()) {
+    const uint32_t tb = adjoint_tile_bits(sv->n);
+    Program P;
+    plan_program_engine(P, lg, sv->n, tb, std::min<uint32_t>(5, tb), false, false, false);
+    // renumber gradient points per pass
+    std::vector<std::vector<int32_t>> params(P.passes.size());
+    uint32_t maxpts = 0;
+    for (size_t pi = 0; pi < P.passes.size(); ++pi) {
+      for (uint32_t o = P.passes[pi].op_begin; o < P.passes[pi].op_end; ++o)
+        if (P.ops[o].kind == OP_GRAD) {
+          params[pi].push_back((int32_t)P.ops[o].gidx);
+          P.ops[o].gidx = (uint32_t)(params[pi].size() - 1);
+        }
+      maxpts = std::max<uint32_t>(maxpts, (uint32_t)params[pi].size());
+    }
+    P.upload_all();
+    const uint64_t ntiles = sv->dim >> tb;
+    sv->adj_part.ensure(std::max<uint64_t>(1, (uint64_t)maxpts * ntiles) * 2 * sizeof(double));
+    sv->adj_out.ensure(std::max<uint32_t>(1, maxpts) * 2 * sizeof(double));
+    std::vector<double> hout(2 * std::max<uint32_t>(1, maxpts));
+    for (size_t pi = 0; pi < P.passes.size(); ++pi) {
+      HIPCHECK(launch_adjoint_pass((int)tb, P.dargs(pi), sv->adj_psi.as<double2>(), sv->adj_lam.as<double2>(),
+                                   sv->adj_part.as<double>(), ntiles, stream()));
+      const uint32_t np = (uint32_t)params[pi].size();
+      if (np == 0) continue;
+      HIPCHECK(launch_grad_reduce(sv->adj_part.as<double>(), ntiles, np, sv->adj_out.as<double>(), stream()));
+      HIPCHECK(hipMemcpyAsync(hout.data(), sv->adj_out.p, np * 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+      HIPCHECK(hipStreamSynchronize(stream()));
+      for (uint32_t k = 0; k < np; ++k) gacc[params[pi][k]].merge(hout[2 * k], hout[2 * k + 1]);
+    }
+  }
+  HIPCHECK(hipStreamSynchronize(stream()));
+  for (uint64_t k = 0; k < n_params; ++k) grads[k] = gacc[k].value() / 2.0;
   API_END
 }
 

@@ -29,7 +29,7 @@ constexpr int kSubArenas = 64;         // allocation granularity of arena capaci
 constexpr int kMaxSubArenas = 65536;   // allocation cursors per arena (the launch uses a power of two)
 constexpr int kMaxLocalBits = 48;
 
-enum OpKind : uint32_t { OP_DENSE = 0, OP_ELEM = 1 };
+enum OpKind : uint32_t { OP_DENSE = 0, OP_ELEM = 1, OP_GRAD = 2 };
 enum InMode : int { IN_RAW = 0, IN_DEC = 1 };
 enum OutMode : int { OUT_RAW = 0, OUT_ENC = 1 };
 enum ZeroMode : int { ZERO_EXACT = 0, ZERO_THRESHOLD = 1 };
@@ -53,6 +53,23 @@ struct Op {
   uint8_t sel_tile[8];     // 1: tile bit, 0: base (local index) bit
   uint32_t nterms, term_off;
   uint32_t ndiag, diag_off;
+  // OP_GRAD (adjoint reverse sweep): partial slot gidx of the pass receives
+  // Re sum conj(lambda) * (i P psi) over the tile, P the generator's Pauli
+  // string: X/Y flips on tile bits gfx, Z/Y signs on tile bits gzt and base
+  // bits gzb, nY Y factors
+  uint32_t gidx, nY;
+  uint64_t gfx, gzt, gzb;
+};
+
+// Pauli-sum application: terms grouped by flip mask; coefficient of a term
+// already includes i^nY and any overall scale
+struct PauliGroup {
+  uint64_t flip;
+  uint32_t t0, nt;
+};
+struct PauliTermZ {
+  uint64_t z;
+  double cr, ci;
 };
 
 struct PhaseTerm {

@@ -528,6 +528,168 @@ __global__ __launch_bounds__(256) void cursor_stats_kernel(const unsigned long l
   }
 }
 
+// ------------------------------------------------ adjoint reverse sweep
+// Reverse sweep of the adjoint method (ParametricQuantumCircuit::
+// backprop_inner_product, src/vqcsim/parametric_circuit.cpp:180-279) over raw
+// psi and lambda of one state: one work-group owns the same tile of both
+// vectors, every dense / elementwise op of the pass (the inverted gates) is
+// applied to both, and each OP_GRAD op adds Re sum conj(lambda) * (i P psi)
+// over the tile, compensated, to partials[(gidx * nblk + tile) * 2 + {0, 1}].
+// i P = R_P(pi) for the reference's R_P(theta) = exp(+i theta P / 2).
+template <int TB>
+__global__ __launch_bounds__(kThreads) void adjoint_pass_kernel(const PassArgs* __restrict__ pa, double2* __restrict__ psi,
+                                                                double2* __restrict__ lam, double* __restrict__ partials,
+                                                                uint64_t nblk, uint64_t tile0) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr uint32_t NT = 1u << TB;
+  constexpr int EPT = (NT + kThreads - 1) / kThreads;
+  __shared__ double red_s[kThreads / 64], red_c[kThreads / 64];
+  double2* tp = reinterpret_cast<double2*>(smem);
+  double2* tl = tp + NT;
+  double2* ptab = tl + NT;
+  const PassArgs& a = *pa;
+  const TileMapDev tm{a.t, a.c, a.nE, a.E};
+  const uint64_t tau = tile0 + blockIdx.x;
+  const uint64_t base = tm.base(tau);
+  uint64_t offs[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const uint32_t p = threadIdx.x + e * kThreads;
+    offs[e] = base + tm.off(p & (NT - 1u));
+    if (NT >= kThreads || p < NT) {
+      tp[p] = psi[offs[e]];
+      tl[p] = lam[offs[e]];
+    }
+  }
+  __syncthreads();
+  for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
+    const Op& op = a.ops[o];
+    if (op.kind == OP_DENSE) {
+      switch (op.k) {
+        case 1: dense_op<1, TB>(op, tp, base, a.coefs); dense_op<1, TB>(op, tl, base, a.coefs); break;
+        case 2: dense_op<2, TB>(op, tp, base, a.coefs); dense_op<2, TB>(op, tl, base, a.coefs); break;
+        case 3: dense_op<3, TB>(op, tp, base, a.coefs); dense_op<3, TB>(op, tl, base, a.coefs); break;
+        default: break;
+      }
+    } else if (op.kind == OP_ELEM) {
+      elem_op<TB>(op, tp, base, a.terms, a.diags, a.coefs, ptab);
+      __syncthreads();
+      elem_op<TB>(op, tl, base, a.terms, a.diags, a.coefs, ptab);
+    } else {
+      const uint32_t fx = (uint32_t)op.gfx, zt = (uint32_t)op.gzt;
+      const int kr = (int)((1u + op.nY) & 3u);  // i * i^nY = i^kr
+      const bool bneg = __popcll(base & op.gzb) & 1;
+      double s = 0.0, c = 0.0;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const uint32_t p = threadIdx.x + e * kThreads;
+        if (NT >= kThreads || p < NT) {
+          const uint32_t q = p ^ fx;
+          const double2 l = tl[p], x = tp[q];
+          // conj(l) * x
+          double re = l.x * x.x + l.y * x.y, im = l.x * x.y - l.y * x.x;
+          const bool neg = bneg ^ (bool)(__popc(q & zt) & 1);
+          re = neg ? -re : re;
+          im = neg ? -im : im;
+          const double v = kr == 0 ? re : kr == 1 ? -im : kr == 2 ? -re : im;
+          dd_add(s, c, v);
+        }
+      }
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      for (int d = 32; d >= 1; d >>= 1) {
+        const double s2 = __shfl_xor(s, d, 64), c2 = __shfl_xor(c, d, 64);
+        dd_merge(s, c, s2, c2);
+      }
+      if (lane == 0) { red_s[wave] = s; red_c[wave] = c; }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double ss = red_s[0], cc = red_c[0];
+        for (int q = 1; q < kThreads / 64; ++q) dd_merge(ss, cc, red_s[q], red_c[q]);
+        partials[((uint64_t)op.gidx * nblk + tau) * 2] = ss;
+        partials[((uint64_t)op.gidx * nblk + tau) * 2 + 1] = cc;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const uint32_t p = threadIdx.x + e * kThreads;
+    if (NT >= kThreads || p < NT) {
+      psi[offs[e]] = tp[p];
+      lam[offs[e]] = tl[p];
+    }
+  }
+}
+
+// out[point] = compensated sum over nblk tiles of the pass's partials
+// (one work-group per gradient point)
+__global__ __launch_bounds__(kThreads) void grad_reduce_kernel(const double* __restrict__ partials, uint64_t nblk,
+                                                               double* __restrict__ out) {
+  __shared__ double red_s[kThreads / 64], red_c[kThreads / 64];
+  const double* P = partials + (uint64_t)blockIdx.x * nblk * 2;
+  double s = 0.0, c = 0.0;
+  for (uint64_t i = threadIdx.x; i < nblk; i += kThreads) dd_merge(s, c, P[2 * i], P[2 * i + 1]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int d = 32; d >= 1; d >>= 1) {
+    const double s2 = __shfl_xor(s, d, 64), c2 = __shfl_xor(c, d, 64);
+    dd_merge(s, c, s2, c2);
+  }
+  if (lane == 0) { red_s[wave] = s; red_c[wave] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ss = red_s[0], cc = red_c[0];
+    for (int q = 1; q < kThreads / 64; ++q) dd_merge(ss, cc, red_s[q], red_c[q]);
+    out[2 * blockIdx.x] = ss;
+    out[2 * blockIdx.x + 1] = cc;
+  }
+}
+
+// out[x] = sum_t c_t (P_t a)[x] for a Pauli sum in one sweep
+// (Observable::apply_to_state, src/cppsim/pauli_operator.cpp:132-183): terms
+// sharing a flip mask f share the load of a[x ^ f]; (P a)[x] = i^nY
+// (-1)^{pc((x^f) & zy)} a[x^f] with i^nY folded into the coefficient.  Also
+// accumulates Re sum conj(a[x]) out[x] into partials[blockIdx] (compensated).
+__global__ __launch_bounds__(kThreads) void pauli_sum_kernel(const double2* __restrict__ a, double2* __restrict__ out,
+                                                             uint64_t n, const PauliGroup* __restrict__ groups,
+                                                             uint32_t ng, const PauliTermZ* __restrict__ terms,
+                                                             double* __restrict__ partials) {
+  __shared__ double red_s[kThreads / 64], red_c[kThreads / 64];
+  double s = 0.0, c = 0.0;
+  for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < n; x += (uint64_t)gridDim.x * kThreads) {
+    double re = 0.0, im = 0.0;
+    for (uint32_t g = 0; g < ng; ++g) {
+      const PauliGroup G = groups[g];
+      const uint64_t y = x ^ G.flip;
+      double fr = 0.0, fi = 0.0;
+      for (uint32_t t = G.t0; t < G.t0 + G.nt; ++t) {
+        const PauliTermZ T = terms[t];
+        const bool neg = __popcll(y & T.z) & 1;
+        fr += neg ? -T.cr : T.cr;
+        fi += neg ? -T.ci : T.ci;
+      }
+      const double2 v = a[y];
+      re += fr * v.x - fi * v.y;
+      im += fr * v.y + fi * v.x;
+    }
+    out[x] = make_double2(re, im);
+    const double2 ax = a[x];
+    dd_add(s, c, ax.x * re + ax.y * im);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int d = 32; d >= 1; d >>= 1) {
+    const double s2 = __shfl_xor(s, d, 64), c2 = __shfl_xor(c, d, 64);
+    dd_merge(s, c, s2, c2);
+  }
+  if (lane == 0) { red_s[wave] = s; red_c[wave] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ss = red_s[0], cc = red_c[0];
+    for (int q = 1; q < kThreads / 64; ++q) dd_merge(ss, cc, red_s[q], red_c[q]);
+    partials[2 * blockIdx.x] = ss;
+    partials[2 * blockIdx.x + 1] = cc;
+  }
+}
+
 // ------------------------------------------------------- kernel launchers
 template <int IN, int OUT, int TB, bool BIG = false>
 static hipError_t launch_pass_tb(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
@@ -645,6 +807,50 @@ hipError_t launch_sv_pauli(const double2* a, uint64_t n, uint64_t f, uint64_t z,
 hipError_t launch_sv_apply_pauli(const double2* a, double2* out, uint64_t n, uint64_t f, uint64_t z, double cr,
                                  double ci, int acc, hipStream_t st) {
   hipLaunchKernelGGL(sv_apply_pauli_kernel, dim3(grid_for(n)), dim3(256), 0, st, a, out, n, f, z, cr, ci, acc);
+  return hipGetLastError();
+}
+
+
+// adjoint pass over 2^L amplitudes in tiles of 2^tb (tb in 4..11)
+hipError_t launch_adjoint_pass(int tb, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
+                               uint64_t ntiles, hipStream_t st) {
+  const uint64_t maxb = 1ull << 30;
+  for (uint64_t t0 = 0; t0 < ntiles; t0 += maxb) {
+    const unsigned cnt = (unsigned)std::min<uint64_t>(ntiles - t0, maxb);
+    const size_t shm = sizeof(double2) * ((2ull << tb) + kPhaseTab);
+    switch (tb) {
+      case 4: hipLaunchKernelGGL(adjoint_pass_kernel<4>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
+      case 5: hipLaunchKernelGGL(adjoint_pass_kernel<5>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
+      case 6: hipLaunchKernelGGL(adjoint_pass_kernel<6>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
+      case 7: hipLaunchKernelGGL(adjoint_pass_kernel<7>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
+      case 8: hipLaunchKernelGGL(adjoint_pass_kernel<8>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
+      case 9: hipLaunchKernelGGL(adjoint_pass_kernel<9>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
+      case 10: hipLaunchKernelGGL(adjoint_pass_kernel<10>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
+      case 11: {
+        static bool attr = false;
+        if (!attr) {
+          (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&adjoint_pass_kernel<11>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+          attr = true;
+        }
+        hipLaunchKernelGGL(adjoint_pass_kernel<11>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0);
+        break;
+      }
+      default: return hipErrorInvalidValue;
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_reduce(const double* partials, uint64_t nblk, uint32_t npoints, double* out, hipStream_t st) {
+  if (npoints == 0) return hipSuccess;
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3(npoints), dim3(kThreads), 0, st, partials, nblk, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_pauli_sum(const double2* a, double2* out, uint64_t n, const PauliGroup* groups, uint32_t ng,
+                            const PauliTermZ* terms, double* partials, unsigned grid, hipStream_t st) {
+  hipLaunchKernelGGL(pauli_sum_kernel, dim3(grid), dim3(kThreads), 0, st, a, out, n, groups, ng, terms, partials);
   return hipGetLastError();
 }
 

@@ -196,7 +196,7 @@ class ParametricQuantumCircuit(QuantumCircuit):
         return c
 
     def backprop(self, obs):
-        """Adjoint gradient d<H>/dtheta (src/vqcsim/parametric_circuit.cpp:268-308)."""
+        """Adjoint gradient d<H>/dtheta from |0...0> (src/vqcsim/parametric_circuit.cpp:281-308)."""
         from .adjoint import adjoint_gradient
 
         return list(adjoint_gradient(self, obs)[1])

@@ -101,6 +101,20 @@ int qsim_sv_expectation_pauli(const qsim_sv* sv, uint64_t n_terms, const uint64_
 int qsim_sv_apply_pauli_sum(const qsim_sv* sv, qsim_sv* out, uint64_t n_terms, const uint64_t* x_masks,
                             const uint64_t* z_masks, const uint32_t* y_counts, const double* coef_re,
                             const double* coef_im);
+/* Adjoint gradient of <H> (ParametricQuantumCircuit::backprop,
+ * src/vqcsim/parametric_circuit.cpp:281-308, with backprop_inner_product
+ * :180-279).  On entry `sv` holds the initial state (the reference starts from
+ * |0...0>); on return it holds U|initial>.  `energy` receives Re<H> and
+ * grads[p] the derivative by parameter p of every parametric gate (Pauli
+ * rotations exp(+i theta P / 2), as ParametricRX/RY/RZ/PauliRotation; other
+ * parametric gates are rejected like the reference's NotImplementedException).
+ * H uses the Pauli-term convention of qsim_sv_expectation_pauli.  Runs the
+ * forward pass, lambda = 2 H psi in one sweep, then the reverse sweep in fused
+ * passes over psi and lambda together. */
+int qsim_sv_backprop(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, const double* pool,
+                     uint64_t pool_len, uint64_t n_terms, const uint64_t* x_masks, const uint64_t* z_masks,
+                     const uint32_t* y_counts, const double* coef_re, const double* coef_im,
+                     uint64_t n_params, double* energy, double* grads);
 void* qsim_sv_device_ptr(qsim_sv* sv);
 uint32_t qsim_sv_qubit_count(const qsim_sv* sv);
 

@@ -1,14 +1,61 @@
-"""GPU adjoint gradients against the oracle's restated backprop and central
-finite differences (h=1e-5) on the uncompressed FP64 path."""
+"""GPU adjoint gradients (qsim_sv_backprop) against the oracle's restated
+backprop (src/vqcsim/parametric_circuit.cpp:180-308) and central finite
+differences (h=1e-5) on the uncompressed FP64 path, plus the torch.autograd
+wrapper."""
 import numpy as np
 import pytest
 
 import pyoracle as O
-from qompsim import QuantumState
+from qompsim import QuantumState, _lib
+from qompsim import gate as G
 from qompsim import workloads as W
 from qompsim.adjoint import adjoint_gradient
+from qompsim.circuit import ParametricQuantumCircuit
+from qompsim.observable import Observable
 
 pytestmark = pytest.mark.gpu
+
+
+def _fd(c, obs, i, h=1e-5):
+    th = c.get_parameters()
+    vals = []
+    for d in (h, -h):
+        cp = c.copy()
+        cp.set_parameter(i, th[i] + d)
+        s = QuantumState(c.get_qubit_count())
+        cp.update_quantum_state(s)
+        vals.append(obs.get_expectation_value(s).real)
+    return (vals[0] - vals[1]) / (2 * h)
+
+
+def _mixed_circuit(n, rng, layers=3):
+    """Parametric X/Y/Z and multi-qubit Pauli rotations between fixed gates."""
+    c = ParametricQuantumCircuit(n)
+    for _ in range(layers):
+        for q in range(n):
+            c.add_gate(G.H(q)) if rng.random() < 0.3 else None
+            kind = rng.integers(3)
+            ang = rng.uniform(0, 2 * np.pi)
+            (c.add_parametric_RX_gate, c.add_parametric_RY_gate, c.add_parametric_RZ_gate)[kind](q, ang)
+        for q in range(0, n - 1, 2):
+            c.add_gate(G.CNOT(q, q + 1))
+        qs = [int(x) for x in rng.choice(n, size=3, replace=False)]
+        c.add_parametric_multi_Pauli_rotation_gate(qs, [int(x) for x in rng.integers(1, 4, size=3)],
+                                                   rng.uniform(0, 2 * np.pi))
+        a, b = (int(x) for x in rng.choice(n, size=2, replace=False))
+        c.add_parametric_multi_Pauli_rotation_gate([a, b], [3, 3], rng.uniform(0, 2 * np.pi))
+    return c
+
+
+def _mixed_observable(n, rng):
+    obs = Observable(n)
+    for _ in range(6):
+        qs = rng.choice(n, size=2, replace=False)
+        ps = rng.choice(["X", "Y", "Z"], size=2)
+        obs.add_operator(float(rng.normal()), f"{ps[0]} {qs[0]} {ps[1]} {qs[1]}")
+    for q in range(n):
+        obs.add_operator(0.3, f"X {q}")
+    return obs
 
 
 def test_adjoint_matches_oracle_and_fd():
@@ -20,15 +67,76 @@ def test_adjoint_matches_oracle_and_fd():
     assert abs(e - e0) < 1e-12
     assert np.max(np.abs(g - g0)) < 1e-10
     th = c.get_parameters()
-    h = 1e-5
     for i in range(0, len(th), 5):
-        cp = c.copy()
-        cp.set_parameter(i, th[i] + h)
-        s = QuantumState(n)
-        cp.update_quantum_state(s)
-        ep = obs.get_expectation_value(s)
-        cp.set_parameter(i, th[i] - h)
-        s = QuantumState(n)
-        cp.update_quantum_state(s)
-        em = obs.get_expectation_value(s)
-        assert abs((ep - em) / (2 * h) - g[i]) < 1e-6
+        assert abs(_fd(c, obs, i) - g[i]) < 1e-6
+
+
+@pytest.mark.parametrize("n", [9, 12])
+def test_adjoint_mixed_paulis_vs_oracle(n, rng):
+    c = _mixed_circuit(n, rng)
+    obs = _mixed_observable(n, rng)
+    e, g = adjoint_gradient(c, obs)
+    e0, g0 = O.backprop(c, obs)
+    assert abs(e - e0) < 1e-11
+    assert np.max(np.abs(g - g0)) < 1e-10
+    for i in (0, len(g) // 2, len(g) - 1):
+        assert abs(_fd(c, obs, i) - g[i]) < 1e-6
+
+
+@pytest.mark.parametrize("tb", ["4", "6", "8", "11"])
+def test_adjoint_tile_sizes(tb, rng, monkeypatch):
+    """Small tiles force many passes, gradient points on base bits and
+    several points per pass."""
+    monkeypatch.setenv("QOMPSIM_ADJ_TILE_BITS", tb)
+    n = 12
+    c = _mixed_circuit(n, rng, layers=2)
+    obs = _mixed_observable(n, rng)
+    e, g = adjoint_gradient(c, obs)
+    e0, g0 = O.backprop(c, obs)
+    assert abs(e - e0) < 1e-11
+    assert np.max(np.abs(g - g0)) < 1e-10
+
+
+def test_adjoint_from_given_state_and_final_state(rng):
+    n = 10
+    c = W.hea(n, layers=2, seed=7)
+    obs = W.tfim_observable(n)
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    v /= np.linalg.norm(v)
+    e, g = adjoint_gradient(c, obs, state=v)
+    e0, g0 = O.backprop(c, obs, init=v)
+    assert abs(e - e0) < 1e-11
+    assert np.max(np.abs(g - g0)) < 1e-10
+
+
+def test_backprop_rejects_non_rotation_parametric():
+    n = 5
+    c = ParametricQuantumCircuit(n)
+    c.add_parametric_RY_gate(0, 0.3)
+    rec, pool = _lib.pack_gates(c.gate_list)
+    rec["pauli_ids"][0][:] = 0  # a parametric gate without a Pauli axis
+    obs = W.tfim_observable(n)
+    xm, zm, ny, cr, ci = obs._arrays()
+    s = QuantumState(n)
+    grads = np.zeros(1)
+    import ctypes
+    e = ctypes.c_double()
+    with pytest.raises(ValueError, match="backprop"):
+        _lib.call("qsim_sv_backprop", s._h, _lib.ptr(rec), 1, _lib.ptr(pool), pool.size, len(obs.terms), _lib.ptr(xm),
+                  _lib.ptr(zm), _lib.ptr(ny), _lib.ptr(cr), _lib.ptr(ci), 1, ctypes.byref(e), _lib.ptr(grads))
+
+
+def test_torch_autograd_function():
+    import torch
+
+    from qompsim.torch_adjoint import expectation
+
+    n = 8
+    c = W.hea(n, layers=2, seed=5)
+    obs = W.tfim_observable(n)
+    th = torch.tensor(c.get_parameters(), dtype=torch.float64, requires_grad=True)
+    E = expectation(th, c, obs)
+    (3.0 * E).backward()
+    e0, g0 = O.backprop(c, obs)
+    assert abs(E.item() - e0) < 1e-12
+    assert np.max(np.abs(th.grad.numpy() - 3.0 * g0)) < 1e-10
