@@ -98,6 +98,148 @@ def cpu_baseline(n_cpu, B_cpu, inner, gates_full, stages_full):
     }
 
 
+def adjoint_bench(n=28, layers=8, reps=3):
+    """HEA-n adjoint gradient of the TFIM energy on the uncompressed GPU state:
+    times the forward pass, the forward+energy evaluation and the whole
+    backprop call (forward, lambda = 2 H psi, fused reverse sweep)."""
+    import numpy as np
+
+    from qompsim import QuantumState, _lib
+    from qompsim import workloads as W
+    from qompsim.adjoint import adjoint_gradient
+
+    c = W.hea(n, layers=layers, seed=3)
+    obs = W.tfim_observable(n)
+    npar = c.get_parameter_count()
+    s = QuantumState(n)
+
+    def timed(fn, k):
+        fn()
+        _lib.call("qsim_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        _lib.call("qsim_synchronize")
+        return (time.perf_counter() - t0) / k
+
+    def fwd():
+        s.set_zero_state()
+        c.update_quantum_state(s)
+
+    def fwd_energy():
+        fwd()
+        obs.get_expectation_value(s)
+
+    t_fwd = timed(fwd, reps)
+    t_eval = timed(fwd_energy, reps)
+    out = {}
+
+    def adj():
+        out["eg"] = adjoint_gradient(c, obs)
+
+    t_adj = timed(adj, reps)
+    energy, grads = out["eg"]
+    th = c.get_parameters()
+    h = 1e-5
+    fd_err = 0.0
+    for i in (0, npar // 3, (2 * npar) // 3, npar - 1):
+        vals = []
+        for d in (h, -h):
+            c.set_parameter(i, th[i] + d)
+            fwd()
+            vals.append(obs.get_expectation_value(s))
+        c.set_parameter(i, th[i])
+        fd_err = max(fd_err, abs((vals[0] - vals[1]) / (2 * h) - grads[i]))
+    t_fd = 2 * npar * t_eval
+    del s
+    return {
+        "workload": f"HEA-{n}, {layers} layers (RY, RZ per qubit; CZ chain), {npar} parameters; "
+                    f"H = sum Z_i Z_i+1 + 0.7 sum X_i ({len(obs.terms)} terms)",
+        "energy": energy,
+        "forward_ms": t_fwd * 1e3,
+        "adjoint_ms": t_adj * 1e3,
+        "adjoint_over_forward": t_adj / t_fwd,
+        "grads_per_s": npar / t_adj,
+        "central_fd_ms": t_fd * 1e3,
+        "central_fd_basis": f"2 x {npar} evaluations x measured forward+energy time {t_eval * 1e3:.2f} ms",
+        "speedup_vs_central_fd": t_fd / t_adj,
+        "max_abs_grad_error_vs_fd": fd_err,
+    }
+
+
+def tfim_bench(n=36, B=22, inner=4, steps=12, n_ref=32):
+    """Transverse-field Ising Trotter evolution on the compressed store: X on
+    qubit 1, then per step RZZ(0.2) on the open chain and RX(0.3) on every
+    qubit.  Each step is registered and run as its own circuit so the
+    compression ratio and the norm can be read after every step; <Z_0> is
+    compared with the same circuit on n_ref qubits, uncompressed (the 12-step
+    light cone of Z_0 covers qubits 0..12)."""
+    from qompsim import CompressedQuantumState, QuantumCircuit, QuantumCircuitOptimizer, QuantumState, _lib
+    from qompsim import workloads as W
+    from qompsim.observable import Observable
+
+    cs = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, verbose=False)
+    raw = 16.0 * 2.0 ** n
+    per_step, gates, t_run, oom = [], 0, 0.0, None
+    for s in range(steps):
+        c = QuantumCircuit(n)
+        if s == 0:
+            c.add_X_gate(1)
+        for i in range(n - 1):
+            c.add_multi_Pauli_rotation_gate([i, i + 1], [3, 3], 0.2)
+        for q in range(n):
+            c.add_RX_gate(q, 0.3)
+        QuantumCircuitOptimizer().optimize(c, 2)
+        cs.partition_and_register_circuit(c, inner)
+        t0 = time.perf_counter()
+        try:
+            cs.apply_circuit()
+        except _lib.QsimError as err:  # HBM exhausted: report the steps that ran
+            log(f"tfim-{n} step {s + 1}: {err}")
+            oom = str(err)
+            break
+        _lib.call("qsim_synchronize")
+        gates += c.get_gate_count() + (1 if s == 0 else 0)
+        dt = time.perf_counter() - t0
+        t_run += dt
+        hist = cs.get_bytes_history()
+        nrm = cs.get_squared_norm()
+        per_step.append({"step": s + 1, "ms": dt * 1e3, "stages": cs.get_num_stages(),
+                         "compressed_GiB": hist[-1] / 2**30, "peak_GiB": max(hist) / 2**30,
+                         "ratio": raw / hist[-1], "norm_drift": nrm - 1.0,
+                         "nonnull_blocks": len(cs.get_nonnull_blocks())})
+        log(f"tfim-{n} step {s + 1}: {dt * 1e3:.0f} ms, {hist[-1] / 2**30:.2f} GiB ({raw / hist[-1]:.1f}x), "
+            f"norm-1 {nrm - 1:.2e}")
+    z0 = Observable(n)
+    z0.add_operator(1.0, "Z 0")
+    zc = cs.get_expectation_value(z0) / cs.get_squared_norm() if oom is None else None
+    peak = max(p["peak_GiB"] for p in per_step) if per_step else None
+    done = len(per_step)
+    del cs
+    sv = QuantumState(n_ref)
+    cref = W.tfim_trotter(n_ref, steps=done)
+    QuantumCircuitOptimizer().optimize(cref, 2)
+    cref.update_quantum_state(sv)
+    zr = Observable(n_ref)
+    zr.add_operator(1.0, "Z 0")
+    zu = zr.get_expectation_value(sv)
+    del sv
+    return {
+        "workload": f"TFIM-{n} Trotter, {steps} steps, compressed (B={B}, I={inner}, eps_rel 1e-3, eps_abs 1e-7)",
+        "steps_completed": done,
+        "stopped_by": oom,
+        "gates": gates,
+        "run_ms": t_run * 1e3,
+        "gates_per_s": gates / t_run if t_run > 0 else None,
+        "peak_compressed_GiB": peak,
+        "min_compression_ratio": raw / (peak * 2**30) if peak else None,
+        "per_step": per_step,
+        "z0_compressed": zc,
+        f"z0_uncompressed_{n_ref}q_after_{done}_steps": zu,
+        "z0_abs_diff": abs(zc - zu) if zc is not None else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +250,9 @@ def main():
     ap.add_argument("--inner-size", type=int, default=4)
     ap.add_argument("--no-fidelity", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--workload", choices=["qft32", "tfim36", "hea28-adjoint"], default="qft32",
+                    help="qft32 is the headline bench line; the others print their parity/measurement record")
     ap.add_argument("--cpu-qubits", type=int, default=25)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one 4 GiB device-to-device copy (counter calibration for profiling)")
@@ -116,6 +261,13 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every block group lives on one "
               "MI355X; the multi-GPU exchange is design-only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "tfim36":
+        print(json.dumps({"workload_record": "tfim36", **tfim_bench(n=args.qubits if args.qubits != 32 else 36)}))
+        return
+    if args.workload == "hea28-adjoint":
+        print(json.dumps({"workload_record": "hea28-adjoint", **adjoint_bench(n=args.qubits if args.qubits != 32 else 28)}))
+        return
 
     import numpy as np
 
@@ -213,6 +365,13 @@ def main():
         result["uncompressed_gates_per_s"] = gates_spec / t_plain
         log(f"fidelity {result['fidelity']:.8f}; uncompressed run {t_plain * 1e3:.1f} ms")
         del sv
+
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_bench()
+        a = result["adjoint"]
+        log(f"adjoint HEA-28: forward {a['forward_ms']:.1f} ms, backprop {a['adjoint_ms']:.1f} ms "
+            f"({a['adjoint_over_forward']:.2f}x forward, {a['speedup_vs_central_fd']:.0f}x central FD), "
+            f"max |g - g_fd| {a['max_abs_grad_error_vs_fd']:.2e}")
 
     if not args.no_cpu:
         ncpu = min(args.cpu_qubits, n)

@@ -792,7 +792,8 @@ struct qsim_cs {
   qsim_cs_config cfg{};
   std::vector<HGate> gates;
   std::vector<CStage> stages;
-  DBuf desc[2], arena[2], cursors[2], cstats;
+  DBuf desc[2], arena[2], cursors[2], cstats, subbase[2];
+  std::vector<uint64_t> h_subbase[2];
   uint64_t cap_words[2] = {0, 0};
   uint32_t nsub[2] = {kSubArenas, kSubArenas};
   DBuf null_flags, nonzero, overflow, bmap, scratch, bytes_tmp;
@@ -830,8 +831,23 @@ static uint32_t pick_nsub(uint64_t tiles) {
   return S;
 }
 
-static void cs_reset_out(qsim_cs* cs, int out, uint32_t nsub) {
-  cs->nsub[out] = nsub;
+// Sub-arena s of output arena `out` spans words [base[s], base[s+1]).
+static void cs_upload_subbase(qsim_cs* cs, int out) {
+  upload(cs->subbase[out], cs->h_subbase[out].data(), cs->h_subbase[out].size() * sizeof(uint64_t));
+}
+
+// Clear the allocation state of output arena `out` for a launch with `nsub`
+// cursors.  A first attempt splits the capacity evenly; a retry after an
+// overflow keeps the per-sub-arena sizes cs_check_overflow measured.
+static void cs_reset_out(qsim_cs* cs, int out, uint32_t nsub, bool keep_partition = false) {
+  if (!keep_partition || cs->nsub[out] != nsub || cs->h_subbase[out].size() != (size_t)nsub + 1) {
+    cs->nsub[out] = nsub;
+    auto& hb = cs->h_subbase[out];
+    hb.resize((size_t)nsub + 1);
+    const uint64_t per = cs->cap_words[out] / nsub;
+    for (uint32_t s = 0; s <= nsub; ++s) hb[s] = (uint64_t)s * per;
+    cs_upload_subbase(cs, out);
+  }
   HIPCHECK(launch_fill_u64(cs->cursors[out].as<unsigned long long>(), (uint64_t)nsub * 16, 0ull, stream()));
   HIPCHECK(launch_fill_u8(cs->overflow.as<uint8_t>(), sizeof(uint32_t), 0, stream()));
 }
@@ -846,17 +862,31 @@ static uint64_t cs_read_cursor_words(qsim_cs* cs, int which, uint64_t* maxsub) {
   return h[0];
 }
 
+// After a launch: false when every tile found room.  On overflow the cursors
+// hold each sub-arena's full demand (tiles that did not fit still counted);
+// the sub-arenas are resized to that demand plus 1/4 and the arena grows
+// only when the total exceeds its capacity, so the retry allocates about what
+// the stage needs however unevenly the demand falls across sub-arenas.
 static bool cs_check_overflow(qsim_cs* cs, int out) {
   uint32_t ov = 0;
   HIPCHECK(hipMemcpyAsync(&ov, cs->overflow.p, sizeof(ov), hipMemcpyDeviceToHost, stream()));
   HIPCHECK(hipStreamSynchronize(stream()));
   if (!ov) return false;
-  uint64_t mx = 0;
-  cs_read_cursor_words(cs, out, &mx);
-  const uint64_t need = mx * cs->nsub[out];
-  uint64_t nw = std::max<uint64_t>(cs->cap_words[out] * 2, need + need / 4);
-  nw = words_for_cap(nw * 8);
-  cs_alloc_arena(cs, out, nw);
+  const uint32_t ns = cs->nsub[out];
+  std::vector<unsigned long long> cur((size_t)ns * 16);
+  HIPCHECK(hipMemcpyAsync(cur.data(), cs->cursors[out].p, cur.size() * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  auto& hb = cs->h_subbase[out];
+  hb.assign((size_t)ns + 1, 0);
+  for (uint32_t s = 0; s < ns; ++s) {
+    const uint64_t d = cur[(size_t)s * 16];
+    hb[s + 1] = hb[s] + d + d / 4 + 64;
+  }
+  if (hb[ns] > cs->cap_words[out]) cs_alloc_arena(cs, out, words_for_cap(hb[ns] * 8));
+  const uint64_t slack = cs->cap_words[out] - hb[ns];
+  hb[ns] += slack;  // leftover capacity goes to the last sub-arena
+  cs_upload_subbase(cs, out);
   return true;
 }
 
@@ -867,7 +897,7 @@ static PassDyn cs_dyn(qsim_cs* cs, int out) {
   dy.arena_in = cs->arena[cs->cur].as<uint64_t>();
   dy.desc_out = cs->desc[out].as<uint64_t>();
   dy.arena_out = cs->arena[out].as<uint64_t>();
-  dy.sub_cap_words = cs->cap_words[out] / cs->nsub[out];
+  dy.sub_base = cs->subbase[out].as<uint64_t>();
   dy.cursors = cs->cursors[out].as<unsigned long long>();
   dy.sub_mask = cs->nsub[out] - 1;
   dy.overflow = cs->overflow.as<uint32_t>();
@@ -960,7 +990,7 @@ static void cs_load(qsim_cs* cs, const double* host, uint64_t basis) {
     {
       const PassArgs& io = cs->io.passes[0];
       const uint32_t tb = io.engine == ENGINE_WAVE ? kWaveTile : io.t;
-      cs_reset_out(cs, out, pick_nsub((host ? cs->nb : 1ull) << (cs->B - tb)));
+      cs_reset_out(cs, out, pick_nsub((host ? cs->nb : 1ull) << (cs->B - tb)), attempt > 0);
     }
     if (host) {
       const uint64_t bpb = cs_blocks_per_batch(cs);
@@ -1098,7 +1128,7 @@ static void cs_run_stage(qsim_cs* cs, uint32_t si) {
     {
       const PassArgs& last = P.passes.back();
       const uint32_t tb = last.engine == ENGINE_WAVE ? kWaveTile : last.t;
-      cs_reset_out(cs, out, pick_nsub(nact << (L - tb)));
+      cs_reset_out(cs, out, pick_nsub(nact << (L - tb)), attempt > 0);
     }
     HIPCHECK(hipEventRecord(cs->ev0, stream()));
     PassDyn dy = cs_dyn(cs, out);
@@ -1599,7 +1629,10 @@ int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** 
   qsim_cs_config c;
   if (cfg) c = *cfg;
   else qsim_cs_default_config(&c);
-  if (!(c.eps_rel > 0.0)) throw std::invalid_argument("eps_rel must be > 0");
+  // eps_rel = 0 is the pure absolute bound (the CPU path's SZ3 ABS mode,
+  // compressed_state.hpp:162-189); a zero total budget has no quantiser
+  if (!(c.eps_rel >= 0.0) || !(c.eps_abs >= 0.0) || !(c.eps_rel + c.eps_abs > 0.0))
+    throw std::invalid_argument("error bounds must be >= 0 with eps_rel + eps_abs > 0");
   require(c.eps_abs >= 0.0, "eps_abs must be >= 0");
   require(c.guard_bits >= 0 && c.guard_bits <= 20, "guard_bits out of range");
   require(c.zero_mode == QSIM_ZERO_EXACT || c.zero_mode == QSIM_ZERO_THRESHOLD, "bad zero_mode");
@@ -1614,7 +1647,10 @@ int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** 
   for (int w = 0; w < 2; ++w) {
     cs->desc[w].alloc(cs->nb * cs->cpb * 8);
     cs->cursors[w].alloc((size_t)kMaxSubArenas * 16 * 8);
-    const uint64_t init = std::min<uint64_t>(std::max<uint64_t>(raw / 6, 8ull << 20), 24ull << 30);
+    // initial capacity; an overflowing stage regrows its output arena to the
+    // measured demand (QOMPSIM_ARENA_KIB sets the start, for tests)
+    uint64_t init = std::min<uint64_t>(std::max<uint64_t>(raw / 16, 8ull << 20), 4ull << 30);
+    if (const char* e = std::getenv("QOMPSIM_ARENA_KIB")) init = std::max<uint64_t>(1, std::atoll(e)) << 10;
     cs_alloc_arena(cs.get(), w, words_for_cap(init));
   }
   cs->null_flags.alloc(cs->nb);
@@ -1717,8 +1753,13 @@ int qsim_cs_apply_circuit(qsim_cs* cs) {
 
 static void cs_norm_inner(qsim_cs* cs, const qsim_sv* sv, DDAcc* acc) {
   const uint64_t bpb = cs_blocks_per_batch(cs);
+  HIPCHECK(hipMemcpyAsync(cs->h_null.data(), cs->null_flags.p, cs->nb, hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
   for (uint64_t b0 = 0; b0 < cs->nb; b0 += bpb) {
     const uint64_t cnt = std::min(bpb, cs->nb - b0);
+    bool any = false;
+    for (uint64_t b = b0; b < b0 + cnt; ++b) any = any || !cs->h_null[b];
+    if (!any) continue;  // null blocks decode to zeros: no contribution
     std::vector<uint32_t> ids(cnt);
     std::iota(ids.begin(), ids.end(), (uint32_t)b0);
     cs_decode_to_scratch(cs, ids);
@@ -1737,6 +1778,55 @@ static void cs_norm_inner(qsim_cs* cs, const qsim_sv* sv, DDAcc* acc) {
       cs->part.accumulate(1, grid, acc);
     }
   }
+}
+
+int qsim_cs_expectation_pauli(qsim_cs* cs, uint64_t n_terms, const uint64_t* xm, const uint64_t* zm,
+                              const uint32_t* ny, const double* cre, const double* cim, double* ore, double* oim) {
+  API_BEGIN
+  // batches of a power-of-two number of blocks, so a batch starting at block
+  // b0 covers global indices (b0 << B) | local and flips stay inside it
+  uint64_t bpb = 1;
+  while (bpb * 2 <= cs_blocks_per_batch(cs)) bpb *= 2;
+  const uint64_t span = bpb << cs->B;
+  for (uint64_t t = 0; t < n_terms; ++t)
+    if (xm[t] >= span)
+      throw std::invalid_argument("expectation: X/Y factors on qubits above the decode batch are not supported");
+  std::vector<DDAcc> acc(2 * n_terms);
+  HIPCHECK(hipMemcpyAsync(cs->h_null.data(), cs->null_flags.p, cs->nb, hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  for (uint64_t b0 = 0; b0 < cs->nb; b0 += bpb) {
+    const uint64_t cnt = std::min(bpb, cs->nb - b0);
+    bool any = false;
+    for (uint64_t b = b0; b < b0 + cnt; ++b) any = any || !cs->h_null[b];
+    if (!any) continue;
+    std::vector<uint32_t> ids(cnt);
+    std::iota(ids.begin(), ids.end(), (uint32_t)b0);
+    cs_decode_to_scratch(cs, ids);
+    const uint64_t n = cnt << cs->B;
+    const unsigned grid = reduce_grid(n);
+    cs->part.ensure(2, grid);
+    for (uint64_t t = 0; t < n_terms; ++t) {
+      HIPCHECK(launch_sv_pauli(cs->scratch.as<double2>(), n, xm[t], zm[t], cs->part.d.as<double>(), grid, stream()));
+      cs->part.fetch(2, grid);
+      DDAcc a[2];
+      cs->part.accumulate(2, grid, a);
+      const double sg = (__builtin_popcountll((b0 << cs->B) & zm[t]) & 1) ? -1.0 : 1.0;
+      acc[2 * t].add(sg * a[0].value());
+      acc[2 * t + 1].add(sg * a[1].value());
+    }
+  }
+  DDAcc tr, ti;
+  for (uint64_t t = 0; t < n_terms; ++t) {
+    double pr, pi;
+    ipow(ny[t], pr, pi);
+    const double sr = acc[2 * t].value(), si = acc[2 * t + 1].value();
+    const double fr = cre[t] * pr - cim[t] * pi, fi = cre[t] * pi + cim[t] * pr;
+    tr.add(fr * sr - fi * si);
+    ti.add(fr * si + fi * sr);
+  }
+  *ore = tr.value();
+  *oim = ti.value();
+  API_END
 }
 
 int qsim_cs_norm_squared(qsim_cs* cs, double* out) {

@@ -35,6 +35,9 @@ enum OutMode : int { OUT_RAW = 0, OUT_ENC = 1 };
 enum ZeroMode : int { ZERO_EXACT = 0, ZERO_THRESHOLD = 1 };
 enum Engine : uint32_t { ENGINE_LDS = 0, ENGINE_WAVE = 1 };
 constexpr uint32_t kWaveTile = 10;  // wave tile bits: 6 lane bits + 4 register slots
+// A dispatch packet gives the grid in work-items as 32-bit values: launches of
+// 256-thread work-groups are split at 2^23 groups (2^31 work-items).
+constexpr uint64_t kMaxLaunchBlocks = 1ull << 23;
 
 // One op of a pass.  Dense: 2^nsel matrices of 2^k x 2^k (row-major, matrix
 // index bit i <-> tile bit tpos[i]); the matrix is selected by the selector
@@ -120,7 +123,7 @@ struct PassDyn {
   const uint64_t* arena_in;
   uint64_t* desc_out;
   uint64_t* arena_out;
-  uint64_t sub_cap_words;   // capacity of each sub-arena (words)
+  const uint64_t* sub_base; // sub-arena s spans words [sub_base[s], sub_base[s + 1])
   unsigned long long* cursors;  // [sub_mask + 1] (stride 16 words)
   uint32_t sub_mask;        // number of sub-arenas - 1 (power of two)
   uint32_t pad_;

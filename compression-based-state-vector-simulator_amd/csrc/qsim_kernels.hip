@@ -303,9 +303,9 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(const PassArgs* __restri
         uint32_t skip = 0;
         if (tot > 0) {
           off = atomicAdd(&dy.cursors[sub * 16], (unsigned long long)tot);
-          if (off + tot > dy.sub_cap_words) { skip = 1; atomicOr(dy.overflow, 1u); }
+          if (off + tot > dy.sub_base[sub + 1] - dy.sub_base[sub]) { skip = 1; atomicOr(dy.overflow, 1u); }
         }
-        cs->tile_base = off + (unsigned long long)sub * dy.sub_cap_words;
+        cs->tile_base = off + dy.sub_base[sub];
         cs->skip = skip;
       }
       cs->stage[wave][lane] = 0ull;
@@ -702,7 +702,7 @@ static hipError_t launch_pass_tb(const PassArgs* d_args, const PassDyn& dy, uint
     attr_set = true;
   }
   PassDyn d = dy;
-  const uint64_t maxg = 1ull << 30;
+  const uint64_t maxg = kMaxLaunchBlocks;
   for (uint64_t t0 = 0; t0 < ntiles; t0 += maxg) {
     const uint64_t cnt = (ntiles - t0 < maxg) ? (ntiles - t0) : maxg;
     d.tile0 = dy.tile0 + t0;
@@ -814,7 +814,7 @@ hipError_t launch_sv_apply_pauli(const double2* a, double2* out, uint64_t n, uin
 // adjoint pass over 2^L amplitudes in tiles of 2^tb (tb in 4..11)
 hipError_t launch_adjoint_pass(int tb, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
                                uint64_t ntiles, hipStream_t st) {
-  const uint64_t maxb = 1ull << 30;
+  const uint64_t maxb = kMaxLaunchBlocks;
   for (uint64_t t0 = 0; t0 < ntiles; t0 += maxb) {
     const unsigned cnt = (unsigned)std::min<uint64_t>(ntiles - t0, maxb);
     const size_t shm = sizeof(double2) * ((2ull << tb) + kPhaseTab);

@@ -679,10 +679,11 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
     __builtin_amdgcn_wave_barrier();
     if (tot > 0) {
       const uint64_t off = readlane64(aoff, 0);
-      if (off + tot > dy.sub_cap_words) {
+      const uint64_t sb0 = dy.sub_base[sub], sb1 = dy.sub_base[sub + 1];
+      if (off + tot > sb1 - sb0) {
         if (lane == 0) atomicOr(dy.overflow, 1u);
       } else {
-        const uint64_t tbase = off + (uint64_t)sub * dy.sub_cap_words;
+        const uint64_t tbase = off + sb0;
         for (uint32_t q = lane; q < tot; q += 64) arena_out[tbase + q] = st[q];
         if (g == 0) desc_out[(uint64_t)mbid * a.cpb + mci] = desc_pack(tbase + loff, w, (int)(int16_t)(uint16_t)(plan >> 8));
       }
@@ -695,7 +696,7 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
 template <int IN, int OUT>
 static hipError_t launch_wave_mode(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
   constexpr uint64_t per_block = kThreads / 64;
-  const uint64_t maxb = 1ull << 30;
+  const uint64_t maxb = kMaxLaunchBlocks;
   PassDyn d = dy;
   for (uint64_t t0 = 0; t0 < ntiles; t0 += maxb * per_block) {
     const uint64_t cnt = std::min<uint64_t>(ntiles - t0, maxb * per_block);

@@ -88,6 +88,7 @@ _SIGS = {
     "qsim_sv_inner_product": (ctypes.c_int, [_vp, _vp, _P(_dbl), _P(_dbl)]),
     "qsim_sv_expectation_pauli": (ctypes.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _P(_dbl), _P(_dbl)]),
     "qsim_sv_apply_pauli_sum": (ctypes.c_int, [_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
+    "qsim_cs_expectation_pauli": (ctypes.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _P(_dbl), _P(_dbl)]),
     "qsim_sv_backprop": (ctypes.c_int, [_vp, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _vp, _vp, _u64, _P(_dbl), _vp]),
     "qsim_sv_device_ptr": (_vp, [_vp]),
     "qsim_sv_qubit_count": (_u32, [_vp]),

@@ -318,6 +318,19 @@ class CompressedQuantumState:
         call("qsim_cs_inner_product_sv", self._h, state._h, ctypes.byref(re), ctypes.byref(im))
         return complex(re.value, im.value)
 
+    def get_expectation_value(self, observable):
+        """<psi|H|psi> over the compressed store (decoded in batches on the GPU);
+        real for an Observable."""
+        if not observable.terms:
+            return 0.0
+        xm, zm, ny, cr, ci = observable._arrays()
+        re, im = ctypes.c_double(), ctypes.c_double()
+        call("qsim_cs_expectation_pauli", self._h, len(observable.terms), ptr(xm), ptr(zm), ptr(ny), ptr(cr), ptr(ci),
+             ctypes.byref(re), ctypes.byref(im))
+        from .observable import Observable
+
+        return re.value if isinstance(observable, Observable) else complex(re.value, im.value)
+
     def fidelity_with(self, state):
         """|<state|self>| / ||self|| (run_sim.py:187-205 after the readout renormalisation)."""
         return abs(self.inner_product_with(state)) / np.sqrt(self.get_squared_norm())

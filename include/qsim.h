@@ -127,7 +127,7 @@ uint32_t qsim_sv_qubit_count(const qsim_sv* sv);
 #define QSIM_ZERO_THRESHOLD 1 /* CPU semantics, compressed_state.hpp:939-951 */
 
 typedef struct qsim_cs_config {
-  double eps_rel;          /* BITCOMP_REL_EB (must be > 0) */
+  double eps_rel;          /* BITCOMP_REL_EB (>= 0; 0 = pure absolute bound) */
   double eps_abs;          /* BITCOMP_ABS_EB */
   int32_t guard_bits;      /* extra quantisation bits below the bound */
   int32_t zero_mode;       /* QSIM_ZERO_EXACT or QSIM_ZERO_THRESHOLD */
@@ -174,6 +174,13 @@ int qsim_cs_max_sv_bytes(const qsim_cs* cs, uint64_t* out);
 int qsim_cs_stage_stats(const qsim_cs* cs, double* out5, uint64_t cap, uint64_t* n);
 #define QSIM_TIMER_COUNT 10 /* decomp gate comp fused_swap allgather hmalloc dmalloc h2d d2h dfree */
 int qsim_cs_times(const qsim_cs* cs, double* out);
+/* sum_t coef_t <psi|P_t|psi> over the compressed store (no reference
+ * counterpart: the reference reads expectation values from the decompressed
+ * vector).  Blocks are decoded in batches; X/Y factors must lie on qubits
+ * inside one decode batch (the block qubits and the low block-id bits). */
+int qsim_cs_expectation_pauli(qsim_cs* cs, uint64_t n_terms, const uint64_t* x_masks, const uint64_t* z_masks,
+                              const uint32_t* y_counts, const double* coef_re, const double* coef_im,
+                              double* out_re, double* out_im);
 int qsim_cs_norm_squared(qsim_cs* cs, double* out);
 int qsim_cs_inner_product_sv(qsim_cs* cs, const qsim_sv* sv, double* re, double* im);
 uint32_t qsim_cs_qubit_count(const qsim_cs* cs);

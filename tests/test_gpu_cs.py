@@ -61,6 +61,30 @@ def test_codec_negative_zero_chunks(rng):
     assert np.array_equal(cs.get_raw_vector().view(np.float64), ref)
 
 
+def test_ghz20_abs_bound_cpu_contract():
+    """GHZ-20 under the CPU path's contract: pure absolute bound 1e-7 (SZ3 ABS
+    mode, substituted by this codec with eps_rel = 0), all-zero threshold
+    1e-6, fidelity mode against the uncompressed state."""
+    fx = json.load(open(os.path.join(GOLD, "ghz20_b12_i4.json")))
+    c = W.ghz(20)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    cs = CompressedQuantumState(20, 12, eps_rel=0.0, eps_abs=1e-7, zero_mode="threshold", zero_threshold=1e-6,
+                                verbose=False)
+    cs.partition_and_register_circuit(c, 4)
+    cs.apply_circuit()
+    assert cs.get_nonnull_blocks() == fx["final_nonnull_blocks"]
+    sv = QuantumState(20)
+    c.update_quantum_state(sv)
+    exact = sv.get_vector()
+    raw = cs.get_raw_vector()
+    assert np.max(np.abs(raw.view(np.float64) - exact.view(np.float64))) <= 1e-7
+    assert cs.fidelity_with(sv) >= 1 - 1e-12
+    oc = O.OracleCompressedState(20, 12, 0.0, 1e-7, 1, "threshold", 1e-6)
+    oc.partition_and_register_circuit(c, 4)
+    oc.apply_circuit()
+    assert np.array_equal(cs.get_null_bitmap(), oc.null_bitmap())
+
+
 @pytest.mark.parametrize("mode", ["exact", "threshold"])
 def test_ghz20_fixture(mode):
     fx = json.load(open(os.path.join(GOLD, "ghz20_b12_i4.json")))
@@ -172,3 +196,62 @@ def test_compressed_engines_agree(n, B, I, monkeypatch):
     assert np.max(np.abs(res[0][0] - res[1][0])) < 1e-6
     exact = O.apply_circuit(O.zero_state(n), c.gate_list)
     assert abs(np.vdot(exact, res[0][0])) > 0.99999
+
+
+def test_compressed_expectation_matches_uncompressed(rng):
+    n, B = 14, 9
+    c = random_circuit(n, 80, rng, max_k=2)
+    sv = QuantumState(n)
+    c.update_quantum_state(sv)
+    cs = CompressedQuantumState(n, B, eps_rel=1e-12, eps_abs=1e-12, guard_bits=2, verbose=False)
+    cs.load(sv.get_vector())
+    obs = W.tfim_observable(n)
+    obs.add_operator(0.25, f"Z 0 Z {n - 1}")
+    obs.add_operator(-0.5, f"Y 3 Z 12")
+    got = cs.get_expectation_value(obs)
+    ref = obs.get_expectation_value(sv)
+    assert abs(got - ref) < 1e-9
+
+
+def test_tfim_light_cone_compressed_vs_uncompressed():
+    """Reduced cfg5: after 12 Trotter steps <Z_0> depends only on qubits
+    0..12, so a 16-qubit uncompressed run pins the 20-qubit compressed one
+    (and the 20-qubit uncompressed run pins the light-cone argument)."""
+    from qompsim.observable import Observable
+
+    z0 = Observable(20)
+    z0.add_operator(1.0, "Z 0")
+    c20 = W.tfim_trotter(20)
+    QuantumCircuitOptimizer().optimize(c20, 2)
+    cs = CompressedQuantumState(20, 12, eps_rel=1e-3, eps_abs=1e-7, verbose=False)
+    cs.partition_and_register_circuit(c20, 4)
+    cs.apply_circuit()
+    zc = cs.get_expectation_value(z0) / cs.get_squared_norm()
+    sv20 = QuantumState(20)
+    c20.update_quantum_state(sv20)
+    z20 = z0.get_expectation_value(sv20)
+    z16_obs = Observable(16)
+    z16_obs.add_operator(1.0, "Z 0")
+    sv16 = QuantumState(16)
+    W.tfim_trotter(16).update_quantum_state(sv16)
+    z16 = z16_obs.get_expectation_value(sv16)
+    assert abs(z20 - z16) < 1e-12
+    assert abs(zc - z16) < 2e-3
+    assert abs(cs.get_squared_norm() - 1.0) < 1e-2
+
+
+def test_arena_overflow_regrowth_is_exact(rng, monkeypatch):
+    """A tiny starting arena forces overflow retries with measured
+    sub-arena sizes; the result must equal a run that never overflowed."""
+    n, B = 16, 10
+    c = random_circuit(n, 60, rng, max_k=2)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    ref = CompressedQuantumState(n, B, verbose=False)
+    ref.partition_and_register_circuit(c, 3)
+    ref.apply_circuit()
+    monkeypatch.setenv("QOMPSIM_ARENA_KIB", "16")
+    cs = CompressedQuantumState(n, B, verbose=False)
+    cs.partition_and_register_circuit(c, 3)
+    cs.apply_circuit()
+    assert np.array_equal(cs.get_raw_vector().view(np.uint64), ref.get_raw_vector().view(np.uint64))
+    assert cs.get_bytes_history() == ref.get_bytes_history()
