@@ -46,6 +46,7 @@ hipError_t launch_cursor_stats(const unsigned long long* cur, uint32_t n, unsign
 hipError_t launch_adjoint_pass(int tb, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
                                uint64_t ntiles, hipStream_t st);
 hipError_t launch_grad_reduce(const double* partials, uint64_t nblk, uint32_t npoints, double* out, hipStream_t st);
+hipError_t launch_accumulate_points(const double* pt, uint32_t np, const int32_t* map, double* acc, hipStream_t st);
 hipError_t launch_pauli_sum(const double2* a, double2* out, uint64_t n, const PauliGroup* groups, uint32_t ng,
                             const PauliTermZ* terms, double* partials, unsigned grid, hipStream_t st);
 hipError_t launch_block_bytes(const uint64_t* desc, const uint8_t* nullf, uint64_t nb, uint64_t cpb,
@@ -54,11 +55,16 @@ hipError_t launch_block_bytes(const uint64_t* desc, const uint8_t* nullf, uint64
 using cplx = std::complex<double>;
 static thread_local std::string g_err;
 
+// A failing runtime call also leaves its code as the thread's "last error",
+// which the next launch check would report again; it is cleared before the
+// throw so one failure is reported once.
 #define HIPCHECK(x)                                                                                   \
   do {                                                                                                \
     hipError_t e_ = (x);                                                                              \
-    if (e_ != hipSuccess)                                                                             \
+    if (e_ != hipSuccess) {                                                                           \
+      (void)hipGetLastError();                                                                        \
       throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " in " + #x);      \
+    }                                                                                                 \
   } while (0)
 
 // ------------------------------------------------------------------ context
@@ -444,6 +450,16 @@ struct Program {
     uploaded = true;
   }
   const PassArgs* dargs(size_t i) const { return d_args.as<PassArgs>() + i; }
+  // forget the plan (device buffers are kept and reused by the next upload)
+  void reset() {
+    L = 0;
+    passes.clear();
+    ops.clear();
+    terms.clear();
+    diags.clear();
+    coefs.clear();
+    uploaded = false;
+  }
 };
 
 static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& lg) {
@@ -1481,7 +1497,7 @@ static LGate grad_point(const HGate& g) {
 static uint32_t adjoint_tile_bits(uint32_t n) {
   uint32_t tb = 10;
   if (const char* e = std::getenv("QOMPSIM_ADJ_TILE_BITS")) tb = (uint32_t)std::atoi(e);
-  tb = std::min<uint32_t>(std::max<uint32_t>(tb, 4), 11);
+  tb = std::min<uint32_t>(std::max<uint32_t>(tb, 4), 12);
   return std::min(tb, n);
 }
 }  // namespace qsim
@@ -1941,5 +1957,357 @@ int qsim_cs_times(const qsim_cs* cs, double* out) {
 
 uint32_t qsim_cs_qubit_count(const qsim_cs* cs) { return cs ? cs->n : 0; }
 uint32_t qsim_cs_block_size(const qsim_cs* cs) { return cs ? cs->B : 0; }
+
+}  // extern "C"
+
+// ================================================================== BATCH
+// Many small circuits, each with its own observable, evaluated together:
+// energy <H> and the adjoint gradient of every instance.  Every instance is
+// planned once (forward passes, lambda = 2 H psi, reverse passes with
+// gradient points); the whole sequence is enqueued on the library stream
+// with no host synchronisation inside, captured once into a hipGraph and
+// replayed.  Work buffers are shared (instances run one after another inside
+// the graph); memory is cleared with fill kernels, not memset nodes.
+namespace qsim {
+struct BInst {
+  uint32_t n = 0;
+  uint64_t nparams = 0;
+  std::vector<HGate> gates;
+  std::vector<PauliGroup> G;
+  std::vector<PauliTermZ> T;
+  DBuf dG, dT;
+  Program fwd, rev;
+  std::vector<uint32_t> map_off, map_cnt;  // per reverse pass: offset/count into dmap
+  std::vector<int32_t> hmap;
+  DBuf dmap;
+  bool planned = false;
+};
+
+struct TuneChoice {
+  int fwd_engine = -1;  // -1: default choice, ENGINE_WAVE or ENGINE_LDS
+  uint32_t fwd_tb = 0;  // LDS tile bits when fwd_engine == ENGINE_LDS
+  uint32_t adj_tb = 0;  // 0: default
+};
+
+static void plan_instance(BInst& I, const TuneChoice& tc) {
+  std::vector<int> qmap(I.n);
+  std::iota(qmap.begin(), qmap.end(), 0);
+  std::vector<LGate> lg;
+  for (auto& g : I.gates) lg.push_back(lower_gate(g, qmap));
+  I.fwd.reset();
+  if (!lg.empty()) {
+    const bool wave = tc.fwd_engine == ENGINE_LDS ? false : wave_engine_ok(lg, I.n);
+    const uint32_t tb = tc.fwd_tb ? tc.fwd_tb : default_tile_bits();
+    plan_program_engine(I.fwd, lg, I.n, std::min(tb, I.n), std::min<uint32_t>(5, I.n), false, false, wave);
+    I.fwd.upload_all();
+  }
+  std::vector<LGate> rg;
+  for (size_t i = I.gates.size(); i-- > 0;) {
+    if (I.gates[i].parametric()) rg.push_back(grad_point(I.gates[i]));
+    rg.push_back(lower_gate(inverse_gate(I.gates[i]), qmap));
+  }
+  I.rev.reset();
+  I.map_off.clear();
+  I.map_cnt.clear();
+  I.hmap.clear();
+  if (!rg.empty()) {
+    const uint32_t tb = tc.adj_tb ? std::min(tc.adj_tb, I.n) : adjoint_tile_bits(I.n);
+    plan_program_engine(I.rev, rg, I.n, tb, std::min<uint32_t>(5, tb), false, false, false);
+    for (auto& a : I.rev.passes) {
+      I.map_off.push_back((uint32_t)I.hmap.size());
+      uint32_t cnt = 0;
+      for (uint32_t o = a.op_begin; o < a.op_end; ++o)
+        if (I.rev.ops[o].kind == OP_GRAD) {
+          I.hmap.push_back((int32_t)I.rev.ops[o].gidx);
+          I.rev.ops[o].gidx = cnt++;
+        }
+      I.map_cnt.push_back(cnt);
+    }
+    I.rev.upload_all();
+    upload(I.dmap, I.hmap.data(), std::max<size_t>(1, I.hmap.size()) * sizeof(int32_t));
+  }
+  upload(I.dG, I.G.data(), std::max<size_t>(1, I.G.size()) * sizeof(PauliGroup));
+  upload(I.dT, I.T.data(), std::max<size_t>(1, I.T.size()) * sizeof(PauliTermZ));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  I.planned = true;
+}
+}  // namespace qsim
+
+struct qsim_batch {
+  std::vector<std::unique_ptr<qsim::BInst>> inst;
+  std::map<uint32_t, qsim::TuneChoice> tune;
+  uint32_t maxn = 0;
+  uint64_t maxparams = 0;
+  qsim::DBuf psi, psiw, lam, part, ptout, acc;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ready = false;  // planned and buffers sized
+  ~qsim_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
+namespace qsim {
+static uint64_t acc_stride(const qsim_batch* b) { return 1 + b->maxparams; }
+
+// size the shared buffers for the planned instances
+static void batch_prepare(qsim_batch* b) {
+  uint64_t maxdim = 1, maxpart = 1, maxpts = 1;
+  for (auto& I : b->inst) {
+    if (!I->planned) plan_instance(*I, b->tune.count(I->n) ? b->tune[I->n] : TuneChoice{});
+    const uint64_t dim = 1ull << I->n;
+    maxdim = std::max(maxdim, dim);
+    maxpart = std::max<uint64_t>(maxpart, reduce_grid(dim));
+    for (size_t p = 0; p < I->rev.passes.size(); ++p) {
+      const uint64_t ntiles = dim >> I->rev.passes[p].t;
+      maxpart = std::max<uint64_t>(maxpart, (uint64_t)I->map_cnt[p] * ntiles);
+      maxpts = std::max<uint64_t>(maxpts, I->map_cnt[p]);
+    }
+  }
+  b->psi.ensure(maxdim * 16);
+  b->psiw.ensure(maxdim * 16);
+  b->lam.ensure(maxdim * 16);
+  b->part.ensure(maxpart * 16);
+  b->ptout.ensure(maxpts * 16);
+  b->acc.ensure(b->inst.size() * acc_stride(b) * 16);
+  b->ready = true;
+}
+
+// Enqueue instance i: psi = U|0>, lambda = 2 H psi with <H> into acc[i][0],
+// then the reverse sweep accumulating gradients into acc[i][1 + p].
+static void batch_enqueue(qsim_batch* b, size_t i) {
+  BInst& I = *b->inst[i];
+  const uint64_t dim = 1ull << I.n;
+  double2* psi = b->psi.as<double2>();
+  double* acc = b->acc.as<double>() + i * acc_stride(b) * 2;
+  HIPCHECK(launch_fill_basis(psi, dim, 0, stream()));
+  for (size_t p = 0; p < I.fwd.passes.size(); ++p) {
+    PassDyn dy{};
+    dy.raw = reinterpret_cast<double*>(psi);
+    HIPCHECK(launch_program_pass(I.fwd, p, IN_RAW, OUT_RAW, dy, 1, stream()));
+  }
+  const unsigned grid = reduce_grid(dim);
+  HIPCHECK(launch_pauli_sum(psi, b->lam.as<double2>(), dim, I.dG.as<PauliGroup>(), (uint32_t)I.G.size(),
+                            I.dT.as<PauliTermZ>(), b->part.as<double>(), grid, stream()));
+  HIPCHECK(launch_grad_reduce(b->part.as<double>(), grid, 1, acc, stream()));
+  HIPCHECK(hipMemcpyAsync(b->psiw.p, psi, dim * 16, hipMemcpyDeviceToDevice, stream()));
+  for (size_t p = 0; p < I.rev.passes.size(); ++p) {
+    const uint32_t tb = I.rev.passes[p].t;
+    const uint64_t ntiles = dim >> tb;
+    HIPCHECK(launch_adjoint_pass((int)tb, I.rev.dargs(p), b->psiw.as<double2>(), b->lam.as<double2>(),
+                                 b->part.as<double>(), ntiles, stream()));
+    const uint32_t np = I.map_cnt[p];
+    if (np == 0) continue;
+    HIPCHECK(launch_grad_reduce(b->part.as<double>(), ntiles, np, b->ptout.as<double>(), stream()));
+    HIPCHECK(launch_accumulate_points(b->ptout.as<double>(), np, I.dmap.as<int32_t>() + I.map_off[p], acc + 2,
+                                      stream()));
+  }
+}
+
+static void batch_enqueue_all(qsim_batch* b) {
+  HIPCHECK(launch_fill_u64(b->acc.as<unsigned long long>(), b->inst.size() * acc_stride(b) * 2, 0ull, stream()));
+  for (size_t i = 0; i < b->inst.size(); ++i) batch_enqueue(b, i);
+}
+
+static void batch_read(qsim_batch* b, double* energies, double* grads) {
+  std::vector<double> h(b->inst.size() * acc_stride(b) * 2);
+  HIPCHECK(hipMemcpyAsync(h.data(), b->acc.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  const uint64_t st = acc_stride(b);
+  for (size_t i = 0; i < b->inst.size(); ++i) {
+    const double* a = h.data() + i * st * 2;
+    if (energies) energies[i] = (a[0] + a[1]) / 2.0;
+    if (grads)
+      for (uint64_t p = 0; p < b->maxparams; ++p)
+        grads[i * b->maxparams + p] = p < b->inst[i]->nparams ? (a[2 + 2 * p] + a[3 + 2 * p]) / 2.0 : 0.0;
+  }
+}
+}  // namespace qsim
+
+extern "C" {
+
+int qsim_batch_create(qsim_batch** out) {
+  API_BEGIN
+  require(out != nullptr, "null argument");
+  *out = new qsim_batch();
+  API_END
+}
+
+int qsim_batch_destroy(qsim_batch* b) {
+  API_BEGIN
+  if (b) (void)hipStreamSynchronize(stream());
+  delete b;
+  API_END
+}
+
+int qsim_batch_add(qsim_batch* b, uint32_t n_qubits, const qsim_gate* gates, uint64_t n_gates, const double* pool,
+                   uint64_t pool_len, uint64_t n_terms, const uint64_t* x_masks, const uint64_t* z_masks,
+                   const uint32_t* y_counts, const double* coef_re, const double* coef_im, uint64_t n_params,
+                   uint64_t* index) {
+  API_BEGIN
+  require(b != nullptr, "null batch");
+  require(n_qubits >= 4 && n_qubits <= 34, "batch instances need 4..34 qubits");
+  auto I = std::make_unique<BInst>();
+  I->n = n_qubits;
+  I->nparams = n_params;
+  I->gates = parse_gates(gates, n_gates, pool, pool_len);
+  check_qubits(I->gates, n_qubits);
+  for (size_t i = 0; i < I->gates.size(); ++i) {
+    const HGate& g = I->gates[i];
+    if (!g.parametric()) continue;
+    if (g.param < 0 || (uint64_t)g.param >= n_params)
+      throw std::out_of_range("gate " + std::to_string(i) + ": parameter index out of range");
+    bool ok = g.c.empty() && !g.pauli.empty();
+    for (auto pid : g.pauli) ok = ok && pid >= 1 && pid <= 3;
+    if (!ok) throw std::invalid_argument("gate " + std::to_string(i) + " does not support backprop (not a Pauli rotation)");
+  }
+  build_pauli_groups(n_terms, x_masks, z_masks, y_counts, coef_re, coef_im, 2.0, I->G, I->T);
+  b->maxn = std::max(b->maxn, n_qubits);
+  b->maxparams = std::max(b->maxparams, n_params);
+  b->inst.push_back(std::move(I));
+  b->ready = false;
+  b->drop_graph();
+  if (index) *index = b->inst.size() - 1;
+  API_END
+}
+
+// Autotuner: for every qubit count in the batch, time the forward engine /
+// tile choices and the adjoint tile sizes on that count's first instance and
+// keep the fastest; later runs plan every instance of that count with it.
+int qsim_batch_tune(qsim_batch* b, int reps) {
+  API_BEGIN
+  require(b != nullptr, "null batch");
+  reps = std::max(1, reps);
+  std::map<uint32_t, size_t> first;
+  for (size_t i = 0; i < b->inst.size(); ++i)
+    if (!first.count(b->inst[i]->n)) first[b->inst[i]->n] = i;
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  for (auto& kv : first) {
+    const uint32_t n = kv.first;
+    BInst& I = *b->inst[kv.second];
+    std::vector<TuneChoice> fwd_c, adj_c;
+    {
+      TuneChoice w;
+      w.fwd_engine = ENGINE_WAVE;
+      fwd_c.push_back(w);
+      for (uint32_t tb : {10u, 12u}) {
+        TuneChoice l;
+        l.fwd_engine = ENGINE_LDS;
+        l.fwd_tb = tb;
+        fwd_c.push_back(l);
+      }
+      for (uint32_t tb : {8u, 9u, 10u, 11u, 12u})
+        if (tb <= n) {
+          TuneChoice a;
+          a.adj_tb = tb;
+          adj_c.push_back(a);
+        }
+    }
+    auto time_one = [&](const TuneChoice& tc) {
+      plan_instance(I, tc);
+      qsim_batch tmp;  // single-instance buffers; the instance itself is borrowed
+      tmp.maxparams = std::max<uint64_t>(1, I.nparams);
+      tmp.inst.emplace_back(&I);
+      float best = 1e30f;
+      try {
+        batch_prepare(&tmp);
+        batch_enqueue_all(&tmp);  // warm: kernel attributes, caches
+        for (int r = 0; r < reps; ++r) {
+          HIPCHECK(hipEventRecord(e0, stream()));
+          batch_enqueue_all(&tmp);
+          HIPCHECK(hipEventRecord(e1, stream()));
+          HIPCHECK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+          best = std::min(best, ms);
+        }
+      } catch (...) {
+        (void)tmp.inst.back().release();
+        throw;
+      }
+      (void)tmp.inst.back().release();
+      return best;
+    };
+    TuneChoice best{};
+    float bt = 1e30f;
+    for (auto& c : fwd_c) {
+      const float t = time_one(c);
+      if (t < bt) { bt = t; best.fwd_engine = c.fwd_engine; best.fwd_tb = c.fwd_tb; }
+    }
+    bt = 1e30f;
+    for (auto& c : adj_c) {
+      TuneChoice tc = best;
+      tc.adj_tb = c.adj_tb;
+      const float t = time_one(tc);
+      if (t < bt) { bt = t; best.adj_tb = c.adj_tb; }
+    }
+    b->tune[n] = best;
+  }
+  HIPCHECK(hipEventDestroy(e0));
+  HIPCHECK(hipEventDestroy(e1));
+  for (auto& I : b->inst) I->planned = false;
+  b->ready = false;
+  b->drop_graph();
+  API_END
+}
+
+int qsim_batch_config(const qsim_batch* b, uint32_t n_qubits, int32_t* fwd_engine, uint32_t* fwd_tile_bits,
+                      uint32_t* adj_tile_bits) {
+  API_BEGIN
+  require(b != nullptr, "null batch");
+  auto it = b->tune.find(n_qubits);
+  const TuneChoice tc = it == b->tune.end() ? TuneChoice{} : it->second;
+  if (fwd_engine) *fwd_engine = tc.fwd_engine;
+  if (fwd_tile_bits) *fwd_tile_bits = tc.fwd_tb;
+  if (adj_tile_bits) *adj_tile_bits = tc.adj_tb;
+  API_END
+}
+
+// energies[i] = <H_i>, grads[i * max_params + p] = d<H_i>/d theta_p.  With
+// use_graph the first call runs eagerly once, then captures the whole batch
+// into a hipGraph; later calls replay it.
+int qsim_batch_run(qsim_batch* b, int use_graph, double* energies, double* grads) {
+  API_BEGIN
+  require(b != nullptr, "null batch");
+  if (b->inst.empty()) return 0;
+  if (!b->ready) {
+    batch_prepare(b);
+    b->drop_graph();
+  }
+  if (!use_graph) {
+    batch_enqueue_all(b);
+  } else {
+    if (!b->exec) {
+      batch_enqueue_all(b);  // eager run: validates launches, sets kernel attributes
+      HIPCHECK(hipStreamSynchronize(stream()));
+      HIPCHECK(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
+      try {
+        batch_enqueue_all(b);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream(), &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIPCHECK(hipStreamEndCapture(stream(), &b->graph));
+      HIPCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIPCHECK(hipGraphLaunch(b->exec, stream()));
+  }
+  batch_read(b, energies, grads);
+  API_END
+}
+
+uint64_t qsim_batch_size(const qsim_batch* b) { return b ? b->inst.size() : 0; }
+uint64_t qsim_batch_max_params(const qsim_batch* b) { return b ? b->maxparams : 0; }
 
 }  // extern "C"

@@ -644,6 +644,21 @@ __global__ __launch_bounds__(kThreads) void grad_reduce_kernel(const double* __r
   }
 }
 
+// acc[map[k]] += pt[k] (compensated pairs) for the gradient points of one
+// pass, in order: several points of a shared parameter add up
+// deterministically.  One thread; a pass has at most a few hundred points.
+__global__ void accumulate_points_kernel(const double* __restrict__ pt, uint32_t np, const int32_t* __restrict__ map,
+                                         double* __restrict__ acc) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (uint32_t k = 0; k < np; ++k) {
+    double* a = acc + 2 * (uint64_t)map[k];
+    double s = a[0], c = a[1];
+    dd_merge(s, c, pt[2 * k], pt[2 * k + 1]);
+    a[0] = s;
+    a[1] = c;
+  }
+}
+
 // out[x] = sum_t c_t (P_t a)[x] for a Pauli sum in one sweep
 // (Observable::apply_to_state, src/cppsim/pauli_operator.cpp:132-183): terms
 // sharing a flip mask f share the load of a[x ^ f]; (P a)[x] = i^nY
@@ -697,8 +712,9 @@ static hipError_t launch_pass_tb(const PassArgs* d_args, const PassDyn& dy, uint
   if (IN == IN_DEC || OUT == OUT_ENC) lds += sizeof(CodecScratch);
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&pass_kernel<IN, OUT, TB, BIG>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pass_kernel<IN, OUT, TB, BIG>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
     attr_set = true;
   }
   PassDyn d = dy;
@@ -811,34 +827,44 @@ hipError_t launch_sv_apply_pauli(const double2* a, double2* out, uint64_t n, uin
 }
 
 
-// adjoint pass over 2^L amplitudes in tiles of 2^tb (tb in 4..11)
+// adjoint pass over 2^L amplitudes in tiles of 2^tb (tb in 4..12)
+template <int TB>
+static hipError_t launch_adjoint_tb(const PassArgs* d_args, double2* psi, double2* lam, double* partials,
+                                    uint64_t ntiles, hipStream_t st) {
+  const size_t shm = sizeof(double2) * ((2ull << TB) + kPhaseTab);
+  static bool attr = false;
+  if (!attr && shm > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&adjoint_pass_kernel<TB>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (e != hipSuccess) return e;
+  }
+  attr = true;
+  for (uint64_t t0 = 0; t0 < ntiles; t0 += kMaxLaunchBlocks) {
+    const unsigned cnt = (unsigned)std::min<uint64_t>(ntiles - t0, kMaxLaunchBlocks);
+    hipLaunchKernelGGL(adjoint_pass_kernel<TB>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_adjoint_pass(int tb, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
                                uint64_t ntiles, hipStream_t st) {
-  const uint64_t maxb = kMaxLaunchBlocks;
-  for (uint64_t t0 = 0; t0 < ntiles; t0 += maxb) {
-    const unsigned cnt = (unsigned)std::min<uint64_t>(ntiles - t0, maxb);
-    const size_t shm = sizeof(double2) * ((2ull << tb) + kPhaseTab);
-    switch (tb) {
-      case 4: hipLaunchKernelGGL(adjoint_pass_kernel<4>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
-      case 5: hipLaunchKernelGGL(adjoint_pass_kernel<5>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
-      case 6: hipLaunchKernelGGL(adjoint_pass_kernel<6>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
-      case 7: hipLaunchKernelGGL(adjoint_pass_kernel<7>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
-      case 8: hipLaunchKernelGGL(adjoint_pass_kernel<8>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
-      case 9: hipLaunchKernelGGL(adjoint_pass_kernel<9>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
-      case 10: hipLaunchKernelGGL(adjoint_pass_kernel<10>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0); break;
-      case 11: {
-        static bool attr = false;
-        if (!attr) {
-          (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&adjoint_pass_kernel<11>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-          attr = true;
-        }
-        hipLaunchKernelGGL(adjoint_pass_kernel<11>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0);
-        break;
-      }
-      default: return hipErrorInvalidValue;
-    }
+  switch (tb) {
+    case 4: return launch_adjoint_tb<4>(d_args, psi, lam, partials, ntiles, st);
+    case 5: return launch_adjoint_tb<5>(d_args, psi, lam, partials, ntiles, st);
+    case 6: return launch_adjoint_tb<6>(d_args, psi, lam, partials, ntiles, st);
+    case 7: return launch_adjoint_tb<7>(d_args, psi, lam, partials, ntiles, st);
+    case 8: return launch_adjoint_tb<8>(d_args, psi, lam, partials, ntiles, st);
+    case 9: return launch_adjoint_tb<9>(d_args, psi, lam, partials, ntiles, st);
+    case 10: return launch_adjoint_tb<10>(d_args, psi, lam, partials, ntiles, st);
+    case 11: return launch_adjoint_tb<11>(d_args, psi, lam, partials, ntiles, st);
+    case 12: return launch_adjoint_tb<12>(d_args, psi, lam, partials, ntiles, st);
+    default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_accumulate_points(const double* pt, uint32_t np, const int32_t* map, double* acc, hipStream_t st) {
+  if (np == 0) return hipSuccess;
+  hipLaunchKernelGGL(accumulate_points_kernel, dim3(1), dim3(64), 0, st, pt, np, map, acc);
   return hipGetLastError();
 }
 

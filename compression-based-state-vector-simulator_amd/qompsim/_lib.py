@@ -89,6 +89,14 @@ _SIGS = {
     "qsim_sv_expectation_pauli": (ctypes.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _P(_dbl), _P(_dbl)]),
     "qsim_sv_apply_pauli_sum": (ctypes.c_int, [_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
     "qsim_cs_expectation_pauli": (ctypes.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _P(_dbl), _P(_dbl)]),
+    "qsim_batch_create": (ctypes.c_int, [_P(_vp)]),
+    "qsim_batch_destroy": (ctypes.c_int, [_vp]),
+    "qsim_batch_add": (ctypes.c_int, [_vp, _u32, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _vp, _vp, _u64, _P(_u64)]),
+    "qsim_batch_tune": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "qsim_batch_config": (ctypes.c_int, [_vp, _u32, _P(ctypes.c_int32), _P(_u32), _P(_u32)]),
+    "qsim_batch_run": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+    "qsim_batch_size": (_u64, [_vp]),
+    "qsim_batch_max_params": (_u64, [_vp]),
     "qsim_sv_backprop": (ctypes.c_int, [_vp, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _vp, _vp, _u64, _P(_dbl), _vp]),
     "qsim_sv_device_ptr": (_vp, [_vp]),
     "qsim_sv_qubit_count": (_u32, [_vp]),

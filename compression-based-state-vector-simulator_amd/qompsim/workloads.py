@@ -112,6 +112,37 @@ def qaoa_maxcut(n, edges, gammas, betas):
     return c
 
 
+def qaoa_maxcut_shared(n, edges, gammas, betas):
+    """QAOA MaxCut with one parameter per layer angle: gate parameter index
+    2l holds RZZ(2 gamma_l) of every edge, 2l+1 RX(2 beta_l) of every qubit,
+    so gradients accumulate per shared parameter (d/dgamma_l = 2 * d/dangle)."""
+    c = ParametricQuantumCircuit(n)
+    for q in range(n):
+        c.add_H_gate(q)
+    for layer, (gm, bt) in enumerate(zip(gammas, betas)):
+        for (a, b) in edges:
+            c.add_parametric_multi_Pauli_rotation_gate([a, b], [3, 3], 2 * gm)
+            c.gate_list[-1]._param_index = 2 * layer
+        for q in range(n):
+            c.add_parametric_RX_gate(q, 2 * bt)
+            c.gate_list[-1]._param_index = 2 * layer + 1
+    return c
+
+
+def qaoa_batch(count=512, sizes=(12, 14, 16, 18, 20, 22, 24), p=4, seed=4):
+    """Seeded instances: random 3-regular graphs, node counts round-robin over
+    `sizes`, gamma, beta ~ U[0, pi)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = sizes[i % len(sizes)]
+        edges = random_3regular(n, rng)
+        gm = rng.uniform(0, np.pi, p)
+        bt = rng.uniform(0, np.pi, p)
+        out.append((n, edges, gm, bt))
+    return out
+
+
 def maxcut_observable(n, edges):
     obs = Observable(n)
     for (a, b) in edges:

@@ -189,6 +189,32 @@ uint32_t qsim_cs_block_size(const qsim_cs* cs);
 /* ---- synchronisation / timing helpers ---------------------------------- */
 int qsim_synchronize(void);
 
+/* ---- batched circuits --------------------------------------------------
+ * Many small circuits of uneven qubit counts, each with its own Hermitian
+ * Pauli-sum observable: energy and adjoint gradient of every instance (the
+ * semantics of qsim_sv_backprop from |0...0>).  qsim_batch_run plans every
+ * instance once and, with use_graph, captures the whole batch into one
+ * hipGraph on the first call and replays it afterwards.  qsim_batch_tune
+ * times the forward engine / tile choices and the adjoint tile sizes per qubit
+ * count and keeps the fastest.  No counterpart in the reference, which runs
+ * one circuit per call (the batch entry points the boundary adds for this
+ * path's batched workload). */
+typedef struct qsim_batch qsim_batch;
+int qsim_batch_create(qsim_batch** out);
+int qsim_batch_destroy(qsim_batch* b);
+int qsim_batch_add(qsim_batch* b, uint32_t n_qubits, const qsim_gate* gates, uint64_t n_gates, const double* pool,
+                   uint64_t pool_len, uint64_t n_terms, const uint64_t* x_masks, const uint64_t* z_masks,
+                   const uint32_t* y_counts, const double* coef_re, const double* coef_im, uint64_t n_params,
+                   uint64_t* index);
+int qsim_batch_tune(qsim_batch* b, int reps);
+/* fwd_engine: -1 default, 0 LDS tile (fwd_tile_bits), 1 wave tile */
+int qsim_batch_config(const qsim_batch* b, uint32_t n_qubits, int32_t* fwd_engine, uint32_t* fwd_tile_bits,
+                      uint32_t* adj_tile_bits);
+/* energies[i]; grads[i * qsim_batch_max_params(b) + p] */
+int qsim_batch_run(qsim_batch* b, int use_graph, double* energies, double* grads);
+uint64_t qsim_batch_size(const qsim_batch* b);
+uint64_t qsim_batch_max_params(const qsim_batch* b);
+
 #ifdef __cplusplus
 }
 #endif
