@@ -47,6 +47,8 @@ hipError_t launch_adjoint_pass(int tb, const PassArgs* d_args, double2* psi, dou
                                uint64_t ntiles, hipStream_t st);
 hipError_t launch_grad_reduce(const double* partials, uint64_t nblk, uint32_t npoints, double* out, hipStream_t st);
 hipError_t launch_accumulate_points(const double* pt, uint32_t np, const int32_t* map, double* acc, hipStream_t st);
+hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
+                               uint64_t ntiles, hipStream_t st);
 hipError_t launch_pauli_sum(const double2* a, double2* out, uint64_t n, const PauliGroup* groups, uint32_t ng,
                             const PauliTermZ* terms, double* partials, unsigned grid, hipStream_t st);
 hipError_t launch_block_bytes(const uint64_t* desc, const uint8_t* nullf, uint64_t nb, uint64_t cpb,
@@ -588,9 +590,9 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
 // c >= kChunkAmpsLog2 (codec chunks are 32 contiguous amplitudes); an
 // op-free pass is added at either end when needed.
 static void plan_program_engine(Program& P, const std::vector<LGate>& lg, uint32_t L, uint32_t tmax, uint32_t cmin,
-                                bool at_least_one_pass, bool codec_ends, bool wave) {
+                                bool at_least_one_pass, bool codec_ends, bool wave, uint32_t wave_bits = kWaveTile) {
   P.L = L;
-  const uint32_t t = wave ? kWaveTile : std::min(tmax, L);
+  const uint32_t t = wave ? wave_bits : std::min(tmax, L);
   if (wave) cmin = 6;
   cmin = std::min(cmin, t);
   // Commutation-aware greedy scheduling: a pass scans the remaining gates in
@@ -661,7 +663,7 @@ static void plan_program_engine(Program& P, const std::vector<LGate>& lg, uint32
       for (uint32_t q = 6; q < pp.c; ++q) E.push_back(q);
       E.insert(E.end(), pp.E.begin(), pp.E.end());
       std::sort(E.begin(), E.end());
-      if (E.size() != kWaveTile - 6) throw std::logic_error("planner: wave tile needs four register qubits");
+      if (E.size() != wave_bits - 6) throw std::logic_error("planner: wave tile register qubit count");
       pp.c = 6;
       pp.E = E;
     }
@@ -711,7 +713,10 @@ static void plan_program(Program& P, const std::vector<LGate>& lg, uint32_t L, u
 static hipError_t launch_program_pass(const Program& P, size_t i, int in_mode, int out_mode, const PassDyn& dy,
                                       uint64_t groups, hipStream_t st) {
   const PassArgs& a = P.passes[i];
-  if (a.engine == ENGINE_WAVE) return launch_wave(in_mode, out_mode, P.dargs(i), dy, groups << (P.L - kWaveTile), st);
+  if (a.engine == ENGINE_WAVE) {
+    if (a.t != kWaveTile) return hipErrorInvalidValue;
+    return launch_wave(in_mode, out_mode, P.dargs(i), dy, groups << (P.L - kWaveTile), st);
+  }
   return launch_pass(in_mode, out_mode, (int)a.t, a.maxk > 3, P.dargs(i), dy, groups << (P.L - a.t), st);
 }
 
@@ -1494,6 +1499,36 @@ static LGate grad_point(const HGate& g) {
   return L;
 }
 
+// Reverse programs run on the wave-tile engine when its shape fits (at least
+// 10 qubits, gates and gradient points on at most two X/Y qubits;
+// QOMPSIM_ADJ_ENGINE=lds forces the LDS-tile engine), else on LDS tiles of
+// 2^tb amplitudes.
+static bool adjoint_wave_ok(const std::vector<LGate>& lg, uint32_t n) {
+  const char* e = std::getenv("QOMPSIM_ADJ_ENGINE");
+  if (e && std::string(e) == "lds") return false;
+  return wave_engine_ok(lg, n);
+}
+
+// The wave form holds psi and lambda in registers: 8 rows per state
+// (512-amplitude tiles, three register qubits) keep it at two waves per SIMD.
+constexpr uint32_t kAdjWaveTile = 9;
+
+static void plan_reverse(Program& P, const std::vector<LGate>& lg, uint32_t n, uint32_t tb, bool wave) {
+  if (wave) plan_program_engine(P, lg, n, kAdjWaveTile, 6, false, false, true, kAdjWaveTile);
+  else plan_program_engine(P, lg, n, tb, std::min<uint32_t>(5, tb), false, false, false);
+}
+
+// tiles of pass p of a reverse program over 2^n amplitudes
+static uint64_t reverse_tiles(const Program& P, size_t p, uint32_t n) { return (1ull << n) >> P.passes[p].t; }
+
+static hipError_t launch_reverse_pass(const Program& P, size_t p, uint32_t n, double2* psi, double2* lam,
+                                      double* partials) {
+  const uint64_t ntiles = reverse_tiles(P, p, n);
+  if (P.passes[p].engine == ENGINE_WAVE)
+    return launch_wave_adjoint((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, stream());
+  return launch_adjoint_pass((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, stream());
+}
+
 static uint32_t adjoint_tile_bits(uint32_t n) {
   uint32_t tb = 10;
   if (const char* e = std::getenv("QOMPSIM_ADJ_TILE_BITS")) tb = (uint32_t)std::atoi(e);
@@ -1569,7 +1604,7 @@ int qsim_sv_backprop(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, cons
   if (!lg.empty()) {
     const uint32_t tb = adjoint_tile_bits(sv->n);
     Program P;
-    plan_program_engine(P, lg, sv->n, tb, std::min<uint32_t>(5, tb), false, false, false);
+    plan_reverse(P, lg, sv->n, tb, adjoint_wave_ok(lg, sv->n));
     // renumber gradient points per pass
     std::vector<std::vector<int32_t>> params(P.passes.size());
     uint32_t maxpts = 0;
@@ -1582,16 +1617,18 @@ int qsim_sv_backprop(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, cons
       maxpts = std::max<uint32_t>(maxpts, (uint32_t)params[pi].size());
     }
     P.upload_all();
-    const uint64_t ntiles = sv->dim >> tb;
-    sv->adj_part.ensure(std::max<uint64_t>(1, (uint64_t)maxpts * ntiles) * 2 * sizeof(double));
+    uint64_t maxtiles = 1;
+    for (size_t pi = 0; pi < P.passes.size(); ++pi) maxtiles = std::max(maxtiles, reverse_tiles(P, pi, sv->n));
+    sv->adj_part.ensure(std::max<uint64_t>(1, (uint64_t)maxpts * maxtiles) * 2 * sizeof(double));
     sv->adj_out.ensure(std::max<uint32_t>(1, maxpts) * 2 * sizeof(double));
     std::vector<double> hout(2 * std::max<uint32_t>(1, maxpts));
     for (size_t pi = 0; pi < P.passes.size(); ++pi) {
-      HIPCHECK(launch_adjoint_pass((int)tb, P.dargs(pi), sv->adj_psi.as<double2>(), sv->adj_lam.as<double2>(),
-                                   sv->adj_part.as<double>(), ntiles, stream()));
+      HIPCHECK(launch_reverse_pass(P, pi, sv->n, sv->adj_psi.as<double2>(), sv->adj_lam.as<double2>(),
+                                   sv->adj_part.as<double>()));
       const uint32_t np = (uint32_t)params[pi].size();
       if (np == 0) continue;
-      HIPCHECK(launch_grad_reduce(sv->adj_part.as<double>(), ntiles, np, sv->adj_out.as<double>(), stream()));
+      HIPCHECK(launch_grad_reduce(sv->adj_part.as<double>(), reverse_tiles(P, pi, sv->n), np,
+                                  sv->adj_out.as<double>(), stream()));
       HIPCHECK(hipMemcpyAsync(hout.data(), sv->adj_out.p, np * 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
       HIPCHECK(hipStreamSynchronize(stream()));
       for (uint32_t k = 0; k < np; ++k) gacc[params[pi][k]].merge(hout[2 * k], hout[2 * k + 1]);
@@ -1987,6 +2024,7 @@ struct TuneChoice {
   int fwd_engine = -1;  // -1: default choice, ENGINE_WAVE or ENGINE_LDS
   uint32_t fwd_tb = 0;  // LDS tile bits when fwd_engine == ENGINE_LDS
   uint32_t adj_tb = 0;  // 0: default
+  int adj_engine = -1;  // -1: default choice, ENGINE_WAVE or ENGINE_LDS
 };
 
 static void plan_instance(BInst& I, const TuneChoice& tc) {
@@ -2012,7 +2050,8 @@ static void plan_instance(BInst& I, const TuneChoice& tc) {
   I.hmap.clear();
   if (!rg.empty()) {
     const uint32_t tb = tc.adj_tb ? std::min(tc.adj_tb, I.n) : adjoint_tile_bits(I.n);
-    plan_program_engine(I.rev, rg, I.n, tb, std::min<uint32_t>(5, tb), false, false, false);
+    const bool wave = tc.adj_engine == ENGINE_LDS ? false : adjoint_wave_ok(rg, I.n);
+    plan_reverse(I.rev, rg, I.n, tb, wave);
     for (auto& a : I.rev.passes) {
       I.map_off.push_back((uint32_t)I.hmap.size());
       uint32_t cnt = 0;
@@ -2066,7 +2105,7 @@ static void batch_prepare(qsim_batch* b) {
     maxdim = std::max(maxdim, dim);
     maxpart = std::max<uint64_t>(maxpart, reduce_grid(dim));
     for (size_t p = 0; p < I->rev.passes.size(); ++p) {
-      const uint64_t ntiles = dim >> I->rev.passes[p].t;
+      const uint64_t ntiles = reverse_tiles(I->rev, p, I->n);
       maxpart = std::max<uint64_t>(maxpart, (uint64_t)I->map_cnt[p] * ntiles);
       maxpts = std::max<uint64_t>(maxpts, I->map_cnt[p]);
     }
@@ -2099,10 +2138,8 @@ static void batch_enqueue(qsim_batch* b, size_t i) {
   HIPCHECK(launch_grad_reduce(b->part.as<double>(), grid, 1, acc, stream()));
   HIPCHECK(hipMemcpyAsync(b->psiw.p, psi, dim * 16, hipMemcpyDeviceToDevice, stream()));
   for (size_t p = 0; p < I.rev.passes.size(); ++p) {
-    const uint32_t tb = I.rev.passes[p].t;
-    const uint64_t ntiles = dim >> tb;
-    HIPCHECK(launch_adjoint_pass((int)tb, I.rev.dargs(p), b->psiw.as<double2>(), b->lam.as<double2>(),
-                                 b->part.as<double>(), ntiles, stream()));
+    const uint64_t ntiles = reverse_tiles(I.rev, p, I.n);
+    HIPCHECK(launch_reverse_pass(I.rev, p, I.n, b->psiw.as<double2>(), b->lam.as<double2>(), b->part.as<double>()));
     const uint32_t np = I.map_cnt[p];
     if (np == 0) continue;
     HIPCHECK(launch_grad_reduce(b->part.as<double>(), ntiles, np, b->ptout.as<double>(), stream()));
@@ -2209,8 +2246,15 @@ int qsim_batch_tune(qsim_batch* b, int reps) {
         if (tb <= n) {
           TuneChoice a;
           a.adj_tb = tb;
+          a.adj_engine = ENGINE_LDS;
           adj_c.push_back(a);
         }
+      if (n >= kWaveTile) {
+        TuneChoice a;
+        a.adj_tb = kAdjWaveTile;
+        a.adj_engine = ENGINE_WAVE;
+        adj_c.push_back(a);
+      }
     }
     auto time_one = [&](const TuneChoice& tc) {
       plan_instance(I, tc);
@@ -2244,11 +2288,13 @@ int qsim_batch_tune(qsim_batch* b, int reps) {
       if (t < bt) { bt = t; best.fwd_engine = c.fwd_engine; best.fwd_tb = c.fwd_tb; }
     }
     bt = 1e30f;
+    TuneChoice fbest = best;
     for (auto& c : adj_c) {
-      TuneChoice tc = best;
+      TuneChoice tc = fbest;
       tc.adj_tb = c.adj_tb;
+      tc.adj_engine = c.adj_engine;
       const float t = time_one(tc);
-      if (t < bt) { bt = t; best.adj_tb = c.adj_tb; }
+      if (t < bt) { bt = t; best.adj_tb = c.adj_tb; best.adj_engine = c.adj_engine; }
     }
     b->tune[n] = best;
   }
@@ -2261,13 +2307,14 @@ int qsim_batch_tune(qsim_batch* b, int reps) {
 }
 
 int qsim_batch_config(const qsim_batch* b, uint32_t n_qubits, int32_t* fwd_engine, uint32_t* fwd_tile_bits,
-                      uint32_t* adj_tile_bits) {
+                      int32_t* adj_engine, uint32_t* adj_tile_bits) {
   API_BEGIN
   require(b != nullptr, "null batch");
   auto it = b->tune.find(n_qubits);
   const TuneChoice tc = it == b->tune.end() ? TuneChoice{} : it->second;
   if (fwd_engine) *fwd_engine = tc.fwd_engine;
   if (fwd_tile_bits) *fwd_tile_bits = tc.fwd_tb;
+  if (adj_engine) *adj_engine = tc.adj_engine;
   if (adj_tile_bits) *adj_tile_bits = tc.adj_tb;
   API_END
 }

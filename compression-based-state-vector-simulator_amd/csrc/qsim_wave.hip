@@ -246,7 +246,8 @@ __device__ __forceinline__ uint32_t wave_sel(const QC Op& op, const WaveCtx& cx,
 }
 
 // one target on lane bit b
-__device__ __forceinline__ void k1_lane(const QC Op& op, double2 (&amp)[kV], const WaveCtx& cx, const QC double* __restrict__ C) {
+template <int KV>
+__device__ __forceinline__ void k1_lane(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
   const uint32_t b = op.tpos[0];
   const uint32_t r = ((uint32_t)cx.lane >> b) & 1u;
   const int m = 1 << b;
@@ -254,7 +255,7 @@ __device__ __forceinline__ void k1_lane(const QC Op& op, double2 (&amp)[kV], con
     if (op.ident_mask & 1ull) return;
     const double2 md = ldc(C, op.mat_off + 3u * r), mo = ldc(C, op.mat_off + 1u + r);
 #pragma unroll
-    for (int v = 0; v < kV; ++v) {
+    for (int v = 0; v < KV; ++v) {
       const double2 p = shfl_xor2(amp[v], m);
       amp[v] = mac2(md, amp[v], mo, p);
       QSIM_ITER_FENCE();
@@ -262,7 +263,7 @@ __device__ __forceinline__ void k1_lane(const QC Op& op, double2 (&amp)[kV], con
     return;
   }
 #pragma unroll
-  for (int v = 0; v < kV; ++v) {
+  for (int v = 0; v < KV; ++v) {
     QSIM_ITER_FENCE();
     const double2 p = shfl_xor2(amp[v], m);
     const uint32_t sel = wave_sel(op, cx, v);
@@ -273,11 +274,11 @@ __device__ __forceinline__ void k1_lane(const QC Op& op, double2 (&amp)[kV], con
 }
 
 // one target on register slot J
-template <int J>
-__device__ __forceinline__ void k1_reg(const QC Op& op, double2 (&amp)[kV], const WaveCtx& cx, const QC double* __restrict__ C) {
+template <int J, int KV>
+__device__ __forceinline__ void k1_reg(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
   constexpr int MJ = 1 << J;
 #pragma unroll
-  for (int v = 0; v < kV; ++v) {
+  for (int v = 0; v < KV; ++v) {
     if (v & MJ) continue;
     QSIM_ITER_FENCE();
     const uint32_t sel = op.nsel ? wave_sel(op, cx, v) : 0u;
@@ -290,11 +291,11 @@ __device__ __forceinline__ void k1_reg(const QC Op& op, double2 (&amp)[kV], cons
 }
 
 // two targets on register slots J0 (matrix bit 0) and J1 (matrix bit 1)
-template <int J0, int J1>
-__device__ __forceinline__ void k2_regreg(const QC Op& op, double2 (&amp)[kV], const WaveCtx& cx, const QC double* __restrict__ C) {
+template <int J0, int J1, int KV>
+__device__ __forceinline__ void k2_regreg(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
   constexpr int M0 = 1 << J0, M1 = 1 << J1;
 #pragma unroll
-  for (int v = 0; v < kV; ++v) {
+  for (int v = 0; v < KV; ++v) {
     if (v & (M0 | M1)) continue;
     QSIM_ITER_FENCE();
     const uint32_t sel = op.nsel ? wave_sel(op, cx, v) : 0u;
@@ -320,15 +321,15 @@ __device__ __forceinline__ void k2_regreg(const QC Op& op, double2 (&amp)[kV], c
 }
 
 // two targets: lane bit b (matrix bit il) and register slot J (matrix bit 1-il)
-template <int J>
-__device__ __forceinline__ void k2_lanereg(const QC Op& op, double2 (&amp)[kV], const WaveCtx& cx, const QC double* __restrict__ C,
+template <int J, int KV>
+__device__ __forceinline__ void k2_lanereg(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C,
                                            uint32_t b, uint32_t il) {
   constexpr int MJ = 1 << J;
   const uint32_t ir = 1u - il;
   const uint32_t rl = ((uint32_t)cx.lane >> b) & 1u;
   const int m = 1 << b;
 #pragma unroll
-  for (int v = 0; v < kV; ++v) {
+  for (int v = 0; v < KV; ++v) {
     if (v & MJ) continue;
     QSIM_ITER_FENCE();
     const double2 a0 = amp[v], a1 = amp[v | MJ];
@@ -356,11 +357,12 @@ __device__ __forceinline__ void k2_lanereg(const QC Op& op, double2 (&amp)[kV], 
 }
 
 // two targets on lane bits b0 (matrix bit 0) and b1 (matrix bit 1)
-__device__ __forceinline__ void k2_lanelane(const QC Op& op, double2 (&amp)[kV], const WaveCtx& cx, const QC double* __restrict__ C) {
+template <int KV>
+__device__ __forceinline__ void k2_lanelane(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
   const uint32_t b0 = op.tpos[0], b1 = op.tpos[1];
   const uint32_t r = (((uint32_t)cx.lane >> b0) & 1u) | ((((uint32_t)cx.lane >> b1) & 1u) << 1);
 #pragma unroll
-  for (int v = 0; v < kV; ++v) {
+  for (int v = 0; v < KV; ++v) {
     QSIM_ITER_FENCE();
     const double2 x1 = shfl_xor2(amp[v], 1 << b0);
     const double2 x2 = shfl_xor2(amp[v], 1 << b1);
@@ -380,7 +382,8 @@ __device__ __forceinline__ void k2_lanelane(const QC Op& op, double2 (&amp)[kV],
   }
 }
 
-__device__ __forceinline__ void wave_dense(const QC Op& op, double2 (&amp)[kV], const WaveCtx& cx, const QC double* __restrict__ C) {
+template <int KV>
+__device__ __forceinline__ void wave_dense(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
   if (op.k == 1) {
     const uint32_t t = op.tpos[0];
     if (t < 6) k1_lane(op, amp, cx, C);
@@ -427,7 +430,8 @@ __device__ __forceinline__ void wave_dense(const QC Op& op, double2 (&amp)[kV], 
 // into a per-lane factor of that slot; the element's phase is the product of
 // the lane factor and the factors of its set slot bits.  Terms on two or more
 // slots are summed per element.
-__device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[kV], const WaveCtx& cx,
+template <int KV>
+__device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx,
                                           const QC PhaseTerm* __restrict__ terms, const QC DiagTable* __restrict__ diags,
                                           const QC double* __restrict__ C) {
   const uint32_t lane = (uint32_t)cx.lane;
@@ -457,33 +461,33 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[kV], c
     if (used & 16u) {
       const double2 F = cis_reduced(aL);
 #pragma unroll
-      for (int v = 0; v < kV; ++v) amp[v] = cmul(amp[v], F);
+      for (int v = 0; v < KV; ++v) amp[v] = cmul(amp[v], F);
     }
     __builtin_amdgcn_sched_barrier(0);
     if (used & 1u) {
       const double2 F = cis_reduced(aF0);
 #pragma unroll
-      for (int v = 1; v < kV; v += 2) amp[v] = cmul(amp[v], F);
+      for (int v = 1; v < KV; v += 2) amp[v] = cmul(amp[v], F);
     }
     __builtin_amdgcn_sched_barrier(0);
     if (used & 2u) {
       const double2 F = cis_reduced(aF1);
 #pragma unroll
-      for (int v = 0; v < kV; ++v)
+      for (int v = 0; v < KV; ++v)
         if (v & 2) amp[v] = cmul(amp[v], F);
     }
     __builtin_amdgcn_sched_barrier(0);
     if (used & 4u) {
       const double2 F = cis_reduced(aF2);
 #pragma unroll
-      for (int v = 0; v < kV; ++v)
+      for (int v = 0; v < KV; ++v)
         if (v & 4) amp[v] = cmul(amp[v], F);
     }
     __builtin_amdgcn_sched_barrier(0);
     if (used & 8u) {
       const double2 F = cis_reduced(aF3);
 #pragma unroll
-      for (int v = 8; v < kV; ++v) amp[v] = cmul(amp[v], F);
+      for (int v = 8; v < KV; ++v) amp[v] = cmul(amp[v], F);
     }
     __builtin_amdgcn_sched_barrier(0);
     if (generic) {
@@ -498,7 +502,7 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[kV], c
         const uint32_t lm = T.tmask & 63u, lv = T.tval & 63u;
         const double2 f = ((lane & lm) == lv) ? cis_reduced(T.angle) : make_double2(1.0, 0.0);
 #pragma unroll
-        for (int v = 0; v < kV; ++v)
+        for (int v = 0; v < KV; ++v)
           if (((uint32_t)v & rm) == rv) amp[v] = cmul(amp[v], f);
       }
     }
@@ -522,7 +526,7 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[kV], c
     (void)slot_pos;
     const uint64_t cbase = Dt.coef_off + idx0;
 #pragma unroll
-    for (int v = 0; v < kV; ++v) {
+    for (int v = 0; v < KV; ++v) {
       QSIM_ITER_FENCE();
       const uint32_t sidx = ((v & 1) ? slot_bit[0] : 0u) | ((v & 2) ? slot_bit[1] : 0u) |
                             ((v & 4) ? slot_bit[2] : 0u) | ((v & 8) ? slot_bit[3] : 0u);
@@ -691,6 +695,176 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
       desc_out[(uint64_t)mbid * a.cpb + mci] = desc_pack(0, 0, 0);
     }
   }
+}
+
+// --------------------------------------------------- adjoint reverse sweep
+// Re(i^kr * conj(l) * x * sign) for one element of a gradient point
+__device__ __forceinline__ double grad_term(double2 l, double2 x, bool neg, int kr) {
+  double re = l.x * x.x + l.y * x.y, im = l.x * x.y - l.y * x.x;
+  re = neg ? -re : re;
+  im = neg ? -im : im;
+  return kr == 0 ? re : kr == 1 ? -im : kr == 2 ? -re : im;
+}
+__device__ __forceinline__ void dd_acc(double& s, double& c, double v) {
+  const double t = s + v, bp = t - s;
+  c += (s - (t - bp)) + (v - bp);
+  s = t;
+}
+// Pauli partner on register qubits only: psi row v pairs with row v ^ M;
+// lambda lives in rows kV/2 .. kV-1
+template <int M>
+__device__ __forceinline__ void grad_rows(const double2 (&amp)[kV], uint32_t zt, bool bneg, int kr, int lane,
+                                          double& s, double& c) {
+  constexpr int R = kV / 2;
+#pragma unroll
+  for (int v = 0; v < R; ++v) {
+    const uint32_t q = ((uint32_t)(v ^ M) << 6) | (uint32_t)lane;
+    dd_acc(s, c, grad_term(amp[R + v], amp[v ^ M], bneg ^ (bool)(__popc(q & zt) & 1), kr));
+  }
+}
+// one DPP move of a double (both halves)
+template <int CTRL, int RMASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, RMASK, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, RMASK, 0xf, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+// wave64 sum through DPP row shifts and row broadcasts; lane 63 ends with
+// the total
+__device__ __forceinline__ double wave_sum_f64(double x) {
+  x += dpp_f64<0x111, 0xf>(x);  // row_shr:1
+  x += dpp_f64<0x112, 0xf>(x);  // row_shr:2
+  x += dpp_f64<0x114, 0xf>(x);  // row_shr:4
+  x += dpp_f64<0x118, 0xf>(x);  // row_shr:8
+  x += dpp_f64<0x142, 0xa>(x);  // row_bcast:15
+  x += dpp_f64<0x143, 0xc>(x);  // row_bcast:31
+  return x;
+}
+
+// Wave-tile form of the adjoint pass (ParametricQuantumCircuit::
+// backprop_inner_product, src/vqcsim/parametric_circuit.cpp:180-279): the
+// wave holds the same 1024-amplitude tile of psi and lambda in registers and
+// applies every dense / elementwise op of the pass (the inverted gates) to
+// both.  An OP_GRAD op stages psi in the wave's LDS buffer, reads the Pauli
+// partner of each element, and adds Re sum conj(lambda) (i P psi) over the
+// tile, compensated, to partials[(gidx * ntiles + tile) * 2 + {0, 1}].
+// The tile is 512 amplitudes of one state: lanes on qubits 0..5 and three
+// register qubits E[0..2].  Register rows 0..7 hold psi and rows 8..15 the
+// same positions of lambda, so the fourth row bit selects the state and no
+// gate ever acts on it: every dense / elementwise op runs once over all 16
+// rows with the forward kernel's code and register footprint, sharing
+// coefficient loads and phase factors between psi and lambda.
+__global__ __launch_bounds__(kThreads, 2) void wave_adjoint_kernel(const PassArgs* __restrict__ pa,
+                                                                   double2* __restrict__ psi,
+                                                                   double2* __restrict__ lam,
+                                                                   double* __restrict__ partials, uint64_t tile0,
+                                                                   uint64_t ntiles) {
+  constexpr int R = kV / 2;  // rows per state
+  __shared__ __align__(16) double2 xbuf[kThreads / 64][4 * 128 + 32];
+  const QC PassArgs& a = *(const QC PassArgs*)(pa);
+  const QC Op* ops = (const QC Op*)(a.ops);
+  const QC PhaseTerm* terms = (const QC PhaseTerm*)(a.terms);
+  const QC DiagTable* diags = (const QC DiagTable*)(a.diags);
+  const QC double* coefs = (const QC double*)(a.coefs);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t wt = tile0 + (uint64_t)blockIdx.x * (kThreads / 64) + wave;
+  if (wt >= ntiles) return;
+  const TileMapConst tm{9u, 6u, 3u, a.E};
+  const uint64_t base = tm.base(wt);
+  const uint64_t e0 = 1ull << a.E[0], e1 = 1ull << a.E[1], e2 = 1ull << a.E[2];
+  auto roff = [&](int v) -> uint64_t { return ((v & 1) ? e0 : 0ull) | ((v & 2) ? e1 : 0ull) | ((v & 4) ? e2 : 0ull); };
+  const WaveCtx cx{lane, base};
+  double2 amp[kV];
+  {
+    const QG double* sp = (const QG double*)(psi) + 2 * (base + lane);
+    const QG double* sl = (const QG double*)(lam) + 2 * (base + lane);
+#pragma unroll
+    for (int v = 0; v < R; ++v) {
+      amp[v] = make_double2(sp[2 * roff(v)], sp[2 * roff(v) + 1]);
+      amp[R + v] = make_double2(sl[2 * roff(v)], sl[2 * roff(v) + 1]);
+    }
+  }
+  double2* xb = xbuf[wave];
+  for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
+    const QC Op& op = ops[o];
+    if (op.kind == OP_DENSE) {
+      wave_dense(op, amp, cx, coefs);
+    } else if (op.kind == OP_ELEM) {
+      wave_elem(op, amp, cx, terms, diags, coefs);
+    } else {
+      const uint32_t fx = (uint32_t)op.gfx, zt = (uint32_t)op.gzt;
+      const int kr = (int)((1u + op.nY) & 3u);  // i * i^nY = i^kr
+      const bool bneg = __popcll(base & op.gzb) & 1;
+      double s = 0.0, c = 0.0;
+      if (fx == 0u) {
+#pragma unroll
+        for (int v = 0; v < R; ++v) {
+          const uint32_t q = ((uint32_t)v << 6) | (uint32_t)lane;
+          dd_acc(s, c, grad_term(amp[R + v], amp[v], bneg ^ (bool)(__popc(q & zt) & 1), kr));
+        }
+      } else if ((fx & 63u) == 0u) {
+        switch (fx >> 6) {
+          case 1: grad_rows<1>(amp, zt, bneg, kr, lane, s, c); break;
+          case 2: grad_rows<2>(amp, zt, bneg, kr, lane, s, c); break;
+          case 3: grad_rows<3>(amp, zt, bneg, kr, lane, s, c); break;
+          case 4: grad_rows<4>(amp, zt, bneg, kr, lane, s, c); break;
+          case 5: grad_rows<5>(amp, zt, bneg, kr, lane, s, c); break;
+          case 6: grad_rows<6>(amp, zt, bneg, kr, lane, s, c); break;
+          default: grad_rows<7>(amp, zt, bneg, kr, lane, s, c); break;
+        }
+      } else if (fx < 64u) {
+#pragma unroll
+        for (int v = 0; v < R; ++v) {
+          QSIM_ITER_FENCE();
+          const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
+          dd_acc(s, c, grad_term(amp[R + v], shfl_xor2(amp[v], (int)fx), bneg ^ (bool)(__popc(q & zt) & 1), kr));
+        }
+      } else {
+        // flips on lanes and register qubits: psi through the wave's LDS
+        // buffer (position 64v + l at slot 68v + l + l/16)
+#pragma unroll
+        for (int v = 0; v < R; ++v) xb[68 * v + lane + (lane >> 4)] = amp[v];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int v = 0; v < R; ++v) {
+          const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
+          dd_acc(s, c, grad_term(amp[R + v], xb[q + (q >> 4)], bneg ^ (bool)(__popc(q & zt) & 1), kr));
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      // tile sum: the lanes' compensated sums added in a DPP tree
+      const double tot = __shfl(wave_sum_f64(s + c), 63, 64);
+      if (lane == 0) {
+        partials[((uint64_t)op.gidx * ntiles + wt) * 2] = tot;
+        partials[((uint64_t)op.gidx * ntiles + wt) * 2 + 1] = 0.0;
+      }
+    }
+  }
+  {
+    QG double* dp = (QG double*)(psi) + 2 * (base + lane);
+    QG double* dl = (QG double*)(lam) + 2 * (base + lane);
+#pragma unroll
+    for (int v = 0; v < R; ++v) {
+      dp[2 * roff(v)] = amp[v].x;
+      dp[2 * roff(v) + 1] = amp[v].y;
+      dl[2 * roff(v)] = amp[R + v].x;
+      dl[2 * roff(v) + 1] = amp[R + v].y;
+    }
+  }
+}
+
+// ntiles counts wave tiles of 2^9 amplitudes of one state
+hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
+                               uint64_t ntiles, hipStream_t st) {
+  constexpr uint64_t per_block = kThreads / 64;
+  if (tile_bits != 9) return hipErrorInvalidValue;
+  for (uint64_t t0 = 0; t0 < ntiles; t0 += kMaxLaunchBlocks * per_block) {
+    const uint64_t cnt = std::min<uint64_t>(ntiles - t0, kMaxLaunchBlocks * per_block);
+    hipLaunchKernelGGL(wave_adjoint_kernel, dim3((unsigned)((cnt + per_block - 1) / per_block)), dim3(kThreads), 0,
+                       st, d_args, psi, lam, partials, t0, ntiles);
+  }
+  return hipGetLastError();
 }
 
 template <int IN, int OUT>

@@ -93,7 +93,7 @@ _SIGS = {
     "qsim_batch_destroy": (ctypes.c_int, [_vp]),
     "qsim_batch_add": (ctypes.c_int, [_vp, _u32, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _vp, _vp, _u64, _P(_u64)]),
     "qsim_batch_tune": (ctypes.c_int, [_vp, ctypes.c_int]),
-    "qsim_batch_config": (ctypes.c_int, [_vp, _u32, _P(ctypes.c_int32), _P(_u32), _P(_u32)]),
+    "qsim_batch_config": (ctypes.c_int, [_vp, _u32, _P(ctypes.c_int32), _P(_u32), _P(ctypes.c_int32), _P(_u32)]),
     "qsim_batch_run": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "qsim_batch_size": (_u64, [_vp]),
     "qsim_batch_max_params": (_u64, [_vp]),

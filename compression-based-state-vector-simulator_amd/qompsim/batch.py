@@ -41,10 +41,12 @@ class BatchedAdjoint:
         call("qsim_batch_tune", self._h, int(reps))
 
     def config(self, n):
-        e, f, a = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_uint32()
-        call("qsim_batch_config", self._h, int(n), ctypes.byref(e), ctypes.byref(f), ctypes.byref(a))
-        eng = {-1: "default", 0: "lds", 1: "wave"}.get(e.value, str(e.value))
-        return {"forward_engine": eng, "forward_tile_bits": f.value, "adjoint_tile_bits": a.value}
+        e, f, ae, a = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_int32(), ctypes.c_uint32()
+        call("qsim_batch_config", self._h, int(n), ctypes.byref(e), ctypes.byref(f), ctypes.byref(ae),
+             ctypes.byref(a))
+        names = {-1: "default", 0: "lds", 1: "wave"}
+        return {"forward_engine": names.get(e.value, str(e.value)), "forward_tile_bits": f.value,
+                "adjoint_engine": names.get(ae.value, str(ae.value)), "adjoint_tile_bits": a.value}
 
     def run(self, use_graph=True):
         """(energies[n_instances], grads[n_instances, max_params])"""

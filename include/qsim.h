@@ -207,9 +207,9 @@ int qsim_batch_add(qsim_batch* b, uint32_t n_qubits, const qsim_gate* gates, uin
                    const uint32_t* y_counts, const double* coef_re, const double* coef_im, uint64_t n_params,
                    uint64_t* index);
 int qsim_batch_tune(qsim_batch* b, int reps);
-/* fwd_engine: -1 default, 0 LDS tile (fwd_tile_bits), 1 wave tile */
+/* engines: -1 default, 0 LDS tile (2^tile_bits amplitudes), 1 wave tile */
 int qsim_batch_config(const qsim_batch* b, uint32_t n_qubits, int32_t* fwd_engine, uint32_t* fwd_tile_bits,
-                      uint32_t* adj_tile_bits);
+                      int32_t* adj_engine, uint32_t* adj_tile_bits);
 /* energies[i]; grads[i * qsim_batch_max_params(b) + p] */
 int qsim_batch_run(qsim_batch* b, int use_graph, double* energies, double* grads);
 uint64_t qsim_batch_size(const qsim_batch* b);

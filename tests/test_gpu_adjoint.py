@@ -88,6 +88,7 @@ def test_adjoint_tile_sizes(tb, rng, monkeypatch):
     """Small tiles force many passes, gradient points on base bits and
     several points per pass."""
     monkeypatch.setenv("QOMPSIM_ADJ_TILE_BITS", tb)
+    monkeypatch.setenv("QOMPSIM_ADJ_ENGINE", "lds")
     n = 12
     c = _mixed_circuit(n, rng, layers=2)
     obs = _mixed_observable(n, rng)
@@ -140,3 +141,17 @@ def test_torch_autograd_function():
     e0, g0 = O.backprop(c, obs)
     assert abs(E.item() - e0) < 1e-12
     assert np.max(np.abs(th.grad.numpy() - 3.0 * g0)) < 1e-10
+
+
+@pytest.mark.parametrize("n", [10, 13])
+def test_adjoint_engines_agree(n, rng, monkeypatch):
+    """Wave-tile and LDS-tile reverse sweeps give the same gradients."""
+    c = _mixed_circuit(n, rng, layers=3)
+    obs = _mixed_observable(n, rng)
+    e_w, g_w = adjoint_gradient(c, obs)
+    monkeypatch.setenv("QOMPSIM_ADJ_ENGINE", "lds")
+    e_l, g_l = adjoint_gradient(c, obs)
+    e0, g0 = O.backprop(c, obs)
+    assert abs(e_w - e0) < 1e-11 and abs(e_l - e0) < 1e-11
+    assert np.max(np.abs(g_w - g0)) < 1e-10
+    assert np.max(np.abs(g_w - g_l)) < 1e-10
