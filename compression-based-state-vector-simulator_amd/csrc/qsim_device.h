@@ -29,12 +29,12 @@ template <class EPtr>
 struct TileMapT {
   uint32_t t, c, nE;
   EPtr E;
-  __device__ uint64_t base(uint64_t tau) const {
+  __device__ __forceinline__ uint64_t base(uint64_t tau) const {
     uint64_t x = tau << c;
     for (uint32_t i = 0; i < nE; ++i) x = ins0_64(x, E[i]);
     return x;
   }
-  __device__ uint64_t off(uint32_t p) const {
+  __device__ __forceinline__ uint64_t off(uint32_t p) const {
     uint64_t o = p & ((1u << c) - 1u);
     for (uint32_t i = 0; i < nE; ++i) o |= (uint64_t)((p >> (c + i)) & 1u) << E[i];
     return o;

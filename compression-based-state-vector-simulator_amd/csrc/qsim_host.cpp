@@ -45,10 +45,13 @@ hipError_t launch_sv_apply_pauli(const double2* a, double2* out, uint64_t n, uin
 hipError_t launch_cursor_stats(const unsigned long long* cur, uint32_t n, unsigned long long* out, hipStream_t st);
 hipError_t launch_adjoint_pass(int tb, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
                                uint64_t ntiles, hipStream_t st);
-hipError_t launch_grad_reduce(const double* partials, uint64_t nblk, uint32_t npoints, double* out, hipStream_t st);
+hipError_t launch_grad_reduce(const double* partials, uint64_t nblk, uint32_t npoints, double* out, double* stage,
+                              hipStream_t st);
 hipError_t launch_accumulate_points(const double* pt, uint32_t np, const int32_t* map, double* acc, hipStream_t st);
 hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
                                uint64_t ntiles, hipStream_t st);
+hipError_t launch_wave_pair_adjoint(const PassArgs* d_args, double2* psi, double2* lam, double* partials,
+                                    uint64_t ntiles, hipStream_t st);
 hipError_t launch_pauli_sum(const double2* a, double2* out, uint64_t n, const PauliGroup* groups, uint32_t ng,
                             const PauliTermZ* terms, double* partials, unsigned grid, hipStream_t st);
 hipError_t launch_block_bytes(const uint64_t* desc, const uint8_t* nullf, uint64_t nb, uint64_t cpb,
@@ -479,10 +482,22 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
     return -1;
   };
   bool elem_open = false;
+  // the open elementwise op is the last op; its terms end the term list
+  auto close_elem = [&]() {
+    if (!elem_open) return;
+    Op& op = P.ops.back();
+    while (op.nterms % kTermGroup) {
+      PhaseTerm T{};
+      T.bval = 1;  // never matches: (base & 0) != 1
+      P.terms.push_back(T);
+      op.nterms++;
+    }
+    elem_open = false;
+  };
   for (size_t gi : pp.gates) {
     const LGate& g = lg[gi];
     if (g.grad >= 0) {
-      elem_open = false;
+      close_elem();
       Op op{};
       op.kind = OP_GRAD;
       op.gidx = (uint32_t)g.grad;  // parameter index; renumbered per pass by the caller
@@ -553,7 +568,7 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
       }
       continue;
     }
-    elem_open = false;
+    close_elem();
     Op op{};
     op.kind = OP_DENSE;
     op.k = (uint32_t)g.N.size();
@@ -578,6 +593,7 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
     P.ops.push_back(op);
     a.maxk = std::max(a.maxk, op.k);
   }
+  close_elem();
   a.op_end = (uint32_t)P.ops.size();
   P.passes.push_back(a);
 }
@@ -762,7 +778,7 @@ struct qsim_sv {
   mutable Partials part;
   // adjoint work buffers (kept between calls): lambda, the uncomputed psi,
   // per-tile gradient partials, reduced points, Pauli-sum tables
-  DBuf adj_lam, adj_psi, adj_part, adj_out, adj_groups, adj_terms;
+  DBuf adj_lam, adj_psi, adj_part, adj_out, adj_stage, adj_groups, adj_terms;
   double2* ptr() const { return reinterpret_cast<double2*>(d.p); }
 };
 
@@ -900,11 +916,24 @@ static bool cs_check_overflow(qsim_cs* cs, int out) {
   HIPCHECK(hipStreamSynchronize(stream()));
   auto& hb = cs->h_subbase[out];
   hb.assign((size_t)ns + 1, 0);
+  uint64_t dmax = 0;
   for (uint32_t s = 0; s < ns; ++s) {
     const uint64_t d = cur[(size_t)s * 16];
     hb[s + 1] = hb[s] + d + d / 4 + 64;
+    dmax = std::max(dmax, d + d / 4 + 64);
   }
-  if (hb[ns] > cs->cap_words[out]) cs_alloc_arena(cs, out, words_for_cap(hb[ns] * 8));
+  // Later stages start from an equal partition again: where device memory
+  // allows, grow the arena so that an equal split holds this stage's largest
+  // sub-arena demand, otherwise to the measured total only.
+  const uint64_t equal_words = dmax * ns;
+  if (equal_words > cs->cap_words[out]) {
+    size_t free_b = 0, total_b = 0;
+    HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t held = cs->cap_words[out] * 8;
+    const uint64_t want = words_for_cap(equal_words * 8) * 8;
+    if (want <= held + free_b / 2) cs_alloc_arena(cs, out, want / 8);
+    else if (hb[ns] > cs->cap_words[out]) cs_alloc_arena(cs, out, words_for_cap(hb[ns] * 8));
+  }
   const uint64_t slack = cs->cap_words[out] - hb[ns];
   hb[ns] += slack;  // leftover capacity goes to the last sub-arena
   cs_upload_subbase(cs, out);
@@ -1509,12 +1538,18 @@ static bool adjoint_wave_ok(const std::vector<LGate>& lg, uint32_t n) {
   return wave_engine_ok(lg, n);
 }
 
-// The wave form holds psi and lambda in registers: 8 rows per state
-// (512-amplitude tiles, three register qubits) keep it at two waves per SIMD.
-constexpr uint32_t kAdjWaveTile = 9;
+// Wave forms: by default a pair of waves per 1024-amplitude tile (psi and
+// lambda in one wave each, the forward engine's tile); QOMPSIM_ADJ_ENGINE=wave9
+// selects one wave holding 8 rows of each state (512-amplitude tiles).
+static uint32_t adjoint_wave_tile() {
+  const char* e = std::getenv("QOMPSIM_ADJ_ENGINE");
+  return (e && std::string(e) == "wave9") ? 9u : kWaveTile;
+}
+constexpr uint32_t kAdjWaveTile = kWaveTile;
 
 static void plan_reverse(Program& P, const std::vector<LGate>& lg, uint32_t n, uint32_t tb, bool wave) {
-  if (wave) plan_program_engine(P, lg, n, kAdjWaveTile, 6, false, false, true, kAdjWaveTile);
+  const uint32_t wt = adjoint_wave_tile();
+  if (wave && n >= wt) plan_program_engine(P, lg, n, wt, 6, false, false, true, wt);
   else plan_program_engine(P, lg, n, tb, std::min<uint32_t>(5, tb), false, false, false);
 }
 
@@ -1524,8 +1559,10 @@ static uint64_t reverse_tiles(const Program& P, size_t p, uint32_t n) { return (
 static hipError_t launch_reverse_pass(const Program& P, size_t p, uint32_t n, double2* psi, double2* lam,
                                       double* partials) {
   const uint64_t ntiles = reverse_tiles(P, p, n);
-  if (P.passes[p].engine == ENGINE_WAVE)
+  if (P.passes[p].engine == ENGINE_WAVE) {
+    if (P.passes[p].t == kWaveTile) return launch_wave_pair_adjoint(P.dargs(p), psi, lam, partials, ntiles, stream());
     return launch_wave_adjoint((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, stream());
+  }
   return launch_adjoint_pass((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, stream());
 }
 
@@ -1621,6 +1658,7 @@ int qsim_sv_backprop(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, cons
     for (size_t pi = 0; pi < P.passes.size(); ++pi) maxtiles = std::max(maxtiles, reverse_tiles(P, pi, sv->n));
     sv->adj_part.ensure(std::max<uint64_t>(1, (uint64_t)maxpts * maxtiles) * 2 * sizeof(double));
     sv->adj_out.ensure(std::max<uint32_t>(1, maxpts) * 2 * sizeof(double));
+    sv->adj_stage.ensure((uint64_t)std::max<uint32_t>(1, maxpts) * kGradReduceChunks * 2 * sizeof(double));
     std::vector<double> hout(2 * std::max<uint32_t>(1, maxpts));
     for (size_t pi = 0; pi < P.passes.size(); ++pi) {
       HIPCHECK(launch_reverse_pass(P, pi, sv->n, sv->adj_psi.as<double2>(), sv->adj_lam.as<double2>(),
@@ -1628,7 +1666,7 @@ int qsim_sv_backprop(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, cons
       const uint32_t np = (uint32_t)params[pi].size();
       if (np == 0) continue;
       HIPCHECK(launch_grad_reduce(sv->adj_part.as<double>(), reverse_tiles(P, pi, sv->n), np,
-                                  sv->adj_out.as<double>(), stream()));
+                                  sv->adj_out.as<double>(), sv->adj_stage.as<double>(), stream()));
       HIPCHECK(hipMemcpyAsync(hout.data(), sv->adj_out.p, np * 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
       HIPCHECK(hipStreamSynchronize(stream()));
       for (uint32_t k = 0; k < np; ++k) gacc[params[pi][k]].merge(hout[2 * k], hout[2 * k + 1]);
@@ -2077,7 +2115,7 @@ struct qsim_batch {
   std::map<uint32_t, qsim::TuneChoice> tune;
   uint32_t maxn = 0;
   uint64_t maxparams = 0;
-  qsim::DBuf psi, psiw, lam, part, ptout, acc;
+  qsim::DBuf psi, psiw, lam, part, ptout, stage, acc;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool ready = false;  // planned and buffers sized
@@ -2115,6 +2153,7 @@ static void batch_prepare(qsim_batch* b) {
   b->lam.ensure(maxdim * 16);
   b->part.ensure(maxpart * 16);
   b->ptout.ensure(maxpts * 16);
+  b->stage.ensure(std::max<uint64_t>(1, maxpts) * kGradReduceChunks * 16);
   b->acc.ensure(b->inst.size() * acc_stride(b) * 16);
   b->ready = true;
 }
@@ -2135,14 +2174,14 @@ static void batch_enqueue(qsim_batch* b, size_t i) {
   const unsigned grid = reduce_grid(dim);
   HIPCHECK(launch_pauli_sum(psi, b->lam.as<double2>(), dim, I.dG.as<PauliGroup>(), (uint32_t)I.G.size(),
                             I.dT.as<PauliTermZ>(), b->part.as<double>(), grid, stream()));
-  HIPCHECK(launch_grad_reduce(b->part.as<double>(), grid, 1, acc, stream()));
+  HIPCHECK(launch_grad_reduce(b->part.as<double>(), grid, 1, acc, nullptr, stream()));
   HIPCHECK(hipMemcpyAsync(b->psiw.p, psi, dim * 16, hipMemcpyDeviceToDevice, stream()));
   for (size_t p = 0; p < I.rev.passes.size(); ++p) {
     const uint64_t ntiles = reverse_tiles(I.rev, p, I.n);
     HIPCHECK(launch_reverse_pass(I.rev, p, I.n, b->psiw.as<double2>(), b->lam.as<double2>(), b->part.as<double>()));
     const uint32_t np = I.map_cnt[p];
     if (np == 0) continue;
-    HIPCHECK(launch_grad_reduce(b->part.as<double>(), ntiles, np, b->ptout.as<double>(), stream()));
+    HIPCHECK(launch_grad_reduce(b->part.as<double>(), ntiles, np, b->ptout.as<double>(), b->stage.as<double>(), stream()));
     HIPCHECK(launch_accumulate_points(b->ptout.as<double>(), np, I.dmap.as<int32_t>() + I.map_off[p], acc + 2,
                                       stream()));
   }

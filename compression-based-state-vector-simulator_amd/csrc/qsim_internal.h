@@ -37,6 +37,8 @@ enum Engine : uint32_t { ENGINE_LDS = 0, ENGINE_WAVE = 1 };
 constexpr uint32_t kWaveTile = 10;  // wave tile bits: 6 lane bits + 4 register slots
 // A dispatch packet gives the grid in work-items as 32-bit values: launches of
 // 256-thread work-groups are split at 2^23 groups (2^31 work-items).
+// first-level chunks per gradient point in the two-level partial reduction
+constexpr uint32_t kGradReduceChunks = 64;
 constexpr uint64_t kMaxLaunchBlocks = 1ull << 23;
 
 // One op of a pass.  Dense: 2^nsel matrices of 2^k x 2^k (row-major, matrix
@@ -75,12 +77,15 @@ struct PauliTermZ {
   double cr, ci;
 };
 
-struct PhaseTerm {
+// 32 bytes: kernels read a term with one scalar load.  An elementwise op's
+// term list is padded to a multiple of kTermGroup with terms that never
+// match (bmask 0, bval 1), so terms are read kTermGroup at a time.
+struct alignas(32) PhaseTerm {
   uint32_t tmask, tval;    // condition on the tile element index p
   uint64_t bmask, bval;    // condition on the tile base (local index)
   double angle;
-  double pad;
 };
+constexpr uint32_t kTermGroup = 2;
 
 struct DiagTable {
   uint32_t m;              // number of qubits

@@ -623,12 +623,13 @@ __global__ __launch_bounds__(kThreads) void adjoint_pass_kernel(const PassArgs* 
 
 // out[point] = compensated sum over nblk tiles of the pass's partials
 // (one work-group per gradient point)
-__global__ __launch_bounds__(kThreads) void grad_reduce_kernel(const double* __restrict__ partials, uint64_t nblk,
-                                                               double* __restrict__ out) {
+// Compensated sum of the (s, c) pairs P[begin, end) by one block; thread 0
+// stores the result pair at out[0..1].
+__device__ __forceinline__ void block_pair_sum(const double* __restrict__ P, uint64_t begin, uint64_t end,
+                                               double* __restrict__ out) {
   __shared__ double red_s[kThreads / 64], red_c[kThreads / 64];
-  const double* P = partials + (uint64_t)blockIdx.x * nblk * 2;
   double s = 0.0, c = 0.0;
-  for (uint64_t i = threadIdx.x; i < nblk; i += kThreads) dd_merge(s, c, P[2 * i], P[2 * i + 1]);
+  for (uint64_t i = begin + threadIdx.x; i < end; i += kThreads) dd_merge(s, c, P[2 * i], P[2 * i + 1]);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int d = 32; d >= 1; d >>= 1) {
     const double s2 = __shfl_xor(s, d, 64), c2 = __shfl_xor(c, d, 64);
@@ -639,9 +640,25 @@ __global__ __launch_bounds__(kThreads) void grad_reduce_kernel(const double* __r
   if (threadIdx.x == 0) {
     double ss = red_s[0], cc = red_c[0];
     for (int q = 1; q < kThreads / 64; ++q) dd_merge(ss, cc, red_s[q], red_c[q]);
-    out[2 * blockIdx.x] = ss;
-    out[2 * blockIdx.x + 1] = cc;
+    out[0] = ss;
+    out[1] = cc;
   }
+}
+
+// one block per point: out[2 p .. 2 p + 1] = sum of the nblk pairs of point p
+__global__ __launch_bounds__(kThreads) void grad_reduce_kernel(const double* __restrict__ partials, uint64_t nblk,
+                                                               double* __restrict__ out) {
+  block_pair_sum(partials + (uint64_t)blockIdx.x * nblk * 2, 0, nblk, out + 2 * (uint64_t)blockIdx.x);
+}
+
+// first level for long partial lists: block (x, y) sums chunk x of point y
+// into stage[2 (y * gridDim.x + x) ..]
+__global__ __launch_bounds__(kThreads) void grad_reduce_chunks_kernel(const double* __restrict__ partials,
+                                                                      uint64_t nblk, uint64_t chunk,
+                                                                      double* __restrict__ stage) {
+  const uint64_t b = (uint64_t)blockIdx.x * chunk, e = std::min<uint64_t>(nblk, b + chunk);
+  block_pair_sum(partials + (uint64_t)blockIdx.y * nblk * 2, b, e,
+                 stage + 2 * ((uint64_t)blockIdx.y * gridDim.x + blockIdx.x));
 }
 
 // acc[map[k]] += pt[k] (compensated pairs) for the gradient points of one
@@ -868,9 +885,19 @@ hipError_t launch_accumulate_points(const double* pt, uint32_t np, const int32_t
   return hipGetLastError();
 }
 
-hipError_t launch_grad_reduce(const double* partials, uint64_t nblk, uint32_t npoints, double* out, hipStream_t st) {
+// stage: room for npoints * kGradReduceChunks pairs, or null for one level
+hipError_t launch_grad_reduce(const double* partials, uint64_t nblk, uint32_t npoints, double* out, double* stage,
+                              hipStream_t st) {
   if (npoints == 0) return hipSuccess;
-  hipLaunchKernelGGL(grad_reduce_kernel, dim3(npoints), dim3(kThreads), 0, st, partials, nblk, out);
+  const uint64_t nch = std::min<uint64_t>(kGradReduceChunks, (nblk + 4095) / 4096);
+  if (stage == nullptr || nch < 2) {
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3(npoints), dim3(kThreads), 0, st, partials, nblk, out);
+    return hipGetLastError();
+  }
+  const uint64_t chunk = (nblk + nch - 1) / nch;
+  hipLaunchKernelGGL(grad_reduce_chunks_kernel, dim3((unsigned)nch, npoints), dim3(kThreads), 0, st, partials, nblk,
+                     chunk, stage);
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3(npoints), dim3(kThreads), 0, st, (const double*)stage, nch, out);
   return hipGetLastError();
 }
 

@@ -38,9 +38,79 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 shfl_xor2(double2 v, int m) {
   return make_double2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
 }
+// value of lane ^ (1 << B) without the LDS crossbar: DPP within a row of 16
+// lanes (B <= 3), the row-swap instructions for B = 4 and 5
+template <int B>
+__device__ __forceinline__ uint32_t lane_xor_u32(uint32_t v, uint32_t lane) {
+  static_assert(B >= 0 && B < 6, "lane bit");
+  if constexpr (B == 0) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xb1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+  } else if constexpr (B == 1) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4e, 0xf, 0xf, false);  // quad_perm [2,3,0,1]
+  } else if constexpr (B == 2) {
+    // row_half_mirror (i ^ 7), then quad_perm [3,2,1,0] (i ^ 3)
+    const int h = __builtin_amdgcn_mov_dpp((int)v, 0x141, 0xf, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp(h, 0x1b, 0xf, 0xf, false);
+  } else if constexpr (B == 3) {
+    // row_mirror (i ^ 15), then row_half_mirror (i ^ 7)
+    const int h = __builtin_amdgcn_mov_dpp((int)v, 0x140, 0xf, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp(h, 0x141, 0xf, 0xf, false);
+  } else if constexpr (B == 4) {
+    // with both operands v: the first result holds rows (0, 0, 2, 2), the
+    // second rows (1, 1, 3, 3)
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane & 16u) ? r[0] : r[1];
+  } else {
+    // first result: halves (lo, lo); second: (hi, hi)
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane & 32u) ? r[0] : r[1];
+  }
+}
+template <int B>
+__device__ __forceinline__ double lane_xor_f64(double v, uint32_t lane) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint64_t r = ((uint64_t)lane_xor_u32<B>((uint32_t)(b >> 32), lane) << 32) | lane_xor_u32<B>((uint32_t)b, lane);
+  return __longlong_as_double((long long)r);
+}
+template <int B>
+__device__ __forceinline__ double2 lane_xor2(double2 v, uint32_t lane) {
+  return make_double2(lane_xor_f64<B>(v.x, lane), lane_xor_f64<B>(v.y, lane));
+}
+// For B = 4 or 5: exchanges the lanes of x whose bit B is set with the lanes
+// of y whose bit B is clear.  Afterwards every lane holds in x the amplitude
+// with bit B clear and in y its partner with bit B set (of row x in the lanes
+// with bit B clear, of row y in the others); a second call restores the rows.
+template <int B>
+__device__ __forceinline__ void swap_half_rows(double2& x, double2& y) {
+  static_assert(B == 4 || B == 5, "row swaps exist for lane bits 4 and 5");
+  uint64_t xb[2] = {(uint64_t)__double_as_longlong(x.x), (uint64_t)__double_as_longlong(x.y)};
+  uint64_t yb[2] = {(uint64_t)__double_as_longlong(y.x), (uint64_t)__double_as_longlong(y.y)};
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    uint32_t xl = (uint32_t)xb[c], xh = (uint32_t)(xb[c] >> 32), yl = (uint32_t)yb[c], yh = (uint32_t)(yb[c] >> 32);
+    if constexpr (B == 4) {
+      const auto rl = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+      const auto rh = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+      xl = rl[0]; yl = rl[1]; xh = rh[0]; yh = rh[1];
+    } else {
+      const auto rl = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+      const auto rh = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+      xl = rl[0]; yl = rl[1]; xh = rh[0]; yh = rh[1];
+    }
+    xb[c] = ((uint64_t)xh << 32) | xl;
+    yb[c] = ((uint64_t)yh << 32) | yl;
+  }
+  x = make_double2(__longlong_as_double((long long)xb[0]), __longlong_as_double((long long)xb[1]));
+  y = make_double2(__longlong_as_double((long long)yb[0]), __longlong_as_double((long long)yb[1]));
+}
+
 // keeps the compiler from hoisting coefficient loads of later register
 // iterations (bounds VGPR live ranges)
 #define QSIM_ITER_FENCE() asm volatile("" ::: "memory")
+
+// one PhaseTerm as eight dwords (a single scalar load)
+typedef uint32_t TermBits __attribute__((ext_vector_type(8)));
+static_assert(sizeof(PhaseTerm) == sizeof(TermBits), "PhaseTerm layout");
 
 __device__ __forceinline__ double2 ldc(const QC double* __restrict__ c, uint64_t idx) {
   return make_double2(c[2 * idx], c[2 * idx + 1]);
@@ -245,23 +315,49 @@ __device__ __forceinline__ uint32_t wave_sel(const QC Op& op, const WaveCtx& cx,
   return sel;
 }
 
-// one target on lane bit b
+// true when no selector bit comes from a lane: the selector is the same on
+// every lane of a row
+__device__ __forceinline__ bool op_sel_uniform(const QC Op& op) {
+  for (uint32_t s = 0; s < op.nsel; ++s)
+    if (op.sel_tile[s] && op.sel_pos[s] < 6) return false;
+  return true;
+}
+
+// one target on lane bit B, one matrix for every row
+template <int B, int KV>
+__device__ __forceinline__ void k1_lane_uniform(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx,
+                                                const QC double* __restrict__ C) {
+  const uint32_t lane = (uint32_t)cx.lane;
+  if (op.ident_mask & 1ull) return;
+  if constexpr (B >= 4) {
+    // rows in pairs: after the half-row swap each lane holds both amplitudes
+    // of a pair, so the matrix entries are wave-uniform
+    const double2 m0 = ldc(C, op.mat_off), m1 = ldc(C, op.mat_off + 1), m2 = ldc(C, op.mat_off + 2),
+                  m3 = ldc(C, op.mat_off + 3);
+#pragma unroll
+    for (int v = 0; v < KV; v += 2) {
+      double2 x = amp[v], y = amp[v + 1];
+      swap_half_rows<B>(x, y);
+      double2 x2 = mac2(m0, x, m1, y), y2 = mac2(m2, x, m3, y);
+      swap_half_rows<B>(x2, y2);
+      amp[v] = x2;
+      amp[v + 1] = y2;
+    }
+  } else {
+    const uint32_t r = (lane >> B) & 1u;
+    const double2 md = ldc(C, op.mat_off + 3u * r), mo = ldc(C, op.mat_off + 1u + r);
+#pragma unroll
+    for (int v = 0; v < KV; ++v) amp[v] = mac2(md, amp[v], mo, lane_xor2<B>(amp[v], lane));
+  }
+}
+
+// one target on lane bit b, matrix chosen per element by the selector bits
 template <int KV>
-__device__ __forceinline__ void k1_lane(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
+__device__ __forceinline__ void k1_lane_sel(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx,
+                                            const QC double* __restrict__ C) {
   const uint32_t b = op.tpos[0];
   const uint32_t r = ((uint32_t)cx.lane >> b) & 1u;
   const int m = 1 << b;
-  if (op.nsel == 0) {
-    if (op.ident_mask & 1ull) return;
-    const double2 md = ldc(C, op.mat_off + 3u * r), mo = ldc(C, op.mat_off + 1u + r);
-#pragma unroll
-    for (int v = 0; v < KV; ++v) {
-      const double2 p = shfl_xor2(amp[v], m);
-      amp[v] = mac2(md, amp[v], mo, p);
-      QSIM_ITER_FENCE();
-    }
-    return;
-  }
 #pragma unroll
   for (int v = 0; v < KV; ++v) {
     QSIM_ITER_FENCE();
@@ -277,11 +373,26 @@ __device__ __forceinline__ void k1_lane(const QC Op& op, double2 (&amp)[KV], con
 template <int J, int KV>
 __device__ __forceinline__ void k1_reg(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
   constexpr int MJ = 1 << J;
+  if (op.nsel == 0) {
+    // one matrix for every row: its entries are loaded once (wave-uniform)
+    if (op.ident_mask & 1ull) return;
+    const uint64_t mo = op.mat_off;
+    const double2 m0 = ldc(C, mo), m1 = ldc(C, mo + 1), m2 = ldc(C, mo + 2), m3 = ldc(C, mo + 3);
+#pragma unroll
+    for (int v = 0; v < KV; ++v) {
+      if (v & MJ) continue;
+      const double2 a = amp[v], b = amp[v | MJ];
+      amp[v] = mac2(m0, a, m1, b);
+      amp[v | MJ] = mac2(m2, a, m3, b);
+    }
+    return;
+  }
+  const bool uni = op_sel_uniform(op);
 #pragma unroll
   for (int v = 0; v < KV; ++v) {
     if (v & MJ) continue;
     QSIM_ITER_FENCE();
-    const uint32_t sel = op.nsel ? wave_sel(op, cx, v) : 0u;
+    const uint32_t sel = uni ? (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_sel(op, cx, v)) : wave_sel(op, cx, v);
     if ((op.ident_mask >> sel) & 1ull) continue;
     const uint64_t mo = op.mat_off + 4ull * sel;
     const double2 a = amp[v], b = amp[v | MJ];
@@ -294,11 +405,14 @@ __device__ __forceinline__ void k1_reg(const QC Op& op, double2 (&amp)[KV], cons
 template <int J0, int J1, int KV>
 __device__ __forceinline__ void k2_regreg(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
   constexpr int M0 = 1 << J0, M1 = 1 << J1;
+  const bool uni = op_sel_uniform(op);
 #pragma unroll
   for (int v = 0; v < KV; ++v) {
     if (v & (M0 | M1)) continue;
     QSIM_ITER_FENCE();
-    const uint32_t sel = op.nsel ? wave_sel(op, cx, v) : 0u;
+    // a wave-uniform selector keeps the matrix in scalar loads
+    const uint32_t sel = op.nsel == 0 ? 0u : uni ? (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_sel(op, cx, v))
+                                                 : wave_sel(op, cx, v);
     if ((op.ident_mask >> sel) & 1ull) continue;
     const uint64_t mo = op.mat_off + 16ull * sel;
     const double2 x[4] = {amp[v], amp[v | M0], amp[v | M1], amp[v | M0 | M1]};
@@ -386,8 +500,18 @@ template <int KV>
 __device__ __forceinline__ void wave_dense(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
   if (op.k == 1) {
     const uint32_t t = op.tpos[0];
-    if (t < 6) k1_lane(op, amp, cx, C);
-    else switch (t - 6) {
+    if (t < 6 && op.nsel != 0) {
+      k1_lane_sel(op, amp, cx, C);
+    } else if (t < 6) {
+      switch (t) {
+        case 0: k1_lane_uniform<0>(op, amp, cx, C); break;
+        case 1: k1_lane_uniform<1>(op, amp, cx, C); break;
+        case 2: k1_lane_uniform<2>(op, amp, cx, C); break;
+        case 3: k1_lane_uniform<3>(op, amp, cx, C); break;
+        case 4: k1_lane_uniform<4>(op, amp, cx, C); break;
+        default: k1_lane_uniform<5>(op, amp, cx, C); break;
+      }
+    } else switch (t - 6) {
         case 0: k1_reg<0>(op, amp, cx, C); break;
         case 1: k1_reg<1>(op, amp, cx, C); break;
         case 2: k1_reg<2>(op, amp, cx, C); break;
@@ -439,22 +563,33 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], c
     double aL = 0.0, aF0 = 0.0, aF1 = 0.0, aF2 = 0.0, aF3 = 0.0;
     uint32_t used = 0;
     bool generic = false;
-    for (uint32_t i = 0; i < op.nterms; ++i) {
-      const QC PhaseTerm& T = terms[op.term_off + i];
-      if ((cx.base & T.bmask) != T.bval) continue;
-      const uint32_t lm = T.tmask & 63u, lv = T.tval & 63u;
-      const uint32_t rm = (T.tmask >> 6) & 15u, rv = (T.tval >> 6) & 15u;
-      const double a = ((lane & lm) == lv) ? T.angle : 0.0;
+    // kTermGroup terms per scalar load round trip (the list is padded)
+    const QC TermBits* tp = (const QC TermBits*)(terms + op.term_off);
+    for (uint32_t i0 = 0; i0 < op.nterms; i0 += kTermGroup) {
+      TermBits tg[kTermGroup];
+#pragma unroll
+      for (uint32_t j = 0; j < kTermGroup; ++j) tg[j] = tp[i0 + j];
+#pragma unroll
+      for (uint32_t j = 0; j < kTermGroup; ++j) {
+      const uint32_t tmask = tg[j][0], tval = tg[j][1];
+      const uint64_t bmask = (uint64_t)tg[j][2] | ((uint64_t)tg[j][3] << 32);
+      const uint64_t bval = (uint64_t)tg[j][4] | ((uint64_t)tg[j][5] << 32);
+      if ((cx.base & bmask) != bval) continue;
+      const double angle = __longlong_as_double((long long)((uint64_t)tg[j][6] | ((uint64_t)tg[j][7] << 32)));
+      const uint32_t lm = tmask & 63u, lv = tval & 63u;
+      const uint32_t rm = (tmask >> 6) & 15u, rv = (tval >> 6) & 15u;
+      const double a = ((lane & lm) == lv) ? angle : 0.0;
       if (rm == 0) { aL += a; used |= 16u; continue; }
       if (rm & (rm - 1u)) { generic = true; continue; }
-      const uint32_t j = __ffs(rm) - 1;
+      const uint32_t js = __ffs(rm) - 1;
       double f = a;
       if (!rv) { aL += a; f = -a; used |= 16u; }
-      aF0 += j == 0 ? f : 0.0;
-      aF1 += j == 1 ? f : 0.0;
-      aF2 += j == 2 ? f : 0.0;
-      aF3 += j == 3 ? f : 0.0;
-      used |= 1u << j;
+      aF0 += js == 0 ? f : 0.0;
+      aF1 += js == 1 ? f : 0.0;
+      aF2 += js == 2 ? f : 0.0;
+      aF3 += js == 3 ? f : 0.0;
+      used |= 1u << js;
+      }
     }
     // each factor is formed right before it is applied; the scheduling
     // barriers keep the compiler from evaluating all five at once
@@ -555,7 +690,7 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
   QG uint64_t* desc_out = (QG uint64_t*)(dy.desc_out);
   QG uint64_t* arena_out = (QG uint64_t*)(dy.arena_out);
   QG uint8_t* nonzero = (QG uint8_t*)(dy.nonzero);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint64_t wt = dy.tile0 + (uint64_t)blockIdx.x * (kThreads / 64) + wave;
   if (wt >= ntiles) return;
   const uint32_t L = a.L;
@@ -767,7 +902,7 @@ __global__ __launch_bounds__(kThreads, 2) void wave_adjoint_kernel(const PassArg
   const QC PhaseTerm* terms = (const QC PhaseTerm*)(a.terms);
   const QC DiagTable* diags = (const QC DiagTable*)(a.diags);
   const QC double* coefs = (const QC double*)(a.coefs);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint64_t wt = tile0 + (uint64_t)blockIdx.x * (kThreads / 64) + wave;
   if (wt >= ntiles) return;
   const TileMapConst tm{9u, 6u, 3u, a.E};
@@ -852,6 +987,91 @@ __global__ __launch_bounds__(kThreads, 2) void wave_adjoint_kernel(const PassArg
       dl[2 * roff(v) + 1] = amp[R + v].y;
     }
   }
+}
+
+// Paired form on the forward engine's 1024-amplitude tiles: a workgroup of
+// two waves holds one tile of psi (wave 0) and the same positions of lambda
+// (wave 1), 16 rows each, and both waves run every dense / elementwise op.
+// At a gradient point the psi wave parks its tile in LDS and the lambda wave
+// reads the Pauli partner of each element from there, whatever the flip mask.
+// Two LDS tiles alternate between consecutive points, so one barrier per
+// point keeps the psi wave from overwriting a tile the lambda wave still reads.
+__global__ __launch_bounds__(128, 2) void wave_pair_adjoint_kernel(const PassArgs* __restrict__ pa,
+                                                                   double2* __restrict__ psi,
+                                                                   double2* __restrict__ lam,
+                                                                   double* __restrict__ partials, uint64_t tile0,
+                                                                   uint64_t ntiles) {
+  __shared__ __align__(16) double2 pbuf[2][1 << kWaveTileBits];
+  const QC PassArgs& a = *(const QC PassArgs*)(pa);
+  const QC Op* ops = (const QC Op*)(a.ops);
+  const QC PhaseTerm* terms = (const QC PhaseTerm*)(a.terms);
+  const QC DiagTable* diags = (const QC DiagTable*)(a.diags);
+  const QC double* coefs = (const QC double*)(a.coefs);
+  const int lane = threadIdx.x & 63;
+  const bool is_lam = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) != 0;
+  const uint64_t wt = tile0 + blockIdx.x;
+  if (wt >= ntiles) return;  // uniform over the workgroup
+  const TileMapConst tm{(uint32_t)kWaveTileBits, 6u, (uint32_t)kRegBits, a.E};
+  const uint64_t base = tm.base(wt);
+  const uint64_t e0 = 1ull << a.E[0], e1 = 1ull << a.E[1], e2 = 1ull << a.E[2], e3 = 1ull << a.E[3];
+  auto roff = [&](int v) -> uint64_t {
+    return ((v & 1) ? e0 : 0ull) | ((v & 2) ? e1 : 0ull) | ((v & 4) ? e2 : 0ull) | ((v & 8) ? e3 : 0ull);
+  };
+  const WaveCtx cx{lane, base};
+  QG double* sv = (QG double*)(is_lam ? lam : psi) + 2 * (base + lane);
+  double2 amp[kV];
+#pragma unroll
+  for (int v = 0; v < kV; ++v) amp[v] = make_double2(sv[2 * roff(v)], sv[2 * roff(v) + 1]);
+  uint32_t pb = 0;
+  for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
+    const QC Op& op = ops[o];
+    if (op.kind == OP_DENSE) {
+      wave_dense(op, amp, cx, coefs);
+    } else if (op.kind == OP_ELEM) {
+      wave_elem(op, amp, cx, terms, diags, coefs);
+    } else {
+      double2* xb = pbuf[pb];
+      pb ^= 1u;
+      if (!is_lam) {
+#pragma unroll
+        for (int v = 0; v < kV; ++v) xb[64 * v + lane] = amp[v];
+      }
+      __syncthreads();
+      if (is_lam) {
+        const uint32_t fx = (uint32_t)op.gfx, zt = (uint32_t)op.gzt;
+        const int kr = (int)((1u + op.nY) & 3u);  // i * i^nY = i^kr
+        const bool bneg = __popcll(base & op.gzb) & 1;
+        double s = 0.0, c = 0.0;
+#pragma unroll
+        for (int v = 0; v < kV; ++v) {
+          const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
+          dd_acc(s, c, grad_term(amp[v], xb[q], bneg ^ (bool)(__popc(q & zt) & 1), kr));
+        }
+        // tile sum: the lanes' compensated sums added in a DPP tree
+        const double tot = __shfl(wave_sum_f64(s + c), 63, 64);
+        if (lane == 0) {
+          partials[((uint64_t)op.gidx * ntiles + wt) * 2] = tot;
+          partials[((uint64_t)op.gidx * ntiles + wt) * 2 + 1] = 0.0;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < kV; ++v) {
+    sv[2 * roff(v)] = amp[v].x;
+    sv[2 * roff(v) + 1] = amp[v].y;
+  }
+}
+
+// ntiles counts 1024-amplitude tiles of one state; one workgroup per tile
+hipError_t launch_wave_pair_adjoint(const PassArgs* d_args, double2* psi, double2* lam, double* partials,
+                                    uint64_t ntiles, hipStream_t st) {
+  for (uint64_t t0 = 0; t0 < ntiles; t0 += kMaxLaunchBlocks) {
+    const uint64_t cnt = std::min<uint64_t>(ntiles - t0, kMaxLaunchBlocks);
+    hipLaunchKernelGGL(wave_pair_adjoint_kernel, dim3((unsigned)cnt), dim3(128), 0, st, d_args, psi, lam, partials,
+                       t0, ntiles);
+  }
+  return hipGetLastError();
 }
 
 // ntiles counts wave tiles of 2^9 amplitudes of one state

@@ -143,15 +143,17 @@ def test_torch_autograd_function():
     assert np.max(np.abs(th.grad.numpy() - 3.0 * g0)) < 1e-10
 
 
-@pytest.mark.parametrize("n", [10, 13])
+@pytest.mark.parametrize("n", [10, 13, 16])
 def test_adjoint_engines_agree(n, rng, monkeypatch):
-    """Wave-tile and LDS-tile reverse sweeps give the same gradients."""
+    """Paired-wave, single-wave (512-amplitude tiles) and LDS-tile reverse
+    sweeps give the oracle's gradients."""
     c = _mixed_circuit(n, rng, layers=3)
     obs = _mixed_observable(n, rng)
-    e_w, g_w = adjoint_gradient(c, obs)
-    monkeypatch.setenv("QOMPSIM_ADJ_ENGINE", "lds")
-    e_l, g_l = adjoint_gradient(c, obs)
     e0, g0 = O.backprop(c, obs)
-    assert abs(e_w - e0) < 1e-11 and abs(e_l - e0) < 1e-11
-    assert np.max(np.abs(g_w - g0)) < 1e-10
-    assert np.max(np.abs(g_w - g_l)) < 1e-10
+    got = {}
+    for eng in ("", "wave9", "lds"):
+        monkeypatch.setenv("QOMPSIM_ADJ_ENGINE", eng)
+        got[eng] = adjoint_gradient(c, obs)
+    for eng, (e, g) in got.items():
+        assert abs(e - e0) < 1e-11, eng
+        assert np.max(np.abs(g - g0)) < 1e-10, eng
