@@ -151,7 +151,36 @@ def adjoint_bench(n=28, layers=8, reps=3):
         c.set_parameter(i, th[i])
         fd_err = max(fd_err, abs((vals[0] - vals[1]) / (2 * h) - grads[i]))
     t_fd = 2 * npar * t_eval
+    # parameter shift: dE/dtheta = (E(theta + pi/2) - E(theta - pi/2)) / 2
+    ps_err = 0.0
+    for i in (1, npar // 2, npar - 2):
+        vals = []
+        for d in (np.pi / 2, -np.pi / 2):
+            c.set_parameter(i, th[i] + d)
+            fwd()
+            vals.append(obs.get_expectation_value(s).real)
+        c.set_parameter(i, th[i])
+        ps_err = max(ps_err, abs((vals[0] - vals[1]) / 2 - grads[i]))
     del s
+    # compressed forward checkpoints every 2 layers
+    from qompsim.adjoint import CheckpointedAdjoint
+
+    seg = 2 * (3 * n - 1)
+    adj = CheckpointedAdjoint(c, seg, block_size=min(20, n))
+    ck = {}
+
+    def ck_fwd():
+        adj.forward_with_checkpoints()
+
+    def ck_bwd():
+        ck["eg"] = adj.adjoint(obs)
+
+    t_ck_fwd = timed(ck_fwd, reps)
+    ck_fwd()
+    t_ck_bwd = timed(ck_bwd, reps)
+    nck, ck_bytes = adj.checkpoint_info()
+    ck_err = float(np.max(np.abs(ck["eg"][1] - grads)))
+    del adj
     return {
         "workload": f"HEA-{n}, {layers} layers (RY, RZ per qubit; CZ chain), {npar} parameters; "
                     f"H = sum Z_i Z_i+1 + 0.7 sum X_i ({len(obs.terms)} terms)",
@@ -164,6 +193,19 @@ def adjoint_bench(n=28, layers=8, reps=3):
         "central_fd_basis": f"2 x {npar} evaluations x measured forward+energy time {t_eval * 1e3:.2f} ms",
         "speedup_vs_central_fd": t_fd / t_adj,
         "max_abs_grad_error_vs_fd": fd_err,
+        "max_abs_grad_error_vs_parameter_shift": ps_err,
+        "reverse_ms_est": (t_adj - t_fwd) * 1e3,
+        "reverse_over_forward_est": (t_adj - t_fwd) / t_fwd,
+        "checkpointed": {
+            "segment_gates": seg,
+            "checkpoints": nck,
+            "checkpoint_GiB": ck_bytes / 2**30,
+            "codec": "absolute bound 1e-13 per double",
+            "forward_with_checkpoints_ms": t_ck_fwd * 1e3,
+            "adjoint_ms": t_ck_bwd * 1e3,
+            "adjoint_over_forward": t_ck_bwd / t_fwd,
+            "max_abs_grad_diff_vs_plain_sweep": ck_err,
+        },
     }
 
 

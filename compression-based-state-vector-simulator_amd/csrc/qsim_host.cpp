@@ -14,6 +14,7 @@
 #include <map>
 #include <memory>
 #include <numeric>
+#include <random>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -48,6 +49,12 @@ hipError_t launch_adjoint_pass(int tb, const PassArgs* d_args, double2* psi, dou
 hipError_t launch_grad_reduce(const double* partials, uint64_t nblk, uint32_t npoints, double* out, double* stage,
                               hipStream_t st);
 hipError_t launch_accumulate_points(const double* pt, uint32_t np, const int32_t* map, double* acc, hipStream_t st);
+hipError_t launch_block_sumsq(const double2* a, uint64_t namps, uint32_t nblocks, double* stage, double* out,
+                              hipStream_t st);
+uint32_t block_scan_tiles(uint64_t namps);
+hipError_t launch_block_sample(const double2* a, uint64_t namps, double* cdf_local, double* tile_sum, double* tile_incl,
+                               const double* u, uint32_t nshots, uint64_t base, unsigned long long* out,
+                               hipStream_t st);
 hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
                                uint64_t ntiles, hipStream_t st);
 hipError_t launch_wave_pair_adjoint(const PassArgs* d_args, double2* psi, double2* lam, double* partials,
@@ -792,24 +799,26 @@ static uint32_t default_tile_bits() {
   return 12;
 }
 
-static void sv_apply_gates(qsim_sv* sv, const std::vector<HGate>& gs) {
-  if (gs.empty()) return;
-  std::vector<int> qmap(sv->n);
+// gates gs[b, e) on the 2^n amplitudes at p (device)
+static void apply_gates_raw(double2* p, uint32_t n, const std::vector<HGate>& gs, size_t b, size_t e) {
+  if (b >= e) return;
+  std::vector<int> qmap(n);
   std::iota(qmap.begin(), qmap.end(), 0);
   std::vector<LGate> lg;
-  lg.reserve(gs.size());
-  for (auto& g : gs) lg.push_back(lower_gate(g, qmap));
+  lg.reserve(e - b);
+  for (size_t i = b; i < e; ++i) lg.push_back(lower_gate(gs[i], qmap));
   Program P;
-  plan_program(P, lg, sv->n, default_tile_bits(), 5, false);
+  plan_program(P, lg, n, default_tile_bits(), 5, false);
   if (P.passes.empty()) return;
   P.upload_all();
   for (size_t i = 0; i < P.passes.size(); ++i) {
     PassDyn dy{};
-    dy.raw = reinterpret_cast<double*>(sv->d.p);
+    dy.raw = reinterpret_cast<double*>(p);
     HIPCHECK(launch_program_pass(P, i, IN_RAW, OUT_RAW, dy, 1, stream()));
   }
   HIPCHECK(hipStreamSynchronize(stream()));
 }
+static void sv_apply_gates(qsim_sv* sv, const std::vector<HGate>& gs) { apply_gates_raw(sv->ptr(), sv->n, gs, 0, gs.size()); }
 }  // namespace qsim
 
 // ====================================================================== CS
@@ -1065,6 +1074,44 @@ static void cs_load(qsim_cs* cs, const double* host, uint64_t basis) {
   cs->cur_bytes = cs_current_bytes(cs);
 }
 
+// Replace the whole state by a device vector of 2^n amplitudes: blocks are
+// encoded straight from it (block b = amplitudes [b 2^B, (b+1) 2^B)).
+static void cs_load_device(qsim_cs* cs, const double2* dev) {
+  const int out = 1 - cs->cur;
+  std::vector<uint32_t> ids(cs->nb);
+  std::iota(ids.begin(), ids.end(), 0u);
+  for (int attempt = 0;; ++attempt) {
+    HIPCHECK(launch_fill_u8(cs->null_flags.as<uint8_t>(), cs->nb, 1, stream()));
+    HIPCHECK(launch_fill_u8(cs->nonzero.as<uint8_t>(), cs->nb, 0, stream()));
+    {
+      const PassArgs& io = cs->io.passes[0];
+      const uint32_t tb = io.engine == ENGINE_WAVE ? kWaveTile : io.t;
+      cs_reset_out(cs, out, pick_nsub(cs->nb << (cs->B - tb)), attempt > 0);
+    }
+    upload(cs->bmap, ids.data(), ids.size() * sizeof(uint32_t));
+    PassDyn dy = cs_dyn(cs, out);
+    dy.raw = (double*)dev;
+    dy.bmap = cs->bmap.as<uint32_t>();
+    HIPCHECK(launch_program_pass(cs->io, 0, IN_RAW, OUT_ENC, dy, ids.size(), stream()));
+    if (!cs_check_overflow(cs, out)) break;
+    if (attempt > 8) throw std::runtime_error("compressed store: arena growth did not converge");
+  }
+  cs_commit(cs, out);
+  cs->cur_bytes = cs_current_bytes(cs);
+}
+
+// Decode the whole state into a device vector of 2^n amplitudes.
+static void cs_read_device(qsim_cs* cs, double2* dev) {
+  std::vector<uint32_t> ids(cs->nb);
+  std::iota(ids.begin(), ids.end(), 0u);
+  upload(cs->bmap, ids.data(), ids.size() * sizeof(uint32_t));
+  PassDyn dy = cs_dyn(cs, 1 - cs->cur);
+  dy.raw = (double*)dev;
+  dy.bmap = cs->bmap.as<uint32_t>();
+  HIPCHECK(launch_program_pass(cs->io, 0, IN_DEC, OUT_RAW, dy, ids.size(), stream()));
+}
+
+
 // make_block_groups (compressed_state.hpp:585-671) for one stage.
 static void stage_groups(qsim_cs* cs, CStage& S) {
   std::set<uint32_t> inner;
@@ -1257,6 +1304,10 @@ static void cs_register(qsim_cs* cs, std::vector<HGate> gs, const std::vector<st
   catch (const std::out_of_range& e) {           \
     g_err = e.what();                            \
     return -3;                                   \
+  }                                              \
+  catch (const std::domain_error& e) {           \
+    g_err = e.what();                            \
+    return -2;                                   \
   }                                              \
   catch (const std::exception& e) {              \
     g_err = e.what();                            \
@@ -1574,14 +1625,11 @@ static uint32_t adjoint_tile_bits(uint32_t n) {
 }
 }  // namespace qsim
 
-int qsim_sv_backprop(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, const double* pool, uint64_t pool_len,
-                     uint64_t n_terms, const uint64_t* xm, const uint64_t* zm, const uint32_t* ny, const double* cre,
-                     const double* cim, uint64_t n_params, double* energy, double* grads) {
-  API_BEGIN
-  require(sv != nullptr && energy != nullptr && (grads != nullptr || n_params == 0), "null argument");
-  require(sv->n >= 4, "backprop needs at least 4 qubits");
-  auto gs = parse_gates(gates, n_gates, pool, pool_len);
-  check_qubits(gs, sv->n);
+namespace qsim {
+// Gates the adjoint accepts (ParametricQuantumCircuit::backprop rejects other
+// parametric gates): parametric gates are Pauli rotations with a valid
+// parameter index; every gate has at most 3 non-diagonal targets.
+static void validate_backprop_gates(const std::vector<HGate>& gs, uint64_t n_params) {
   for (size_t i = 0; i < gs.size(); ++i) {
     const HGate& g = gs[i];
     if (!g.parametric()) continue;
@@ -1609,71 +1657,189 @@ int qsim_sv_backprop(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, cons
     }
     if (nd > 3) throw std::invalid_argument("backprop: gates with more than 3 non-diagonal targets");
   }
-  const uint64_t bytes = sv->dim * sizeof(double2);
-  // forward: psi = U psi_in
-  sv_apply_gates(sv, gs);
-  // lambda = 2 H psi; <H> = Re<psi|lambda> / 2
+}
+
+// lambda = 2 H psi (one sweep over Pauli groups); returns <H> = Re<psi|lambda>/2
+static double lambda_and_energy(qsim_sv* sv, const double2* psi, double2* lam, uint64_t n_terms, const uint64_t* xm,
+                                const uint64_t* zm, const uint32_t* ny, const double* cre, const double* cim) {
   std::vector<PauliGroup> G;
   std::vector<PauliTermZ> T;
   const uint32_t ng = build_pauli_groups(n_terms, xm, zm, ny, cre, cim, 2.0, G, T);
-  sv->adj_lam.ensure(bytes);
-  sv->adj_psi.ensure(bytes);
   upload(sv->adj_groups, G.data(), G.size() * sizeof(PauliGroup));
   upload(sv->adj_terms, T.data(), T.size() * sizeof(PauliTermZ));
   const unsigned grid = reduce_grid(sv->dim);
   sv->part.ensure(1, grid);
-  HIPCHECK(launch_pauli_sum(sv->ptr(), sv->adj_lam.as<double2>(), sv->dim, sv->adj_groups.as<PauliGroup>(), ng,
-                            sv->adj_terms.as<PauliTermZ>(), sv->part.d.as<double>(), grid, stream()));
+  HIPCHECK(launch_pauli_sum(psi, lam, sv->dim, sv->adj_groups.as<PauliGroup>(), ng, sv->adj_terms.as<PauliTermZ>(),
+                            sv->part.d.as<double>(), grid, stream()));
   sv->part.fetch(1, grid);
   DDAcc eacc;
   sv->part.accumulate(1, grid, &eacc);
-  *energy = eacc.value() / 2.0;
-  HIPCHECK(hipMemcpyAsync(sv->adj_psi.p, sv->d.p, bytes, hipMemcpyDeviceToDevice, stream()));
-  // reverse sweep: gradient point, then the inverse, gate by gate from the end
+  return eacc.value() / 2.0;
+}
+
+// Reverse sweep over gates gs[b, e) of one state pair: psi_w holds the state
+// after gate e-1 and lambda the costate there; on return both are pulled back
+// to before gate b and the gradient points of the range are added to gacc.
+static void reverse_sweep(qsim_sv* sv, double2* psi_w, double2* lam, const std::vector<HGate>& gs, size_t b, size_t e,
+                          std::vector<DDAcc>& gacc) {
   std::vector<int> qmap(sv->n);
   std::iota(qmap.begin(), qmap.end(), 0);
   std::vector<LGate> lg;
-  for (size_t i = gs.size(); i-- > 0;) {
+  for (size_t i = e; i-- > b;) {
     if (gs[i].parametric()) lg.push_back(grad_point(gs[i]));
     lg.push_back(lower_gate(inverse_gate(gs[i]), qmap));
   }
+  if (lg.empty()) return;
+  const uint32_t tb = adjoint_tile_bits(sv->n);
+  Program P;
+  plan_reverse(P, lg, sv->n, tb, adjoint_wave_ok(lg, sv->n));
+  // renumber gradient points per pass
+  std::vector<std::vector<int32_t>> params(P.passes.size());
+  uint32_t maxpts = 0;
+  for (size_t pi = 0; pi < P.passes.size(); ++pi) {
+    for (uint32_t o = P.passes[pi].op_begin; o < P.passes[pi].op_end; ++o)
+      if (P.ops[o].kind == OP_GRAD) {
+        params[pi].push_back((int32_t)P.ops[o].gidx);
+        P.ops[o].gidx = (uint32_t)(params[pi].size() - 1);
+      }
+    maxpts = std::max<uint32_t>(maxpts, (uint32_t)params[pi].size());
+  }
+  P.upload_all();
+  uint64_t maxtiles = 1;
+  for (size_t pi = 0; pi < P.passes.size(); ++pi) maxtiles = std::max(maxtiles, reverse_tiles(P, pi, sv->n));
+  sv->adj_part.ensure(std::max<uint64_t>(1, (uint64_t)maxpts * maxtiles) * 2 * sizeof(double));
+  sv->adj_out.ensure(std::max<uint32_t>(1, maxpts) * 2 * sizeof(double));
+  sv->adj_stage.ensure((uint64_t)std::max<uint32_t>(1, maxpts) * kGradReduceChunks * 2 * sizeof(double));
+  std::vector<double> hout(2 * std::max<uint32_t>(1, maxpts));
+  for (size_t pi = 0; pi < P.passes.size(); ++pi) {
+    HIPCHECK(launch_reverse_pass(P, pi, sv->n, psi_w, lam, sv->adj_part.as<double>()));
+    const uint32_t np = (uint32_t)params[pi].size();
+    if (np == 0) continue;
+    HIPCHECK(launch_grad_reduce(sv->adj_part.as<double>(), reverse_tiles(P, pi, sv->n), np, sv->adj_out.as<double>(),
+                                sv->adj_stage.as<double>(), stream()));
+    HIPCHECK(hipMemcpyAsync(hout.data(), sv->adj_out.p, np * 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    HIPCHECK(hipStreamSynchronize(stream()));
+    for (uint32_t k = 0; k < np; ++k) gacc[params[pi][k]].merge(hout[2 * k], hout[2 * k + 1]);
+  }
+}
+}  // namespace qsim
+
+int qsim_sv_backprop(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, const double* pool, uint64_t pool_len,
+                     uint64_t n_terms, const uint64_t* xm, const uint64_t* zm, const uint32_t* ny, const double* cre,
+                     const double* cim, uint64_t n_params, double* energy, double* grads) {
+  API_BEGIN
+  require(sv != nullptr && energy != nullptr && (grads != nullptr || n_params == 0), "null argument");
+  require(sv->n >= 4, "backprop needs at least 4 qubits");
+  auto gs = parse_gates(gates, n_gates, pool, pool_len);
+  check_qubits(gs, sv->n);
+  validate_backprop_gates(gs, n_params);
+  const uint64_t bytes = sv->dim * sizeof(double2);
+  // forward: psi = U psi_in
+  sv_apply_gates(sv, gs);
+  // lambda = 2 H psi; <H> = Re<psi|lambda> / 2
+  sv->adj_lam.ensure(bytes);
+  sv->adj_psi.ensure(bytes);
+  *energy = lambda_and_energy(sv, sv->ptr(), sv->adj_lam.as<double2>(), n_terms, xm, zm, ny, cre, cim);
+  HIPCHECK(hipMemcpyAsync(sv->adj_psi.p, sv->d.p, bytes, hipMemcpyDeviceToDevice, stream()));
+  // reverse sweep: gradient point, then the inverse, gate by gate from the end
   std::vector<DDAcc> gacc(n_params);
-  if (!lg.empty()) {
-    const uint32_t tb = adjoint_tile_bits(sv->n);
-    Program P;
-    plan_reverse(P, lg, sv->n, tb, adjoint_wave_ok(lg, sv->n));
-    // renumber gradient points per pass
-    std::vector<std::vector<int32_t>> params(P.passes.size());
-    uint32_t maxpts = 0;
-    for (size_t pi = 0; pi < P.passes.size(); ++pi) {
-      for (uint32_t o = P.passes[pi].op_begin; o < P.passes[pi].op_end; ++o)
-        if (P.ops[o].kind == OP_GRAD) {
-          params[pi].push_back((int32_t)P.ops[o].gidx);
-          P.ops[o].gidx = (uint32_t)(params[pi].size() - 1);
-        }
-      maxpts = std::max<uint32_t>(maxpts, (uint32_t)params[pi].size());
-    }
-    P.upload_all();
-    uint64_t maxtiles = 1;
-    for (size_t pi = 0; pi < P.passes.size(); ++pi) maxtiles = std::max(maxtiles, reverse_tiles(P, pi, sv->n));
-    sv->adj_part.ensure(std::max<uint64_t>(1, (uint64_t)maxpts * maxtiles) * 2 * sizeof(double));
-    sv->adj_out.ensure(std::max<uint32_t>(1, maxpts) * 2 * sizeof(double));
-    sv->adj_stage.ensure((uint64_t)std::max<uint32_t>(1, maxpts) * kGradReduceChunks * 2 * sizeof(double));
-    std::vector<double> hout(2 * std::max<uint32_t>(1, maxpts));
-    for (size_t pi = 0; pi < P.passes.size(); ++pi) {
-      HIPCHECK(launch_reverse_pass(P, pi, sv->n, sv->adj_psi.as<double2>(), sv->adj_lam.as<double2>(),
-                                   sv->adj_part.as<double>()));
-      const uint32_t np = (uint32_t)params[pi].size();
-      if (np == 0) continue;
-      HIPCHECK(launch_grad_reduce(sv->adj_part.as<double>(), reverse_tiles(P, pi, sv->n), np,
-                                  sv->adj_out.as<double>(), sv->adj_stage.as<double>(), stream()));
-      HIPCHECK(hipMemcpyAsync(hout.data(), sv->adj_out.p, np * 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
-      HIPCHECK(hipStreamSynchronize(stream()));
-      for (uint32_t k = 0; k < np; ++k) gacc[params[pi][k]].merge(hout[2 * k], hout[2 * k + 1]);
-    }
+  reverse_sweep(sv, sv->adj_psi.as<double2>(), sv->adj_lam.as<double2>(), gs, 0, gs.size(), gacc);
+  HIPCHECK(hipStreamSynchronize(stream()));
+  for (uint64_t k = 0; k < n_params; ++k) grads[k] = gacc[k].value() / 2.0;
+  API_END
+}
+
+// Adjoint with compressed forward checkpoints: the forward pass stores the
+// state at the start of every segment of `segment_gates` gates in a
+// compressed store; the reverse sweep restarts each segment from its
+// checkpoint (decode, forward over the segment) instead of carrying the
+// uncomputed state through the whole circuit, so codec or rounding drift of
+// the uncomputed state never crosses a segment boundary.
+struct qsim_adj {
+  uint32_t n = 0;
+  uint64_t seg = 0;
+  std::vector<qsim::HGate> gs;
+  std::vector<qsim_cs*> ckpt;
+  bool ready = false;
+  void clear() {
+    for (auto* c : ckpt) qsim_cs_destroy(c);
+    ckpt.clear();
+    ready = false;
+  }
+  ~qsim_adj() { clear(); }
+};
+
+int qsim_adj_create(qsim_adj** out) {
+  API_BEGIN
+  require(out != nullptr, "null argument");
+  *out = new qsim_adj();
+  API_END
+}
+
+int qsim_adj_destroy(qsim_adj* adj) {
+  API_BEGIN
+  delete adj;
+  API_END
+}
+
+int qsim_adj_forward(qsim_adj* adj, qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, const double* pool,
+                     uint64_t pool_len, uint64_t segment_gates, uint32_t block_size, const qsim_cs_config* codec) {
+  API_BEGIN
+  require(adj != nullptr && sv != nullptr && codec != nullptr, "null argument");
+  require(segment_gates >= 1, "segment_gates must be at least 1");
+  require(sv->n >= 5, "checkpointed adjoint needs at least 5 qubits");
+  adj->clear();
+  adj->gs = parse_gates(gates, n_gates, pool, pool_len);
+  check_qubits(adj->gs, sv->n);
+  adj->n = sv->n;
+  adj->seg = segment_gates;
+  const uint32_t B = std::min<uint32_t>(std::max<uint32_t>(block_size, 5), sv->n);
+  const uint64_t G = adj->gs.size();
+  for (uint64_t b = 0; b < G || b == 0; b += segment_gates) {
+    qsim_cs* c = nullptr;
+    if (qsim_cs_create(sv->n, B, codec, &c) != 0) throw std::runtime_error(g_err);
+    adj->ckpt.push_back(c);
+    cs_load_device(c, sv->ptr());
+    apply_gates_raw(sv->ptr(), sv->n, adj->gs, b, std::min<uint64_t>(G, b + segment_gates));
+    if (G == 0) break;
+  }
+  adj->ready = true;
+  API_END
+}
+
+int qsim_adj_backward(qsim_adj* adj, qsim_sv* sv, uint64_t n_terms, const uint64_t* xm, const uint64_t* zm,
+                      const uint32_t* ny, const double* cre, const double* cim, uint64_t n_params, double* energy,
+                      double* grads) {
+  API_BEGIN
+  require(adj != nullptr && sv != nullptr && energy != nullptr && (grads != nullptr || n_params == 0),
+          "null argument");
+  require(adj->ready && adj->n == sv->n, "qsim_adj_backward needs a qsim_adj_forward on this state first");
+  validate_backprop_gates(adj->gs, n_params);
+  const uint64_t bytes = sv->dim * sizeof(double2);
+  sv->adj_lam.ensure(bytes);
+  sv->adj_psi.ensure(bytes);
+  *energy = lambda_and_energy(sv, sv->ptr(), sv->adj_lam.as<double2>(), n_terms, xm, zm, ny, cre, cim);
+  std::vector<DDAcc> gacc(n_params);
+  const uint64_t G = adj->gs.size();
+  for (size_t s = adj->ckpt.size(); s-- > 0;) {
+    const uint64_t b = (uint64_t)s * adj->seg, e = std::min<uint64_t>(G, b + adj->seg);
+    cs_read_device(adj->ckpt[s], sv->adj_psi.as<double2>());
+    apply_gates_raw(sv->adj_psi.as<double2>(), sv->n, adj->gs, b, e);
+    reverse_sweep(sv, sv->adj_psi.as<double2>(), sv->adj_lam.as<double2>(), adj->gs, b, e, gacc);
   }
   HIPCHECK(hipStreamSynchronize(stream()));
   for (uint64_t k = 0; k < n_params; ++k) grads[k] = gacc[k].value() / 2.0;
+  API_END
+}
+
+// bytes held by the checkpoints (payload words and descriptors of each store)
+int qsim_adj_checkpoint_bytes(const qsim_adj* adj, uint64_t* n_checkpoints, uint64_t* bytes) {
+  API_BEGIN
+  require(adj != nullptr, "null argument");
+  if (n_checkpoints) *n_checkpoints = adj->ckpt.size();
+  uint64_t tot = 0;
+  for (auto* c : adj->ckpt) tot += c->cur_bytes;
+  if (bytes) *bytes = tot;
   API_END
 }
 
@@ -1982,6 +2148,92 @@ int qsim_cs_set_basis(qsim_cs* cs, uint64_t index) {
 int qsim_cs_load(qsim_cs* cs, const double* host) {
   API_BEGIN
   cs_load(cs, host, 0);
+  API_END
+}
+
+// CompressedQuantumState::sampling_blockwise (src/cppsim/compressed_state.hpp:1600-1681),
+// single process: block masses from the decoded blocks (compensated on the
+// GPU, where the reference adds sequentially), a multinomial split of the
+// shots over blocks by sequential binomial draws from mt19937_64(seed), per
+// block uniforms from mt19937_64(seed + 1) scaled by the block total and
+// searched in the block's cumulative distribution, then one shuffle with the
+// first generator.  The draws use the same standard-library distributions as
+// the reference, so equal masses and distributions give equal samples.
+int qsim_cs_sampling_blockwise(qsim_cs* cs, uint32_t n_shots, uint32_t seed, uint64_t* out) {
+  API_BEGIN
+  require(cs != nullptr && (out != nullptr || n_shots == 0), "null argument");
+  const uint64_t namps = 1ull << cs->B;
+  HIPCHECK(hipMemcpyAsync(cs->h_null.data(), cs->null_flags.p, cs->nb, hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  // block masses
+  std::vector<double> mass(cs->nb, 0.0);
+  const uint64_t bpb = cs_blocks_per_batch(cs);
+  DBuf stage, pairs;
+  stage.ensure(bpb * kGradReduceChunks * 16);
+  pairs.ensure(bpb * 16);
+  std::vector<double> hp(2 * bpb);
+  for (uint64_t b0 = 0; b0 < cs->nb; b0 += bpb) {
+    const uint64_t cnt = std::min(bpb, cs->nb - b0);
+    bool any = false;
+    for (uint64_t b = b0; b < b0 + cnt; ++b) any = any || !cs->h_null[b];
+    if (!any) continue;
+    std::vector<uint32_t> ids(cnt);
+    std::iota(ids.begin(), ids.end(), (uint32_t)b0);
+    cs_decode_to_scratch(cs, ids);
+    HIPCHECK(launch_block_sumsq(cs->scratch.as<double2>(), namps, (uint32_t)cnt, stage.as<double>(), pairs.as<double>(),
+                                stream()));
+    HIPCHECK(hipMemcpyAsync(hp.data(), pairs.p, cnt * 16, hipMemcpyDeviceToHost, stream()));
+    HIPCHECK(hipStreamSynchronize(stream()));
+    for (uint64_t i = 0; i < cnt; ++i)
+      if (!cs->h_null[b0 + i]) mass[b0 + i] = hp[2 * i] + hp[2 * i + 1];
+  }
+  double total = 0.0;
+  for (double m : mass) total += m;
+  if (!(total > 0.0) || !std::isfinite(total))
+    throw std::domain_error("CompressedQuantumState::sampling_blockwise: state has no finite positive probability mass");
+  // multinomial split of the shots over blocks
+  std::mt19937_64 rng(seed);
+  std::vector<uint32_t> shots(cs->nb, 0);
+  uint32_t left = n_shots;
+  double rest = total;
+  for (size_t g = 0; g < mass.size() && left > 0; ++g) {
+    const double p = rest > 0 ? std::min(1.0, mass[g] / rest) : 0.0;
+    std::binomial_distribution<uint32_t> bin(left, p);
+    const uint32_t k = (g + 1 == mass.size()) ? left : bin(rng);
+    shots[g] = k;
+    left -= k;
+    rest -= mass[g];
+  }
+  // per-block draws
+  std::mt19937_64 rng_local((uint64_t)seed + 1);
+  std::uniform_real_distribution<double> uni(0.0, 1.0);
+  const uint32_t nt = block_scan_tiles(namps);
+  DBuf cdf, tsum, tincl, du, dout;
+  cdf.ensure(namps * 8);
+  tsum.ensure((uint64_t)nt * 8);
+  tincl.ensure((uint64_t)nt * 8);
+  std::vector<uint64_t> mine;
+  mine.reserve(n_shots);
+  std::vector<double> u;
+  for (uint64_t b = 0; b < cs->nb; ++b) {
+    const uint32_t k = shots[b];
+    if (k == 0) continue;
+    u.resize(k);
+    for (uint32_t i = 0; i < k; ++i) u[i] = uni(rng_local);
+    cs_decode_to_scratch(cs, {(uint32_t)b});
+    du.ensure((uint64_t)k * 8);
+    dout.ensure((uint64_t)k * 8);
+    upload(du, u.data(), (uint64_t)k * 8);
+    HIPCHECK(launch_block_sample(cs->scratch.as<double2>(), namps, cdf.as<double>(), tsum.as<double>(),
+                                 tincl.as<double>(), du.as<double>(), k, b * namps,
+                                 dout.as<unsigned long long>(), stream()));
+    const size_t at = mine.size();
+    mine.resize(at + k);
+    HIPCHECK(hipMemcpyAsync(mine.data() + at, dout.p, (uint64_t)k * 8, hipMemcpyDeviceToHost, stream()));
+    HIPCHECK(hipStreamSynchronize(stream()));
+  }
+  std::shuffle(mine.begin(), mine.end(), rng);
+  std::copy(mine.begin(), mine.end(), out);
   API_END
 }
 

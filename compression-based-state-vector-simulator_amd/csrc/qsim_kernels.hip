@@ -661,6 +661,121 @@ __global__ __launch_bounds__(kThreads) void grad_reduce_chunks_kernel(const doub
                  stage + 2 * ((uint64_t)blockIdx.y * gridDim.x + blockIdx.x));
 }
 
+// ------------------------------------------------------ block-wise sampling
+// Per-block probability mass for CompressedQuantumState::sampling_blockwise
+// (src/cppsim/compressed_state.hpp:1612-1622): block (x, y) adds |a|^2 over
+// chunk x of decoded block y (compensated) into stage pairs; a
+// grad_reduce_kernel launch then folds the chunks of each block.
+__global__ __launch_bounds__(kThreads) void block_sumsq_kernel(const double2* __restrict__ a, uint64_t namps,
+                                                               uint64_t chunk, double* __restrict__ stage) {
+  __shared__ double red_s[kThreads / 64], red_c[kThreads / 64];
+  const double2* A = a + (uint64_t)blockIdx.y * namps;
+  const uint64_t b = (uint64_t)blockIdx.x * chunk, e = std::min<uint64_t>(namps, b + chunk);
+  double s = 0.0, c = 0.0;
+  for (uint64_t i = b + threadIdx.x; i < e; i += kThreads) {
+    const double2 v = A[i];
+    dd_add(s, c, fma(v.x, v.x, v.y * v.y));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int d = 32; d >= 1; d >>= 1) {
+    const double s2 = __shfl_xor(s, d, 64), c2 = __shfl_xor(c, d, 64);
+    dd_merge(s, c, s2, c2);
+  }
+  if (lane == 0) { red_s[wave] = s; red_c[wave] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ss = red_s[0], cc = red_c[0];
+    for (int q = 1; q < kThreads / 64; ++q) dd_merge(ss, cc, red_s[q], red_c[q]);
+    double* o = stage + 2 * ((uint64_t)blockIdx.y * gridDim.x + blockIdx.x);
+    o[0] = ss;
+    o[1] = cc;
+  }
+}
+
+// Cumulative distribution of one decoded block in two levels
+// (compressed_state.hpp:1654-1656 builds it sequentially): each workgroup
+// scans kScanTile consecutive |a|^2 into cdf_local (inclusive, within the
+// tile) and stores its tile total; cdf[j] = tile_prefix[tile(j)] + cdf_local[j].
+constexpr uint32_t kScanPer = 16, kScanTile = kThreads * kScanPer;
+__global__ __launch_bounds__(kThreads) void block_scan_kernel(const double2* __restrict__ a, uint64_t namps,
+                                                              double* __restrict__ cdf_local,
+                                                              double* __restrict__ tile_sum) {
+  __shared__ double sh[kThreads];
+  const uint64_t t0 = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanPer;
+  double loc[kScanPer];
+  double run = 0.0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanPer; ++k) {
+    const uint64_t i = t0 + k;
+    double p = 0.0;
+    if (i < namps) {
+      const double2 v = a[i];
+      p = fma(v.x, v.x, v.y * v.y);
+    }
+    run += p;
+    loc[k] = run;
+  }
+  // exclusive scan of the thread totals (Hillis-Steele in LDS)
+  sh[threadIdx.x] = run;
+  __syncthreads();
+  for (uint32_t d = 1; d < kThreads; d <<= 1) {
+    const double add = threadIdx.x >= d ? sh[threadIdx.x - d] : 0.0;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  const double off = threadIdx.x ? sh[threadIdx.x - 1] : 0.0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanPer; ++k) {
+    const uint64_t i = t0 + k;
+    if (i < namps) cdf_local[i] = off + loc[k];
+  }
+  if (threadIdx.x == kThreads - 1) tile_sum[blockIdx.x] = sh[kThreads - 1];
+}
+
+// One thread: tile_incl[t] = sum of tile_sum[0..t] (in order); total at the end.
+__global__ void block_scan_tiles_kernel(const double* __restrict__ tile_sum, uint32_t ntile,
+                                        double* __restrict__ tile_incl) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double acc = 0.0;
+  for (uint32_t t = 0; t < ntile; ++t) {
+    acc += tile_sum[t];
+    tile_incl[t] = acc;
+  }
+}
+
+// One thread per shot: r = u * total, index = first j with cdf[j] >= r
+// (std::lower_bound, compressed_state.hpp:1659-1662), clamped to the block.
+__global__ void block_search_kernel(const double* __restrict__ cdf_local, const double* __restrict__ tile_incl,
+                                    uint32_t ntile, uint64_t namps, const double* __restrict__ u, uint32_t nshots,
+                                    uint64_t base, unsigned long long* __restrict__ out) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nshots) return;
+  const double r = u[s] * tile_incl[ntile - 1];
+  // first tile whose inclusive total reaches r
+  uint32_t lo = 0, hi = ntile;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (tile_incl[mid] < r) lo = mid + 1;
+    else hi = mid;
+  }
+  uint64_t j;
+  if (lo >= ntile) {
+    j = namps;
+  } else {
+    const double off = lo ? tile_incl[lo - 1] : 0.0;
+    uint64_t a = (uint64_t)lo * kScanTile, b = std::min<uint64_t>(namps, a + kScanTile);
+    while (a < b) {
+      const uint64_t mid = (a + b) >> 1;
+      if (off + cdf_local[mid] < r) a = mid + 1;
+      else b = mid;
+    }
+    j = a;
+  }
+  if (j >= namps) j = namps - 1;
+  out[s] = base + j;
+}
+
 // acc[map[k]] += pt[k] (compensated pairs) for the gradient points of one
 // pass, in order: several points of a shared parameter add up
 // deterministically.  One thread; a pass has at most a few hundred points.
@@ -882,6 +997,36 @@ hipError_t launch_adjoint_pass(int tb, const PassArgs* d_args, double2* psi, dou
 hipError_t launch_accumulate_points(const double* pt, uint32_t np, const int32_t* map, double* acc, hipStream_t st) {
   if (np == 0) return hipSuccess;
   hipLaunchKernelGGL(accumulate_points_kernel, dim3(1), dim3(64), 0, st, pt, np, map, acc);
+  return hipGetLastError();
+}
+
+// masses of nblocks consecutive decoded blocks of namps amplitudes into
+// out[2 b .. 2 b + 1] (compensated pairs); stage holds nblocks *
+// kGradReduceChunks pairs
+hipError_t launch_block_sumsq(const double2* a, uint64_t namps, uint32_t nblocks, double* stage, double* out,
+                              hipStream_t st) {
+  if (nblocks == 0) return hipSuccess;
+  const uint64_t nch = std::max<uint64_t>(1, std::min<uint64_t>(kGradReduceChunks, (namps + 4095) / 4096));
+  const uint64_t chunk = (namps + nch - 1) / nch;
+  hipLaunchKernelGGL(block_sumsq_kernel, dim3((unsigned)nch, nblocks), dim3(kThreads), 0, st, a, namps, chunk, stage);
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3(nblocks), dim3(kThreads), 0, st, (const double*)stage, nch, out);
+  return hipGetLastError();
+}
+
+// tiles of the two-level block CDF
+uint32_t block_scan_tiles(uint64_t namps) { return (uint32_t)((namps + kScanTile - 1) / kScanTile); }
+
+// Samples of one decoded block: u holds nshots uniforms in [0, 1) on the
+// device, out receives base + index per shot.  cdf_local: namps doubles;
+// tile_sum, tile_incl: block_scan_tiles(namps) doubles each.
+hipError_t launch_block_sample(const double2* a, uint64_t namps, double* cdf_local, double* tile_sum, double* tile_incl,
+                               const double* u, uint32_t nshots, uint64_t base, unsigned long long* out,
+                               hipStream_t st) {
+  const uint32_t nt = block_scan_tiles(namps);
+  hipLaunchKernelGGL(block_scan_kernel, dim3(nt), dim3(kThreads), 0, st, a, namps, cdf_local, tile_sum);
+  hipLaunchKernelGGL(block_scan_tiles_kernel, dim3(1), dim3(64), 0, st, (const double*)tile_sum, nt, tile_incl);
+  hipLaunchKernelGGL(block_search_kernel, dim3((nshots + 255) / 256), dim3(256), 0, st, (const double*)cdf_local,
+                     (const double*)tile_incl, nt, namps, u, nshots, base, out);
   return hipGetLastError();
 }
 

@@ -1038,17 +1038,40 @@ __global__ __launch_bounds__(128, 2) void wave_pair_adjoint_kernel(const PassArg
       }
       __syncthreads();
       if (is_lam) {
+        // Re(i^kr conj(l) x) times the Z/Y sign (-1)^pc(q & zt) of the
+        // partner index q: i^kr picks Re(conj(l) x) or Im(conj(l) x) and an
+        // overall sign; the per-element sign is a flip of the high dword
         const uint32_t fx = (uint32_t)op.gfx, zt = (uint32_t)op.gzt;
-        const int kr = (int)((1u + op.nY) & 3u);  // i * i^nY = i^kr
-        const bool bneg = __popcll(base & op.gzb) & 1;
-        double s = 0.0, c = 0.0;
+        const uint32_t kr = (1u + op.nY) & 3u;  // i * i^nY = i^kr
+        const bool flip_all = (__popcll(base & op.gzb) & 1) ^ (kr == 1u || kr == 2u);
+        const uint32_t lsgn = (uint32_t)(__popc(((uint32_t)lane ^ fx) & zt & 63u) & 1) << 31;
+        double acc0 = 0.0, acc1 = 0.0;
+        if ((kr & 1u) == 0u) {
 #pragma unroll
-        for (int v = 0; v < kV; ++v) {
-          const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
-          dd_acc(s, c, grad_term(amp[v], xb[q], bneg ^ (bool)(__popc(q & zt) & 1), kr));
+          for (int v = 0; v < kV; ++v) {
+            const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
+            const double2 x = xb[q], l = amp[v];
+            const uint32_t sg = lsgn ^ ((uint32_t)(__popc((((uint32_t)v << 6) ^ (fx & ~63u)) & zt) & 1) << 31);
+            double t = fma(l.x, x.x, l.y * x.y);
+            t = __longlong_as_double(__double_as_longlong(t) ^ ((long long)sg << 32));
+            if (v & 1) acc1 += t;
+            else acc0 += t;
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < kV; ++v) {
+            const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
+            const double2 x = xb[q], l = amp[v];
+            const uint32_t sg = lsgn ^ ((uint32_t)(__popc((((uint32_t)v << 6) ^ (fx & ~63u)) & zt) & 1) << 31);
+            double t = fma(l.x, x.y, -(l.y * x.x));
+            t = __longlong_as_double(__double_as_longlong(t) ^ ((long long)sg << 32));
+            if (v & 1) acc1 += t;
+            else acc0 += t;
+          }
         }
-        // tile sum: the lanes' compensated sums added in a DPP tree
-        const double tot = __shfl(wave_sum_f64(s + c), 63, 64);
+        // tile sum: the lanes' sums added in a DPP tree
+        double tot = __shfl(wave_sum_f64(acc0 + acc1), 63, 64);
+        tot = flip_all ? -tot : tot;
         if (lane == 0) {
           partials[((uint64_t)op.gidx * ntiles + wt) * 2] = tot;
           partials[((uint64_t)op.gidx * ntiles + wt) * 2 + 1] = 0.0;

@@ -98,6 +98,11 @@ _SIGS = {
     "qsim_batch_size": (_u64, [_vp]),
     "qsim_batch_max_params": (_u64, [_vp]),
     "qsim_sv_backprop": (ctypes.c_int, [_vp, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _vp, _vp, _u64, _P(_dbl), _vp]),
+    "qsim_adj_create": (ctypes.c_int, [_P(_vp)]),
+    "qsim_adj_destroy": (ctypes.c_int, [_vp]),
+    "qsim_adj_forward": (ctypes.c_int, [_vp, _vp, _vp, _u64, _vp, _u64, _u64, _u32, _P(CsConfig)]),
+    "qsim_adj_backward": (ctypes.c_int, [_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _u64, _P(_dbl), _vp]),
+    "qsim_adj_checkpoint_bytes": (ctypes.c_int, [_vp, _P(_u64), _P(_u64)]),
     "qsim_sv_device_ptr": (_vp, [_vp]),
     "qsim_sv_qubit_count": (_u32, [_vp]),
     "qsim_cs_default_config": (ctypes.c_int, [_P(CsConfig)]),
@@ -114,6 +119,7 @@ _SIGS = {
     "qsim_cs_set_basis": (ctypes.c_int, [_vp, _u64]),
     "qsim_cs_load": (ctypes.c_int, [_vp, _vp]),
     "qsim_cs_null_bitmap": (ctypes.c_int, [_vp, _vp]),
+    "qsim_cs_sampling_blockwise": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp]),
     "qsim_cs_block_bytes": (ctypes.c_int, [_vp, _vp]),
     "qsim_cs_bytes_history": (ctypes.c_int, [_vp, _vp, _u64, _P(_u64)]),
     "qsim_cs_max_sv_bytes": (ctypes.c_int, [_vp, _P(_u64)]),

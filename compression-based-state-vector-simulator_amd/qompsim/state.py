@@ -245,6 +245,19 @@ class CompressedQuantumState:
             raise ValueError("vector length does not match the qubit count")
         call("qsim_cs_load", self._h, ptr(v))
 
+    def sampling_blockwise(self, sampling_count, random_seed):
+        """Basis-state samples drawn block by block
+        (CompressedQuantumState::sampling_blockwise,
+        src/cppsim/compressed_state.hpp:1600-1681): a list of
+        `sampling_count` indices; ValueError when the state has no finite
+        positive probability mass."""
+        n = int(sampling_count)
+        if n < 0 or n >= 1 << 32:
+            raise ValueError("sampling_count out of range")
+        out = np.zeros(max(1, n), dtype=np.uint64)
+        call("qsim_cs_sampling_blockwise", self._h, n, int(random_seed) & 0xFFFFFFFF, ptr(out))
+        return [int(x) for x in out[:n]]
+
     def get_null_bitmap(self):
         out = np.empty(self.nblocks, dtype=np.uint8)
         call("qsim_cs_null_bitmap", self._h, ptr(out))

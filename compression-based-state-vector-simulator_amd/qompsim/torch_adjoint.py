@@ -1,22 +1,29 @@
 """torch.autograd.Function over the GPU adjoint: <H>(theta) for a
-ParametricQuantumCircuit, differentiable in theta.  The forward call runs
-qsim_sv_backprop once (energy and the full gradient together); backward scales
-the stored gradient by the incoming grad_output.  Parameters live in a float64
-CPU or GPU tensor of length get_parameter_count().
+ParametricQuantumCircuit, differentiable in theta.  Parameters live in a
+float64 CPU or GPU tensor of length get_parameter_count().
+
+Without checkpoints the forward call runs qsim_sv_backprop once (energy and
+the full gradient together) and backward scales the stored gradient.  With
+`segment_gates` the forward call runs forward_with_checkpoints and returns
+<H>; backward runs the checkpointed reverse sweep (qsim_adj_backward).
 """
 import numpy as np
 import torch
 
-from .adjoint import adjoint_gradient
+from .adjoint import CheckpointedAdjoint, adjoint_gradient
+
+
+def _set_theta(theta, circuit):
+    th = theta.detach().to("cpu", torch.float64).numpy()
+    if th.shape != (circuit.get_parameter_count(),):
+        raise ValueError("theta must hold one value per circuit parameter")
+    circuit.set_parameters(th)
 
 
 class AdjointExpectation(torch.autograd.Function):
     @staticmethod
     def forward(ctx, theta, circuit, obs):
-        th = theta.detach().to("cpu", torch.float64).numpy()
-        if th.shape != (circuit.get_parameter_count(),):
-            raise ValueError("theta must hold one value per circuit parameter")
-        circuit.set_parameters(th)
+        _set_theta(theta, circuit)
         energy, grads = adjoint_gradient(circuit, obs)
         ctx.save_for_backward(torch.as_tensor(np.asarray(grads), dtype=theta.dtype, device=theta.device))
         return torch.as_tensor(energy, dtype=theta.dtype, device=theta.device)
@@ -27,6 +34,29 @@ class AdjointExpectation(torch.autograd.Function):
         return grad_output * grads, None, None
 
 
-def expectation(theta, circuit, obs):
-    """<H> as a differentiable function of theta."""
-    return AdjointExpectation.apply(theta, circuit, obs)
+class CheckpointedExpectation(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta, circuit, obs, segment_gates, block_size, codec):
+        _set_theta(theta, circuit)
+        adj = CheckpointedAdjoint(circuit, segment_gates, block_size, codec)
+        psi = adj.forward_with_checkpoints()
+        energy = obs.get_expectation_value(psi).real
+        ctx.adj, ctx.obs = adj, obs
+        ctx.meta = (theta.dtype, theta.device)
+        return torch.as_tensor(energy, dtype=theta.dtype, device=theta.device)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        _, grads = ctx.adj.adjoint(ctx.obs)
+        dtype, device = ctx.meta
+        g = torch.as_tensor(np.asarray(grads), dtype=dtype, device=device)
+        ctx.adj = None
+        return grad_output * g, None, None, None, None, None
+
+
+def expectation(theta, circuit, obs, segment_gates=None, block_size=20, codec=None):
+    """<H> as a differentiable function of theta; with `segment_gates` the
+    backward pass uses compressed forward checkpoints every that many gates."""
+    if segment_gates is None:
+        return AdjointExpectation.apply(theta, circuit, obs)
+    return CheckpointedExpectation.apply(theta, circuit, obs, int(segment_gates), int(block_size), codec)

@@ -115,6 +115,7 @@ int qsim_sv_backprop(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, cons
                      uint64_t pool_len, uint64_t n_terms, const uint64_t* x_masks, const uint64_t* z_masks,
                      const uint32_t* y_counts, const double* coef_re, const double* coef_im,
                      uint64_t n_params, double* energy, double* grads);
+
 void* qsim_sv_device_ptr(qsim_sv* sv);
 uint32_t qsim_sv_qubit_count(const qsim_sv* sv);
 
@@ -163,6 +164,12 @@ int qsim_cs_read_block(qsim_cs* cs, uint64_t block, double* host_amplitudes);
 int qsim_cs_set_basis(qsim_cs* cs, uint64_t index);
 int qsim_cs_load(qsim_cs* cs, const double* host_amplitudes);
 int qsim_cs_null_bitmap(const qsim_cs* cs, uint8_t* out);
+/* sampling_blockwise (compressed_state.hpp:1600-1681): n_shots basis indices
+ * into out, drawn with seed as the reference does in one process (block
+ * masses, binomial split over blocks from mt19937_64(seed), per-block
+ * uniforms from mt19937_64(seed + 1), shuffle).  A state without finite
+ * positive mass fails with status -2 (the reference's std::domain_error). */
+int qsim_cs_sampling_blockwise(qsim_cs* cs, uint32_t n_shots, uint32_t seed, uint64_t* out);
 int qsim_cs_block_bytes(qsim_cs* cs, uint64_t* out);
 /* Total compressed bytes after the initial state and after every stage
  * (append_total_SV_bytes, compressed_state.hpp:374-437); history[0] is the
@@ -185,6 +192,26 @@ int qsim_cs_norm_squared(qsim_cs* cs, double* out);
 int qsim_cs_inner_product_sv(qsim_cs* cs, const qsim_sv* sv, double* re, double* im);
 uint32_t qsim_cs_qubit_count(const qsim_cs* cs);
 uint32_t qsim_cs_block_size(const qsim_cs* cs);
+
+/* Adjoint with compressed forward checkpoints (the forward_with_checkpoints(k)
+ * / adjoint(params, obs, grad_out) pair; the reference has no counterpart, it
+ * extends ParametricQuantumCircuit::backprop, parametric_circuit.cpp:281-308).
+ * qsim_adj_forward applies the circuit to `sv` and stores, in compressed
+ * stores of `block_size` with the codec settings `codec`, the state at the
+ * start of every segment of `segment_gates` gates.  qsim_adj_backward then
+ * forms lambda = 2 H psi from `sv` (which must still hold the forward
+ * result), and for each segment from the last restores its checkpoint,
+ * recomputes the segment and sweeps it backwards with lambda: energy and
+ * gradients as qsim_sv_backprop, with drift confined to one segment. */
+typedef struct qsim_adj qsim_adj;
+int qsim_adj_create(qsim_adj** out);
+int qsim_adj_destroy(qsim_adj* adj);
+int qsim_adj_forward(qsim_adj* adj, qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, const double* pool,
+                     uint64_t pool_len, uint64_t segment_gates, uint32_t block_size, const qsim_cs_config* codec);
+int qsim_adj_backward(qsim_adj* adj, qsim_sv* sv, uint64_t n_terms, const uint64_t* x_masks, const uint64_t* z_masks,
+                      const uint32_t* y_counts, const double* coef_re, const double* coef_im, uint64_t n_params,
+                      double* energy, double* grads);
+int qsim_adj_checkpoint_bytes(const qsim_adj* adj, uint64_t* n_checkpoints, uint64_t* bytes);
 
 /* ---- synchronisation / timing helpers ---------------------------------- */
 int qsim_synchronize(void);

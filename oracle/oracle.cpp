@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstring>
 #include <set>
+#include <random>
 #include <vector>
 
 namespace {
@@ -544,7 +545,9 @@ void orc_codec_roundtrip(const double* in, uint64_t nd, double er, double ea, in
   *payload_words = tot;
 }
 
+// Blocks hold whole codec chunks (32 amplitudes): 5 <= B <= n, else null.
 void* orc_cs_create(uint32_t n, uint32_t B, double er, double ea, int guard, int zero_mode, double thr) {
+  if (B < 5 || B > n) return nullptr;
   auto* s = new CState();
   s->n = n;
   s->B = B;
@@ -622,6 +625,63 @@ uint64_t orc_cs_bytes_history(void* h, uint64_t* out, uint64_t cap) {
 }
 
 uint64_t orc_cs_total_bytes(void* h) { return total_bytes(*static_cast<CState*>(h)); }
+
+// sampling_blockwise (src/cppsim/compressed_state.hpp:1600-1681), one
+// process: sequential block masses of the decoded blocks, binomial split of
+// the shots from mt19937_64(seed), per-block uniforms from
+// mt19937_64(seed + 1) against the sequential block CDF (lower_bound), and a
+// final shuffle with the first generator.  Returns -2 when the state has no
+// finite positive mass, else 0.
+int orc_cs_sampling_blockwise(void* h, uint32_t n_shots, uint32_t seed, uint64_t* out) {
+  CState& s = *static_cast<CState*>(h);
+  std::vector<cplx> blk(s.namps);
+  std::vector<double> mass(s.nb, 0.0);
+  for (uint64_t b = 0; b < s.nb; ++b) {
+    if (s.blocks[b].null) continue;
+    decode_block(s.blocks[b], s.namps, blk.data());
+    double m = 0.0;
+    for (uint64_t j = 0; j < s.namps; ++j) m += std::norm(blk[j]);
+    mass[b] = m;
+  }
+  double total = 0.0;
+  for (double m : mass) total += m;
+  if (!(total > 0.0) || !std::isfinite(total)) return -2;
+  std::mt19937_64 rng(seed);
+  std::vector<uint32_t> shots(mass.size(), 0);
+  uint32_t left = n_shots;
+  double rest = total;
+  for (size_t g = 0; g < mass.size() && left > 0; g++) {
+    double p = rest > 0 ? std::min(1.0, mass[g] / rest) : 0.0;
+    std::binomial_distribution<uint32_t> bin(left, p);
+    uint32_t k = (g + 1 == mass.size()) ? left : bin(rng);
+    shots[g] = k;
+    left -= k;
+    rest -= mass[g];
+  }
+  std::vector<uint64_t> mine;
+  std::mt19937_64 rng_local((uint64_t)seed + 1);
+  std::uniform_real_distribution<double> uni(0.0, 1.0);
+  std::vector<double> cdf(s.namps);
+  for (uint64_t b = 0; b < s.nb; b++) {
+    uint32_t k = shots[b];
+    if (k == 0) continue;
+    decode_block(s.blocks[b], s.namps, blk.data());
+    double acc = 0.0;
+    for (uint64_t j = 0; j < s.namps; j++) {
+      acc += std::norm(blk[j]);
+      cdf[j] = acc;
+    }
+    for (uint32_t t = 0; t < k; t++) {
+      double r = uni(rng_local) * acc;
+      uint64_t j = std::lower_bound(cdf.begin(), cdf.end(), r) - cdf.begin();
+      if (j >= s.namps) j = s.namps - 1;
+      mine.push_back(b * s.namps + j);
+    }
+  }
+  std::shuffle(mine.begin(), mine.end(), rng);
+  std::copy(mine.begin(), mine.end(), out);
+  return 0;
+}
 
 // Adjoint gradient: psi = U|init>, lambda = 2 H psi, reverse sweep.
 void orc_backprop(uint32_t n, const double* init, const void* gates, uint64_t ng, const double* pool, uint64_t nt,

@@ -36,6 +36,7 @@ for name, res, args in [
     ("orc_cs_null_bitmap", None, [_vp, _vp]),
     ("orc_cs_bytes_history", _u64, [_vp, _vp, _u64]),
     ("orc_cs_total_bytes", _u64, [_vp]),
+    ("orc_cs_sampling_blockwise", _int, [_vp, _u32, _u32, _vp]),
     ("orc_backprop", None, [_u32, _vp, _vp, _u64, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _P(_dbl)]),
 ]:
     f = getattr(lib, name)
@@ -134,6 +135,8 @@ class OracleCompressedState:
         self.n, self.B = n, B
         zm = 1 if zero_mode == "threshold" else 0
         self._h = lib.orc_cs_create(n, B, eps_rel, eps_abs, guard, zm, zero_threshold)
+        if not self._h:
+            raise ValueError("block_size must satisfy 5 <= block_size <= qubit_count")
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -175,6 +178,12 @@ class OracleCompressedState:
         out = np.zeros(1 << (self.n - self.B), dtype=np.uint8)
         lib.orc_cs_null_bitmap(self._h, _p(out))
         return out.astype(bool)
+
+    def sampling_blockwise(self, count, seed):
+        out = np.zeros(max(1, count), dtype=np.uint64)
+        if lib.orc_cs_sampling_blockwise(self._h, count, seed, _p(out)) != 0:
+            raise ValueError("state has no finite positive probability mass")
+        return out[:count].tolist()
 
     def bytes_history(self):
         n = lib.orc_cs_bytes_history(self._h, None, 0)

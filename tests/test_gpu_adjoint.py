@@ -157,3 +157,94 @@ def test_adjoint_engines_agree(n, rng, monkeypatch):
     for eng, (e, g) in got.items():
         assert abs(e - e0) < 1e-11, eng
         assert np.max(np.abs(g - g0)) < 1e-10, eng
+
+
+def _energy(c, obs, th):
+    cp = c.copy()
+    cp.set_parameters(th)
+    s = QuantumState(c.get_qubit_count())
+    cp.update_quantum_state(s)
+    return obs.get_expectation_value(s).real
+
+
+def test_adjoint_vs_parameter_shift():
+    """Pauli rotations: dE/dtheta = (E(theta + pi/2) - E(theta - pi/2)) / 2."""
+    n = 10
+    c = W.hea(n, layers=3, seed=11)
+    obs = W.tfim_observable(n)
+    e, g = adjoint_gradient(c, obs)
+    th = np.asarray(c.get_parameters())
+    for i in (0, 7, len(th) // 2, len(th) - 1):
+        tp, tm = th.copy(), th.copy()
+        tp[i] += np.pi / 2
+        tm[i] -= np.pi / 2
+        shift = (_energy(c, obs, tp) - _energy(c, obs, tm)) / 2
+        assert abs(shift - g[i]) < 1e-10, i
+
+
+@pytest.mark.parametrize("seg", [1, 9, 83, 10**6])
+def test_checkpointed_adjoint_matches_oracle(seg, rng):
+    """forward_with_checkpoints + adjoint: segments of one gate, a few gates,
+    about one layer, and the whole circuit give the oracle's gradients."""
+    from qompsim.adjoint import CheckpointedAdjoint
+
+    n = 12
+    c = W.hea(n, layers=3, seed=5)
+    obs = W.tfim_observable(n)
+    adj = CheckpointedAdjoint(c, seg, block_size=8)
+    adj.forward_with_checkpoints()
+    e, g = adj.adjoint(obs)
+    e0, g0 = O.backprop(c, obs)
+    assert abs(e - e0) < 1e-11
+    assert np.max(np.abs(g - g0)) < 1e-10
+    k, nbytes = adj.checkpoint_info()
+    assert k == max(1, -(-len(c.gate_list) // seg))
+    assert nbytes > 0
+    # a second adjoint call on the same forward run gives the same result
+    e2, g2 = adj.adjoint(obs)
+    assert e2 == e and np.array_equal(g2, g)
+
+
+def test_checkpointed_adjoint_lossy_codec_bound(rng):
+    """Checkpoints with an absolute bound of 1e-7 per double: each segment
+    restarts from a state within that bound, so the gradient error stays of
+    the bound's order (here well under 1e-4) while the plain sweep is exact."""
+    from qompsim.adjoint import CheckpointedAdjoint, checkpoint_codec
+
+    n = 12
+    c = _mixed_circuit(n, rng, layers=3)
+    obs = _mixed_observable(n, rng)
+    adj = CheckpointedAdjoint(c, 20, block_size=8, codec=checkpoint_codec(eps_rel=0.0, eps_abs=1e-7))
+    adj.forward_with_checkpoints()
+    e, g = adj.adjoint(obs)
+    e0, g0 = O.backprop(c, obs)
+    assert abs(e - e0) < 1e-11  # lambda comes from the exact forward state
+    err = np.max(np.abs(g - g0))
+    assert err < 1e-4
+
+
+def test_torch_checkpointed_expectation():
+    import torch
+
+    from qompsim.torch_adjoint import expectation
+
+    n = 10
+    c = W.hea(n, layers=4, seed=6)
+    obs = W.tfim_observable(n)
+    th = torch.tensor(c.get_parameters(), dtype=torch.float64, requires_grad=True)
+    E = expectation(th, c, obs, segment_gates=2 * (3 * n - 1))
+    (2.0 * E).backward()
+    e0, g0 = O.backprop(c, obs)
+    assert abs(E.item() - e0) < 1e-11
+    assert np.max(np.abs(th.grad.numpy() - 2.0 * g0)) < 1e-10
+
+
+def test_checkpointed_adjoint_requires_forward():
+    from qompsim import _lib
+    from qompsim.adjoint import CheckpointedAdjoint
+
+    n = 8
+    c = W.hea(n, layers=1, seed=1)
+    adj = CheckpointedAdjoint(c, 5)
+    with pytest.raises(ValueError, match="forward"):
+        adj.adjoint(W.tfim_observable(n))

@@ -135,9 +135,12 @@ def test_compressed_qft_vs_oracle(n, B, I, tb):
     assert len(cs.get_stage_stats()) == cs.get_num_stages()
 
 
-def test_multipass_random_circuit(rng):
+def test_multipass_random_circuit(monkeypatch):
+    # LDS tiles of 2^6 amplitudes on every stage: stages with high qubits
+    # need several passes through the HBM staging buffer
+    monkeypatch.setenv("QOMPSIM_ENGINE", "lds")
     n, B = 13, 7
-    c = random_circuit(n, 120, rng, max_k=2)
+    c = random_circuit(n, 120, np.random.default_rng(77), max_k=2)
     # near-lossless codec settings: the relative budget would otherwise allow
     # errors of eps_rel * |x| per double
     cs = CompressedQuantumState(n, B, eps_rel=1e-12, eps_abs=1e-10, guard_bits=2, tile_bits=6, verbose=False)
@@ -255,3 +258,43 @@ def test_arena_overflow_regrowth_is_exact(rng, monkeypatch):
     cs.apply_circuit()
     assert np.array_equal(cs.get_raw_vector().view(np.uint64), ref.get_raw_vector().view(np.uint64))
     assert cs.get_bytes_history() == ref.get_bytes_history()
+
+
+@pytest.mark.parametrize("n,B", [(14, 8), (18, 12)])
+def test_sampling_blockwise_matches_oracle(n, B, rng):
+    """Same loaded state (bit-exact codec on both sides), same seeds: the GPU
+    draws the oracle's samples; null blocks are never sampled."""
+    v = rand_state(n, rng)
+    v[(np.arange(1 << n) >> B) % 4 == 1] = 0
+    v /= np.linalg.norm(v)
+    cs = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, guard_bits=1, verbose=False)
+    cs.load(v)
+    oc = O.OracleCompressedState(n, B, 1e-3, 1e-7, 1)
+    oc.load(v)
+    for shots, seed in ((1000, 0), (3, 77), (0, 5), (20000, 123456789)):
+        got = cs.sampling_blockwise(shots, seed)
+        assert len(got) == shots
+        assert got == oc.sampling_blockwise(shots, seed), (shots, seed)
+        assert all(((x >> B) % 4) != 1 for x in got)
+
+
+def test_sampling_blockwise_after_circuit():
+    """GHZ-16 run compressed: every sample is 0 or 2^n - 1, about half each."""
+    n, B = 16, 8
+    c = QuantumCircuit(n)
+    c.add_gate(gate.H(0))
+    for i in range(n - 1):
+        c.add_gate(gate.CNOT(i, i + 1))
+    cs = CompressedQuantumState(n, B, verbose=False)
+    cs.partition_and_register_circuit(c, 4)
+    cs.apply_circuit()
+    s = cs.sampling_blockwise(4000, 9)
+    assert set(s) <= {0, (1 << n) - 1}
+    assert abs(s.count(0) - 2000) < 5 * np.sqrt(1000)
+
+
+def test_sampling_blockwise_zero_mass():
+    cs = CompressedQuantumState(10, 5, verbose=False)
+    cs.load(np.zeros(1 << 10, dtype=complex))
+    with pytest.raises(ValueError):
+        cs.sampling_blockwise(4, 1)

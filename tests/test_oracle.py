@@ -252,3 +252,55 @@ def test_oracle_stage_loop_fidelity_and_bound(rng):
     fid = abs(np.vdot(exact, v))
     assert fid > 0.99999
     assert len(cs.bytes_history()) == len(cs.stages()) + 1
+
+
+# ---------------------------------------------------------- block sampling
+def test_sampling_blockwise_basis_and_null_blocks():
+    """A basis state samples only its index; indices of null blocks never
+    appear; the count is exact and the draw is a function of the seed."""
+    n, B = 11, 5
+    cs = O.OracleCompressedState(n, B)
+    v = np.zeros(1 << n, dtype=complex)
+    v[613] = 1.0
+    cs.load(v)
+    assert cs.sampling_blockwise(257, 5) == [613] * 257
+    assert cs.sampling_blockwise(0, 5) == []
+    rng = np.random.default_rng(3)
+    w = rand_state(n, rng)
+    w[(np.arange(1 << n) >> B) % 3 == 0] = 0  # every third block all-zero
+    cs.load(w / np.linalg.norm(w))
+    s1 = cs.sampling_blockwise(4000, 11)
+    assert len(s1) == 4000 and s1 == cs.sampling_blockwise(4000, 11)
+    assert s1 != cs.sampling_blockwise(4000, 12)
+    assert all(((x >> B) % 3) != 0 for x in s1)
+
+
+def test_sampling_blockwise_distribution():
+    """Empirical frequencies match |decoded amplitude|^2 within 5 sigma; the
+    draw is not a permutation of a fixed order (first half and second half
+    agree in distribution)."""
+    n, B = 7, 5
+    rng = np.random.default_rng(8)
+    w = rand_state(n, rng) * np.sqrt(np.linspace(0.2, 2.0, 1 << n))
+    w /= np.linalg.norm(w)
+    cs = O.OracleCompressedState(n, B, eps_rel=1e-12, eps_abs=1e-14, guard=2)
+    cs.load(w)
+    p = np.abs(cs.get_vector(normalise=False)) ** 2
+    p /= p.sum()
+    N = 200000
+    s = np.array(cs.sampling_blockwise(N, 2024))
+    f = np.bincount(s, minlength=1 << n) / N
+    sig = np.sqrt(p * (1 - p) / N)
+    assert np.all(np.abs(f - p) < 5 * sig + 1e-12)
+    h1 = np.bincount(s[: N // 2], minlength=1 << n) / (N // 2)
+    assert np.max(np.abs(h1 - p)) < 6 * np.max(np.sqrt(p * (1 - p) / (N // 2)))
+
+
+def test_sampling_blockwise_zero_mass():
+    n, B = 8, 5
+    cs = O.OracleCompressedState(n, B)
+    cs.load(np.zeros(1 << n, dtype=complex))
+    with pytest.raises(ValueError):
+        cs.sampling_blockwise(10, 1)
+    with pytest.raises(ValueError):
+        O.OracleCompressedState(8, 4)
