@@ -448,7 +448,15 @@ struct Program {
 
   void upload_all() {
     upload(d_ops, ops.data(), ops.size() * sizeof(Op));
-    upload(d_terms, terms.data(), terms.size() * sizeof(PhaseTerm));
+    // kTermGroup never-matching terms past the end: the kernels read the
+    // next group of an op's terms ahead of the current one
+    std::vector<PhaseTerm> tpad(terms);
+    for (uint32_t i = 0; i < kTermGroup; ++i) {
+      PhaseTerm T{};
+      T.bval = 1;
+      tpad.push_back(T);
+    }
+    upload(d_terms, tpad.data(), tpad.size() * sizeof(PhaseTerm));
     upload(d_diags, diags.data(), diags.size() * sizeof(DiagTable));
     upload(d_coefs, coefs.data(), coefs.size() * sizeof(double));
     for (auto& p : passes) {

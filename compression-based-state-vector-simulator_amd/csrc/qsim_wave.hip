@@ -264,6 +264,9 @@ __device__ __forceinline__ uint32_t lane_quantise(double2 (&amp)[kV], const Code
   raw = raw || hmax >= ((uint32_t)(51 + 1023 + e) << 20);  // some |x * 2^-e| >= 2^51 (or non-finite)
   if (raw) return 64u;
   const double M = 6755399441055744.0;  // 1.5 * 2^52
+  // x * 2^-e is exact here (|x * 2^-e| < 2^51 and 2^-e >= 2^-1074), so one
+  // fma rounds x * 2^-e + M exactly as the scaled sum would
+  const double sc = ldexp(1.0, -e);
   uint32_t w;
   if (hmax < ((uint32_t)(30 + 1023 + e) << 20)) {
     // every |x * 2^-e| < 2^30: 32-bit codes; the low dword of the shifted
@@ -271,8 +274,8 @@ __device__ __forceinline__ uint32_t lane_quantise(double2 (&amp)[kV], const Code
     uint32_t orr = 0;
 #pragma unroll
     for (int q = 0; q < kV; ++q) {
-      const int k0 = (int)(uint32_t)__double_as_longlong(ldexp(amp[q].x, -e) + M);
-      const int k1 = (int)(uint32_t)__double_as_longlong(ldexp(amp[q].y, -e) + M);
+      const int k0 = (int)(uint32_t)__double_as_longlong(fma(amp[q].x, sc, M));
+      const int k1 = (int)(uint32_t)__double_as_longlong(fma(amp[q].y, sc, M));
       const uint32_t u0 = ((uint32_t)k0 << 1) ^ (uint32_t)(k0 >> 31), u1 = ((uint32_t)k1 << 1) ^ (uint32_t)(k1 >> 31);
       orr |= u0 | u1;
       amp[q] = make_double2(__longlong_as_double((long long)u0), __longlong_as_double((long long)u1));
@@ -283,8 +286,8 @@ __device__ __forceinline__ uint32_t lane_quantise(double2 (&amp)[kV], const Code
     uint64_t orr = 0;
 #pragma unroll
     for (int q = 0; q < kV; ++q) {
-      const int64_t k0 = __double_as_longlong(ldexp(amp[q].x, -e) + M) - 0x4338000000000000ll;
-      const int64_t k1 = __double_as_longlong(ldexp(amp[q].y, -e) + M) - 0x4338000000000000ll;
+      const int64_t k0 = __double_as_longlong(fma(amp[q].x, sc, M)) - 0x4338000000000000ll;
+      const int64_t k1 = __double_as_longlong(fma(amp[q].y, sc, M)) - 0x4338000000000000ll;
       const uint64_t u0 = ((uint64_t)k0 << 1) ^ (uint64_t)(k0 >> 63), u1 = ((uint64_t)k1 << 1) ^ (uint64_t)(k1 >> 63);
       orr |= u0 | u1;
       amp[q] = make_double2(__longlong_as_double((long long)u0), __longlong_as_double((long long)u1));
@@ -563,12 +566,17 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], c
     double aL = 0.0, aF0 = 0.0, aF1 = 0.0, aF2 = 0.0, aF3 = 0.0;
     uint32_t used = 0;
     bool generic = false;
-    // kTermGroup terms per scalar load round trip (the list is padded)
+    // kTermGroup terms per scalar load round trip (each op's list is padded
+    // to whole groups and the array ends with one spare group), the next
+    // group requested before the current one is applied
     const QC TermBits* tp = (const QC TermBits*)(terms + op.term_off);
-    for (uint32_t i0 = 0; i0 < op.nterms; i0 += kTermGroup) {
-      TermBits tg[kTermGroup];
+    TermBits tg[kTermGroup];
 #pragma unroll
-      for (uint32_t j = 0; j < kTermGroup; ++j) tg[j] = tp[i0 + j];
+    for (uint32_t j = 0; j < kTermGroup; ++j) tg[j] = tp[j];
+    for (uint32_t i0 = 0; i0 < op.nterms; i0 += kTermGroup) {
+      TermBits tn[kTermGroup];
+#pragma unroll
+      for (uint32_t j = 0; j < kTermGroup; ++j) tn[j] = tp[i0 + kTermGroup + j];
 #pragma unroll
       for (uint32_t j = 0; j < kTermGroup; ++j) {
       const uint32_t tmask = tg[j][0], tval = tg[j][1];
@@ -590,6 +598,8 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], c
       aF3 += js == 3 ? f : 0.0;
       used |= 1u << js;
       }
+#pragma unroll
+      for (uint32_t j = 0; j < kTermGroup; ++j) tg[j] = tn[j];
     }
     // each factor is formed right before it is applied; the scheduling
     // barriers keep the compiler from evaluating all five at once
