@@ -28,7 +28,7 @@ namespace qsim {
 
 hipError_t launch_pass(int in_mode, int out_mode, int tb, bool big, const PassArgs* d_args, const PassDyn& dy,
                        uint64_t ntiles, hipStream_t st);
-hipError_t launch_wave(int in_mode, int out_mode, const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles,
+hipError_t launch_wave(int tile_bits, int in_mode, int out_mode, const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles,
                        hipStream_t st);
 hipError_t launch_fill_basis(double2* v, uint64_t n, uint64_t idx, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, uint64_t n, uint8_t val, hipStream_t st);
@@ -722,9 +722,27 @@ static bool wave_engine_ok(const std::vector<LGate>& lg, uint32_t L) {
   return true;
 }
 
+// Compressed programs (codec_ends) that fit one wave pass of 1024-amplitude
+// tiles are re-planned on 512-amplitude tiles (eight rows per lane, three
+// waves per SIMD, more latency hidden in the codec) when that still takes one
+// pass.  QOMPSIM_WAVE_TILE_BITS=10 keeps the larger tile.
+static uint32_t compressed_wave_bits() {
+  const char* e = std::getenv("QOMPSIM_WAVE_TILE_BITS");
+  return (e && std::atoi(e) == (int)kWaveTile) ? kWaveTile : kWaveTileSmall;
+}
+
 static void plan_program(Program& P, const std::vector<LGate>& lg, uint32_t L, uint32_t tmax, uint32_t cmin,
                          bool at_least_one_pass, bool codec_ends = false) {
-  plan_program_engine(P, lg, L, tmax, cmin, at_least_one_pass, codec_ends, wave_engine_ok(lg, L));
+  const bool wave = wave_engine_ok(lg, L);
+  plan_program_engine(P, lg, L, tmax, cmin, at_least_one_pass, codec_ends, wave);
+  if (wave && codec_ends && P.passes.size() == 1 && compressed_wave_bits() == kWaveTileSmall) {
+    Program Q;
+    plan_program_engine(Q, lg, L, tmax, cmin, at_least_one_pass, codec_ends, true, kWaveTileSmall);
+    if (Q.passes.size() == 1) {
+      P.reset();
+      plan_program_engine(P, lg, L, tmax, cmin, at_least_one_pass, codec_ends, true, kWaveTileSmall);
+    }
+  }
   if (std::getenv("QOMPSIM_PLAN_LOG")) {
     fprintf(stderr, "[plan] L=%u gates=%zu passes=%zu\n", L, lg.size(), P.passes.size());
     for (auto& a : P.passes) {
@@ -745,8 +763,8 @@ static hipError_t launch_program_pass(const Program& P, size_t i, int in_mode, i
                                       uint64_t groups, hipStream_t st) {
   const PassArgs& a = P.passes[i];
   if (a.engine == ENGINE_WAVE) {
-    if (a.t != kWaveTile) return hipErrorInvalidValue;
-    return launch_wave(in_mode, out_mode, P.dargs(i), dy, groups << (P.L - kWaveTile), st);
+    if (a.t != kWaveTile && a.t != kWaveTileSmall) return hipErrorInvalidValue;
+    return launch_wave((int)a.t, in_mode, out_mode, P.dargs(i), dy, groups << (P.L - a.t), st);
   }
   return launch_pass(in_mode, out_mode, (int)a.t, a.maxk > 3, P.dargs(i), dy, groups << (P.L - a.t), st);
 }
@@ -1056,7 +1074,7 @@ static void cs_load(qsim_cs* cs, const double* host, uint64_t basis) {
     HIPCHECK(launch_fill_u8(cs->nonzero.as<uint8_t>(), cs->nb, 0, stream()));
     {
       const PassArgs& io = cs->io.passes[0];
-      const uint32_t tb = io.engine == ENGINE_WAVE ? kWaveTile : io.t;
+      const uint32_t tb = io.t;
       cs_reset_out(cs, out, pick_nsub((host ? cs->nb : 1ull) << (cs->B - tb)), attempt > 0);
     }
     if (host) {
@@ -1093,7 +1111,7 @@ static void cs_load_device(qsim_cs* cs, const double2* dev) {
     HIPCHECK(launch_fill_u8(cs->nonzero.as<uint8_t>(), cs->nb, 0, stream()));
     {
       const PassArgs& io = cs->io.passes[0];
-      const uint32_t tb = io.engine == ENGINE_WAVE ? kWaveTile : io.t;
+      const uint32_t tb = io.t;
       cs_reset_out(cs, out, pick_nsub(cs->nb << (cs->B - tb)), attempt > 0);
     }
     upload(cs->bmap, ids.data(), ids.size() * sizeof(uint32_t));
@@ -1232,7 +1250,7 @@ static void cs_run_stage(qsim_cs* cs, uint32_t si) {
     HIPCHECK(launch_fill_u8(cs->nonzero.as<uint8_t>(), cs->nb, 0, stream()));
     {
       const PassArgs& last = P.passes.back();
-      const uint32_t tb = last.engine == ENGINE_WAVE ? kWaveTile : last.t;
+      const uint32_t tb = last.t;
       cs_reset_out(cs, out, pick_nsub(nact << (L - tb)), attempt > 0);
     }
     HIPCHECK(hipEventRecord(cs->ev0, stream()));

@@ -35,6 +35,7 @@ enum OutMode : int { OUT_RAW = 0, OUT_ENC = 1 };
 enum ZeroMode : int { ZERO_EXACT = 0, ZERO_THRESHOLD = 1 };
 enum Engine : uint32_t { ENGINE_LDS = 0, ENGINE_WAVE = 1 };
 constexpr uint32_t kWaveTile = 10;  // wave tile bits: 6 lane bits + 4 register slots
+constexpr uint32_t kWaveTileSmall = 9;  // 6 lane bits + 3 register slots (8 rows per lane)
 // A dispatch packet gives the grid in work-items as 32-bit values: launches of
 // 256-thread work-groups are split at 2^23 groups (2^31 work-items).
 // first-level chunks per gradient point in the two-level partial reduction

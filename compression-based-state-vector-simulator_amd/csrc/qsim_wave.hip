@@ -154,31 +154,36 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 }
 
 // LDS slot of tile position p for the layout transposes: one pad slot per 16
-// positions.  Wave-layout accesses (64v + l) become slot 68v + l + l/16 and
-// chunk-layout accesses (16L + q) slot 17L + q: a per-lane base plus an
-// immediate offset, and free of bank conflicts in both directions.
-constexpr uint32_t kXSlots = 1024 + 64;
+// positions (slot p + p/16).  Wave-layout accesses (64v + l) become slot
+// 68v + l + l/16 and chunk-layout accesses (KV L + q, q < KV) slot
+// KV L + (KV L)/16 + q: a per-lane base plus an immediate offset, and free of
+// bank conflicts in both directions for KV = 16.
+template <int KV>
+constexpr uint32_t x_slots() { return 68u * KV; }
+constexpr uint32_t kXSlots = x_slots<kV>();
 
 // wave layout (lane l, row v: tile position 64v + l) -> chunk layout
-// (lane L, row q: tile position 16L + q), through the wave's LDS buffer
-__device__ __forceinline__ void to_chunk_layout(double2 (&amp)[kV], double2* xb, uint32_t lane) {
+// (lane L, row q: tile position KV L + q), through the wave's LDS buffer
+template <int KV>
+__device__ __forceinline__ void to_chunk_layout(double2 (&amp)[KV], double2* xb, uint32_t lane) {
   double2* wa = xb + lane + (lane >> 4);
 #pragma unroll
-  for (int v = 0; v < kV; ++v) wa[68 * v] = amp[v];
+  for (int v = 0; v < KV; ++v) wa[68 * v] = amp[v];
   __builtin_amdgcn_wave_barrier();
-  const double2* ca = xb + 17u * lane;
+  const double2* ca = xb + KV * lane + ((KV * lane) >> 4);
 #pragma unroll
-  for (int q = 0; q < kV; ++q) amp[q] = ca[q];
+  for (int q = 0; q < KV; ++q) amp[q] = ca[q];
   __builtin_amdgcn_wave_barrier();
 }
-__device__ __forceinline__ void to_wave_layout(double2 (&amp)[kV], double2* xb, uint32_t lane) {
-  double2* ca = xb + 17u * lane;
+template <int KV>
+__device__ __forceinline__ void to_wave_layout(double2 (&amp)[KV], double2* xb, uint32_t lane) {
+  double2* ca = xb + KV * lane + ((KV * lane) >> 4);
 #pragma unroll
-  for (int q = 0; q < kV; ++q) ca[q] = amp[q];
+  for (int q = 0; q < KV; ++q) ca[q] = amp[q];
   __builtin_amdgcn_wave_barrier();
   const double2* wa = xb + lane + (lane >> 4);
 #pragma unroll
-  for (int v = 0; v < kV; ++v) amp[v] = wa[68 * v];
+  for (int v = 0; v < KV; ++v) amp[v] = wa[68 * v];
   __builtin_amdgcn_wave_barrier();
 }
 
@@ -230,24 +235,62 @@ __device__ __forceinline__ void stage_or(uint64_t* stg, uint32_t bit, uint64_t c
 // budget overflows) takes the exact per-double exponent path.
 // rint is the 1.5*2^52 shift (round-to-nearest-even for |s| < 2^51).
 // Every lane of the wave must call it.
-__device__ __forceinline__ uint32_t lane_quantise(double2 (&amp)[kV], const CodecParams& cp, uint32_t ovf_hi) {
+// value of lane ^ 2 (quad_perm [2,3,0,1])
+__device__ __forceinline__ uint32_t swap2_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4e, 0xf, 0xf, false);
+}
+__device__ __forceinline__ double swap2_f64(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint64_t r = ((uint64_t)swap2_u32((uint32_t)(b >> 32)) << 32) | swap2_u32((uint32_t)b);
+  return __longlong_as_double((long long)r);
+}
+// reductions over the 32/KV lanes that share a chunk (lane ^ 1, and lane ^ 2
+// when a chunk spans four lanes)
+template <int KV>
+__device__ __forceinline__ double chunk_fmin(double x) {
+  x = fmin(x, swap1_f64(x));
+  if constexpr (KV == 8) x = fmin(x, swap2_f64(x));
+  return x;
+}
+template <int KV>
+__device__ __forceinline__ uint32_t chunk_umax(uint32_t x) {
+  x = max(x, swap1_u32(x));
+  if constexpr (KV == 8) x = max(x, swap2_u32(x));
+  return x;
+}
+template <int KV>
+__device__ __forceinline__ int chunk_imin(int x) {
+  x = min(x, swap1_i32(x));
+  if constexpr (KV == 8) x = min(x, (int)swap2_u32((uint32_t)x));
+  return x;
+}
+template <int KV>
+__device__ __forceinline__ uint32_t chunk_uor(uint32_t x) {
+  x |= swap1_u32(x);
+  if constexpr (KV == 8) x |= swap2_u32(x);
+  return x;
+}
+
+template <int KV>
+__device__ __forceinline__ uint32_t lane_quantise(double2 (&amp)[KV], const CodecParams& cp, uint32_t ovf_hi) {
+  static_assert(KV == 16 || KV == 8, "a chunk spans two or four lanes");
   double mn = __longlong_as_double(0x7ff0000000000000ll);  // +inf
   uint32_t hmax = 0;
 #pragma unroll
-  for (int q = 0; q < kV; ++q) {
+  for (int q = 0; q < KV; ++q) {
     const uint64_t bx = (uint64_t)__double_as_longlong(amp[q].x), by = (uint64_t)__double_as_longlong(amp[q].y);
     mn = vmin_abs(mn, __longlong_as_double((long long)(bx - 1ull)));
     mn = vmin_abs(mn, __longlong_as_double((long long)(by - 1ull)));
     hmax = max(hmax, max((uint32_t)(bx >> 32) & 0x7fffffffu, (uint32_t)(by >> 32) & 0x7fffffffu));
   }
-  mn = fmin(mn, swap1_f64(mn));
-  hmax = max(hmax, swap1_u32(hmax));
+  mn = chunk_fmin<KV>(mn);
+  hmax = chunk_umax<KV>(hmax);
   int le;
   if (hmax >= ovf_hi) {
     int l2 = 0x7fffffff;
 #pragma unroll
-    for (int q = 0; q < kV; ++q) l2 = min(l2, min(lane_exponent(amp[q].x, cp), lane_exponent(amp[q].y, cp)));
-    le = min(l2, swap1_i32(l2));
+    for (int q = 0; q < KV; ++q) l2 = min(l2, min(lane_exponent(amp[q].x, cp), lane_exponent(amp[q].y, cp)));
+    le = chunk_imin<KV>(l2);
   } else {
     const bool nzc = (uint64_t)__double_as_longlong(mn) != 0x7ff0000000000000ull;
     le = nzc ? lane_exponent(__longlong_as_double(__double_as_longlong(mn) + 1ll), cp) : 0x7fffffff;
@@ -256,7 +299,7 @@ __device__ __forceinline__ uint32_t lane_quantise(double2 (&amp)[kV], const Code
     // all-zero chunk: no payload; its codes must be 0 (a -0.0 bit pattern
     // would otherwise be OR-ed into the next chunk's first word)
 #pragma unroll
-    for (int q = 0; q < kV; ++q) amp[q] = make_double2(0.0, 0.0);
+    for (int q = 0; q < KV; ++q) amp[q] = make_double2(0.0, 0.0);
     return 0u;
   }
   bool raw = le == (int)0x80000000 || le < -1000;
@@ -273,26 +316,26 @@ __device__ __forceinline__ uint32_t lane_quantise(double2 (&amp)[kV], const Code
     // double is the integer in two's complement
     uint32_t orr = 0;
 #pragma unroll
-    for (int q = 0; q < kV; ++q) {
+    for (int q = 0; q < KV; ++q) {
       const int k0 = (int)(uint32_t)__double_as_longlong(fma(amp[q].x, sc, M));
       const int k1 = (int)(uint32_t)__double_as_longlong(fma(amp[q].y, sc, M));
       const uint32_t u0 = ((uint32_t)k0 << 1) ^ (uint32_t)(k0 >> 31), u1 = ((uint32_t)k1 << 1) ^ (uint32_t)(k1 >> 31);
       orr |= u0 | u1;
       amp[q] = make_double2(__longlong_as_double((long long)u0), __longlong_as_double((long long)u1));
     }
-    orr |= swap1_u32(orr);
+    orr = chunk_uor<KV>(orr);
     w = orr ? 32u - (uint32_t)__clz((int)orr) : 0u;
   } else {
     uint64_t orr = 0;
 #pragma unroll
-    for (int q = 0; q < kV; ++q) {
+    for (int q = 0; q < KV; ++q) {
       const int64_t k0 = __double_as_longlong(fma(amp[q].x, sc, M)) - 0x4338000000000000ll;
       const int64_t k1 = __double_as_longlong(fma(amp[q].y, sc, M)) - 0x4338000000000000ll;
       const uint64_t u0 = ((uint64_t)k0 << 1) ^ (uint64_t)(k0 >> 63), u1 = ((uint64_t)k1 << 1) ^ (uint64_t)(k1 >> 63);
       orr |= u0 | u1;
       amp[q] = make_double2(__longlong_as_double((long long)u0), __longlong_as_double((long long)u1));
     }
-    orr |= ((uint64_t)swap1_u32((uint32_t)(orr >> 32)) << 32) | swap1_u32((uint32_t)orr);
+    orr = ((uint64_t)chunk_uor<KV>((uint32_t)(orr >> 32)) << 32) | chunk_uor<KV>((uint32_t)orr);
     w = orr ? 64u - (uint32_t)__clzll(orr) : 0u;
   }
   return w | ((uint32_t)(e & 0xffff) << 8);
@@ -376,6 +419,7 @@ __device__ __forceinline__ void k1_lane_sel(const QC Op& op, double2 (&amp)[KV],
 template <int J, int KV>
 __device__ __forceinline__ void k1_reg(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
   constexpr int MJ = 1 << J;
+  if constexpr (MJ >= KV) return;  // no such register qubit in this tile shape
   if (op.nsel == 0) {
     // one matrix for every row: its entries are loaded once (wave-uniform)
     if (op.ident_mask & 1ull) return;
@@ -408,6 +452,7 @@ __device__ __forceinline__ void k1_reg(const QC Op& op, double2 (&amp)[KV], cons
 template <int J0, int J1, int KV>
 __device__ __forceinline__ void k2_regreg(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
   constexpr int M0 = 1 << J0, M1 = 1 << J1;
+  if constexpr (M0 >= KV || M1 >= KV) return;  // no such register qubit in this tile shape
   const bool uni = op_sel_uniform(op);
 #pragma unroll
   for (int v = 0; v < KV; ++v) {
@@ -442,6 +487,7 @@ template <int J, int KV>
 __device__ __forceinline__ void k2_lanereg(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C,
                                            uint32_t b, uint32_t il) {
   constexpr int MJ = 1 << J;
+  if constexpr (MJ >= KV) return;  // no such register qubit in this tile shape
   const uint32_t ir = 1u - il;
   const uint32_t rl = ((uint32_t)cx.lane >> b) & 1u;
   const int m = 1 << b;
@@ -681,13 +727,24 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], c
 }
 
 // ------------------------------------------------------------- the kernel
-template <int IN, int OUT>
-__global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __restrict__ pa, PassDyn dy, uint64_t ntiles) {
-  // per wave: the layout transposes (kXSlots complex128) and, reused after the
-  // encoder's transpose, the packed payload of the tile (at most 2048 words)
-  // plus slack for the zero-valued spill OR past the last field
-  static_assert(2 * kXSlots >= 2 * 1024 + 8, "staging must fit the transpose buffer");
-  __shared__ __align__(16) uint64_t wbuf[kThreads / 64][2 * kXSlots];
+// KV complex rows per lane: a wave tile of 64 KV amplitudes (lanes on local
+// qubits 0..5, log2 KV register qubits).  KV = 16 is the default tile; KV = 8
+// halves the registers of a wave so that three waves fit a SIMD, for stages
+// whose non-diagonal gates need only three register qubits.  In chunk layout
+// 32/KV lanes share one 32-amplitude codec chunk: lane L holds amplitudes
+// KV g .. KV g + KV - 1 (g = L mod 32/KV) of chunk L / (32/KV).
+template <int IN, int OUT, int KV>
+__global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const PassArgs* __restrict__ pa, PassDyn dy,
+                                                                          uint64_t ntiles) {
+  constexpr int RB = KV == 16 ? 4 : 3;            // register qubits
+  constexpr uint32_t TB = 6 + RB;                 // tile bits
+  constexpr uint32_t LPC = 32 / KV;               // lanes per chunk
+  constexpr uint32_t XS = x_slots<KV>();
+  // per wave: the layout transposes (XS complex128) and, reused after the
+  // encoder's transpose, the packed payload of the tile (at most 128 KV
+  // words) plus slack for the zero-valued spill OR past the last field
+  static_assert(2 * XS >= 128 * KV + 8, "staging must fit the transpose buffer");
+  __shared__ __align__(16) uint64_t wbuf[kThreads / 64][2 * XS];
   const QC PassArgs& a = *(const QC PassArgs*)(pa);
   const QC Op* ops = (const QC Op*)(a.ops);
   const QC PhaseTerm* terms = (const QC PhaseTerm*)(a.terms);
@@ -704,25 +761,27 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
   const uint64_t wt = dy.tile0 + (uint64_t)blockIdx.x * (kThreads / 64) + wave;
   if (wt >= ntiles) return;
   const uint32_t L = a.L;
-  const uint64_t slot = wt >> (L - kWaveTileBits);
-  const uint64_t tau = wt & ((1ull << (L - kWaveTileBits)) - 1ull);
-  const TileMapConst tm{kWaveTileBits, 6u, (uint32_t)kRegBits, a.E};
+  const uint64_t slot = wt >> (L - TB);
+  const uint64_t tau = wt & ((1ull << (L - TB)) - 1ull);
+  const TileMapConst tm{TB, 6u, (uint32_t)RB, a.E};
   const uint64_t base = tm.base(tau);
-  const uint64_t e0 = 1ull << a.E[0], e1 = 1ull << a.E[1], e2 = 1ull << a.E[2], e3 = 1ull << a.E[3];
+  const uint64_t e0 = 1ull << a.E[0], e1 = 1ull << a.E[1], e2 = 1ull << a.E[2];
+  const uint64_t e3 = RB > 3 ? 1ull << a.E[3] : 0ull;
   auto roff = [&](int v) -> uint64_t {
     return ((v & 1) ? e0 : 0ull) | ((v & 2) ? e1 : 0ull) | ((v & 4) ? e2 : 0ull) | ((v & 8) ? e3 : 0ull);
   };
   const WaveCtx cx{lane, base};
-  double2 amp[kV];
+  double2 amp[KV];
   double2* xb = reinterpret_cast<double2*>(wbuf[wave]);
-  const uint32_t g = (uint32_t)lane & 1u;  // half of the chunk (chunk layout)
+  const uint32_t g = (uint32_t)lane & (LPC - 1u);  // part of the chunk (chunk layout)
+  const uint32_t d0 = 2u * KV * g;                 // first double of this lane's part
 
-  // chunk metadata (chunk layout): lanes 2c and 2c+1 both describe chunk c,
-  // whose first amplitude is tile position 32c
+  // chunk metadata (chunk layout): the LPC lanes of chunk c all describe it;
+  // its first amplitude is tile position 32c
   uint64_t mdesc = 0;
   uint32_t mbid = 0, mci = 0, mnull = 1;
   if constexpr (IN == IN_DEC || OUT == OUT_ENC) {
-    const uint64_t loc = base | tm.off(32u * ((uint32_t)lane >> 1));
+    const uint64_t loc = base | tm.off(32u * ((uint32_t)lane / LPC));
     mbid = bmap[(slot << a.khi) + (loc >> a.B)];
     mci = (uint32_t)((loc & ((1ull << a.B) - 1ull)) >> kChunkAmpsLog2);
     mnull = null_in[mbid];
@@ -733,9 +792,9 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
   if constexpr (IN == IN_RAW) {
     const QG double* src = (const QG double*)(dy.raw) + 2 * ((slot << L) + base + lane);
 #pragma unroll
-    for (int v = 0; v < kV; ++v) amp[v] = make_double2(src[2 * roff(v)], src[2 * roff(v) + 1]);
+    for (int v = 0; v < KV; ++v) amp[v] = make_double2(src[2 * roff(v)], src[2 * roff(v) + 1]);
   } else {
-    // this lane's codes are doubles 32g .. 32g+31 of its chunk
+    // this lane's codes are doubles d0 .. d0 + 2 KV - 1 of its chunk
     const uint32_t w = desc_w(mdesc);
     if (ballot64(w > 32u) == 0) {
       // every code of the tile has at most 32 bits: all payload loads are
@@ -743,18 +802,18 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
       const int e = desc_e(mdesc);
       const QG uint32_t* P32 = (const QG uint32_t*)(arena_in + desc_off(mdesc));
       const uint32_t m = (uint32_t)((1ull << w) - 1ull);
-      uint32_t dw[kV][3];
+      uint32_t dw[KV][3];
 #pragma unroll
-      for (int q = 0; q < kV; ++q) {
-        const uint32_t bit = (32u * g + 2u * q) * w;
+      for (int q = 0; q < KV; ++q) {
+        const uint32_t bit = (d0 + 2u * q) * w;
         const QG uint32_t* P = P32 + (bit >> 5);
         dw[q][0] = P[0];
         dw[q][1] = P[1];
         dw[q][2] = P[2];
       }
 #pragma unroll
-      for (int q = 0; q < kV; ++q) {
-        const uint32_t sh = ((32u * g + 2u * q) * w) & 31u;
+      for (int q = 0; q < KV; ++q) {
+        const uint32_t sh = ((d0 + 2u * q) * w) & 31u;
         const uint32_t lo = __builtin_amdgcn_alignbit(dw[q][1], dw[q][0], sh);
         const uint32_t hi = __builtin_amdgcn_alignbit(dw[q][2], dw[q][1], sh);
         const uint64_t x = ((uint64_t)hi << 32) | lo;
@@ -764,8 +823,8 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
       }
     } else {
 #pragma unroll
-      for (int q = 0; q < kV; ++q)
-        amp[q] = make_double2(decode_at(mdesc, arena_in, 32u * g + 2u * q), decode_at(mdesc, arena_in, 32u * g + 2u * q + 1u));
+      for (int q = 0; q < KV; ++q)
+        amp[q] = make_double2(decode_at(mdesc, arena_in, d0 + 2u * q), decode_at(mdesc, arena_in, d0 + 2u * q + 1u));
     }
     to_wave_layout(amp, xb, (uint32_t)lane);
   }
@@ -781,7 +840,7 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
   if constexpr (OUT == OUT_RAW) {
     QG double* dst = (QG double*)(dy.raw) + 2 * ((slot << L) + base + lane);
 #pragma unroll
-    for (int v = 0; v < kV; ++v) {
+    for (int v = 0; v < KV; ++v) {
       dst[2 * roff(v)] = amp[v].x;
       dst[2 * roff(v) + 1] = amp[v].y;
     }
@@ -791,7 +850,7 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
     if (ballot64(mnull != 0u) != 0) {
       bool nz = false;
 #pragma unroll
-      for (int q = 0; q < kV; ++q)
+      for (int q = 0; q < KV; ++q)
         nz = nz || (a.zero_mode == ZERO_EXACT ? (amp[q].x != 0.0 || amp[q].y != 0.0)
                                               : (fabs(amp[q].x) > a.zero_thr || fabs(amp[q].y) > a.zero_thr));
       if (mnull && nz) nonzero[mbid] = 1;
@@ -814,13 +873,13 @@ __global__ __launch_bounds__(kThreads, 2) void wave_kernel(const PassArgs* __res
     if (ballot64(w > 32u) == 0) {
       // two codes of at most 32 bits form one field of at most 64 bits
 #pragma unroll
-      for (int q = 0; q < kV; ++q)
-        stage_or(sc, (32u * g + 2u * q) * w,
+      for (int q = 0; q < KV; ++q)
+        stage_or(sc, (d0 + 2u * q) * w,
                  (uint64_t)__double_as_longlong(amp[q].x) | ((uint64_t)__double_as_longlong(amp[q].y) << w));
     } else {
 #pragma unroll
-      for (int q = 0; q < kV; ++q) {
-        const uint32_t di = 32u * g + 2u * q;
+      for (int q = 0; q < KV; ++q) {
+        const uint32_t di = d0 + 2u * q;
         stage_or(sc, di * w, (uint64_t)__double_as_longlong(amp[q].x));
         stage_or(sc, (di + 1u) * w, (uint64_t)__double_as_longlong(amp[q].y));
       }
@@ -1120,7 +1179,7 @@ hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* p
   return hipGetLastError();
 }
 
-template <int IN, int OUT>
+template <int IN, int OUT, int KV>
 static hipError_t launch_wave_mode(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
   constexpr uint64_t per_block = kThreads / 64;
   const uint64_t maxb = kMaxLaunchBlocks;
@@ -1128,19 +1187,28 @@ static hipError_t launch_wave_mode(const PassArgs* d_args, const PassDyn& dy, ui
   for (uint64_t t0 = 0; t0 < ntiles; t0 += maxb * per_block) {
     const uint64_t cnt = std::min<uint64_t>(ntiles - t0, maxb * per_block);
     d.tile0 = dy.tile0 + t0;
-    hipLaunchKernelGGL((wave_kernel<IN, OUT>), dim3((unsigned)((cnt + per_block - 1) / per_block)), dim3(kThreads), 0,
+    hipLaunchKernelGGL((wave_kernel<IN, OUT, KV>), dim3((unsigned)((cnt + per_block - 1) / per_block)), dim3(kThreads), 0,
                        st, d_args, d, dy.tile0 + ntiles);
   }
   return hipGetLastError();
 }
 
-// ntiles counts wave tiles (2^10 amplitudes each) of this launch
-hipError_t launch_wave(int in_mode, int out_mode, const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles,
-                       hipStream_t st) {
-  if (in_mode == IN_RAW && out_mode == OUT_RAW) return launch_wave_mode<IN_RAW, OUT_RAW>(d_args, dy, ntiles, st);
-  if (in_mode == IN_DEC && out_mode == OUT_RAW) return launch_wave_mode<IN_DEC, OUT_RAW>(d_args, dy, ntiles, st);
-  if (in_mode == IN_RAW && out_mode == OUT_ENC) return launch_wave_mode<IN_RAW, OUT_ENC>(d_args, dy, ntiles, st);
-  return launch_wave_mode<IN_DEC, OUT_ENC>(d_args, dy, ntiles, st);
+template <int KV>
+static hipError_t launch_wave_kv(int in_mode, int out_mode, const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles,
+                                 hipStream_t st) {
+  if (in_mode == IN_RAW && out_mode == OUT_RAW) return launch_wave_mode<IN_RAW, OUT_RAW, KV>(d_args, dy, ntiles, st);
+  if (in_mode == IN_DEC && out_mode == OUT_RAW) return launch_wave_mode<IN_DEC, OUT_RAW, KV>(d_args, dy, ntiles, st);
+  if (in_mode == IN_RAW && out_mode == OUT_ENC) return launch_wave_mode<IN_RAW, OUT_ENC, KV>(d_args, dy, ntiles, st);
+  return launch_wave_mode<IN_DEC, OUT_ENC, KV>(d_args, dy, ntiles, st);
+}
+
+// ntiles counts wave tiles (2^tile_bits amplitudes each, tile_bits 10 or 9)
+// of this launch
+hipError_t launch_wave(int tile_bits, int in_mode, int out_mode, const PassArgs* d_args, const PassDyn& dy,
+                       uint64_t ntiles, hipStream_t st) {
+  if (tile_bits == 10) return launch_wave_kv<16>(in_mode, out_mode, d_args, dy, ntiles, st);
+  if (tile_bits == 9) return launch_wave_kv<8>(in_mode, out_mode, d_args, dy, ntiles, st);
+  return hipErrorInvalidValue;
 }
 
 }  // namespace qsim
