@@ -186,8 +186,10 @@ def adjoint_bench(n=28, layers=8, reps=3):
                     f"H = sum Z_i Z_i+1 + 0.7 sum X_i ({len(obs.terms)} terms)",
         "energy": energy,
         "forward_ms": t_fwd * 1e3,
+        "forward_plus_energy_ms": t_eval * 1e3,
         "adjoint_ms": t_adj * 1e3,
         "adjoint_over_forward": t_adj / t_fwd,
+        "adjoint_over_forward_plus_energy": t_adj / t_eval,
         "grads_per_s": npar / t_adj,
         "central_fd_ms": t_fd * 1e3,
         "central_fd_basis": f"2 x {npar} evaluations x measured forward+energy time {t_eval * 1e3:.2f} ms",

@@ -57,10 +57,12 @@ hipError_t launch_block_sample(const double2* a, uint64_t namps, double* cdf_loc
                                hipStream_t st);
 hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
                                uint64_t ntiles, hipStream_t st);
-hipError_t launch_wave_pair_adjoint(const PassArgs* d_args, double2* psi, double2* lam, double* partials,
-                                    uint64_t ntiles, hipStream_t st);
+hipError_t launch_wave_pair_adjoint(int tile_bits, const PassArgs* d_args, double2* psi, double2* lam,
+                                    double* partials, uint64_t ntiles, hipStream_t st);
 hipError_t launch_pauli_sum(const double2* a, double2* out, uint64_t n, const PauliGroup* groups, uint32_t ng,
                             const PauliTermZ* terms, double* partials, unsigned grid, hipStream_t st);
+hipError_t launch_pauli_expect(const double2* a, uint64_t n, const PauliGroup* groups, uint32_t ng,
+                               const PauliTermZ* terms, double* partials, unsigned grid, hipStream_t st);
 hipError_t launch_block_bytes(const uint64_t* desc, const uint8_t* nullf, uint64_t nb, uint64_t cpb,
                               unsigned long long* out, hipStream_t st);
 
@@ -1523,29 +1525,6 @@ static void ipow(uint32_t ny, double& r, double& i) {
   }
 }
 
-int qsim_sv_expectation_pauli(const qsim_sv* sv, uint64_t n_terms, const uint64_t* xm, const uint64_t* zm,
-                              const uint32_t* ny, const double* cre, const double* cim, double* ore, double* oim) {
-  API_BEGIN
-  const unsigned grid = reduce_grid(sv->dim);
-  sv->part.ensure(2, grid);
-  DDAcc tr, ti;
-  for (uint64_t t = 0; t < n_terms; ++t) {
-    HIPCHECK(launch_sv_pauli(sv->ptr(), sv->dim, xm[t], zm[t], sv->part.d.as<double>(), grid, stream()));
-    sv->part.fetch(2, grid);
-    DDAcc acc[2];
-    sv->part.accumulate(2, grid, acc);
-    double pr, pi;
-    ipow(ny[t], pr, pi);
-    const double sr = acc[0].value(), si = acc[1].value();
-    // coef * i^ny * (sr + i si)
-    const double fr = cre[t] * pr - cim[t] * pi, fi = cre[t] * pi + cim[t] * pr;
-    tr.add(fr * sr - fi * si);
-    ti.add(fr * si + fi * sr);
-  }
-  *ore = tr.value();
-  *oim = ti.value();
-  API_END
-}
 
 namespace qsim {
 // Pauli-sum tables: terms grouped by flip mask (X/Y qubits); the coefficient
@@ -1575,6 +1554,38 @@ static uint32_t build_pauli_groups(uint64_t n_terms, const uint64_t* xm, const u
   }
   return (uint32_t)G.size();
 }
+
+}  // namespace qsim
+
+int qsim_sv_expectation_pauli(const qsim_sv* sv, uint64_t n_terms, const uint64_t* xm, const uint64_t* zm,
+                              const uint32_t* ny, const double* cre, const double* cim, double* ore, double* oim) {
+  API_BEGIN
+  require(sv != nullptr && ore != nullptr && oim != nullptr, "null argument");
+  if (n_terms == 0) {
+    *ore = *oim = 0.0;
+    return 0;
+  }
+  // every term in one sweep: terms sharing an X/Y flip mask share the load of
+  // the flipped amplitude
+  std::vector<PauliGroup> G;
+  std::vector<PauliTermZ> T;
+  const uint32_t ng = build_pauli_groups(n_terms, xm, zm, ny, cre, cim, 1.0, G, T);
+  qsim_sv* s = const_cast<qsim_sv*>(sv);
+  upload(s->adj_groups, G.data(), G.size() * sizeof(PauliGroup));
+  upload(s->adj_terms, T.data(), T.size() * sizeof(PauliTermZ));
+  const unsigned grid = reduce_grid(sv->dim);
+  s->part.ensure(2, grid);  // (re, im) compensated pairs per block
+  HIPCHECK(launch_pauli_expect(sv->ptr(), sv->dim, s->adj_groups.as<PauliGroup>(), ng, s->adj_terms.as<PauliTermZ>(),
+                               s->part.d.as<double>(), grid, stream()));
+  s->part.fetch(2, grid);
+  DDAcc acc[2];
+  s->part.accumulate(2, grid, acc);
+  *ore = acc[0].value();
+  *oim = acc[1].value();
+  API_END
+}
+
+namespace qsim {
 
 static HGate inverse_gate(const HGate& g) {
   HGate h = g;
@@ -1616,11 +1627,16 @@ static bool adjoint_wave_ok(const std::vector<LGate>& lg, uint32_t n) {
 }
 
 // Wave forms: by default a pair of waves per 1024-amplitude tile (psi and
-// lambda in one wave each, the forward engine's tile); QOMPSIM_ADJ_ENGINE=wave9
-// selects one wave holding 8 rows of each state (512-amplitude tiles).
+// lambda in one wave each, the forward engine's tile).  QOMPSIM_ADJ_ENGINE=pair9
+// pairs waves of 8 rows on 512-amplitude tiles (three waves per SIMD);
+// QOMPSIM_ADJ_ENGINE=wave9 selects one wave holding 8 rows of each state.
 static uint32_t adjoint_wave_tile() {
   const char* e = std::getenv("QOMPSIM_ADJ_ENGINE");
-  return (e && std::string(e) == "wave9") ? 9u : kWaveTile;
+  return (e && (std::string(e) == "wave9" || std::string(e) == "pair9")) ? 9u : kWaveTile;
+}
+static bool adjoint_pair9() {
+  const char* e = std::getenv("QOMPSIM_ADJ_ENGINE");
+  return e && std::string(e) == "pair9";
 }
 constexpr uint32_t kAdjWaveTile = kWaveTile;
 
@@ -1637,7 +1653,8 @@ static hipError_t launch_reverse_pass(const Program& P, size_t p, uint32_t n, do
                                       double* partials) {
   const uint64_t ntiles = reverse_tiles(P, p, n);
   if (P.passes[p].engine == ENGINE_WAVE) {
-    if (P.passes[p].t == kWaveTile) return launch_wave_pair_adjoint(P.dargs(p), psi, lam, partials, ntiles, stream());
+    if (P.passes[p].t == kWaveTile || adjoint_pair9())
+      return launch_wave_pair_adjoint((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, stream());
     return launch_wave_adjoint((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, stream());
   }
   return launch_adjoint_pass((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, stream());

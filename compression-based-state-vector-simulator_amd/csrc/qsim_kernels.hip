@@ -796,12 +796,17 @@ __global__ void accumulate_points_kernel(const double* __restrict__ pt, uint32_t
 // sharing a flip mask f share the load of a[x ^ f]; (P a)[x] = i^nY
 // (-1)^{pc((x^f) & zy)} a[x^f] with i^nY folded into the coefficient.  Also
 // accumulates Re sum conj(a[x]) out[x] into partials[blockIdx] (compensated).
+// STORE = false: <a|H|a> only (Observable::get_expectation_value,
+// src/cppsim/pauli_operator.cpp): nothing is stored, and the block writes
+// compensated (re, im) partials as four doubles.
+template <bool STORE>
 __global__ __launch_bounds__(kThreads) void pauli_sum_kernel(const double2* __restrict__ a, double2* __restrict__ out,
                                                              uint64_t n, const PauliGroup* __restrict__ groups,
                                                              uint32_t ng, const PauliTermZ* __restrict__ terms,
                                                              double* __restrict__ partials) {
   __shared__ double red_s[kThreads / 64], red_c[kThreads / 64];
-  double s = 0.0, c = 0.0;
+  __shared__ double red_s2[kThreads / 64], red_c2[kThreads / 64];
+  double s = 0.0, c = 0.0, s2 = 0.0, c2 = 0.0;
   for (uint64_t x = (uint64_t)blockIdx.x * kThreads + threadIdx.x; x < n; x += (uint64_t)gridDim.x * kThreads) {
     double re = 0.0, im = 0.0;
     for (uint32_t g = 0; g < ng; ++g) {
@@ -818,22 +823,46 @@ __global__ __launch_bounds__(kThreads) void pauli_sum_kernel(const double2* __re
       re += fr * v.x - fi * v.y;
       im += fr * v.y + fi * v.x;
     }
-    out[x] = make_double2(re, im);
     const double2 ax = a[x];
+    if constexpr (STORE) {
+      out[x] = make_double2(re, im);
+    } else {
+      dd_add(s2, c2, ax.x * im - ax.y * re);
+    }
     dd_add(s, c, ax.x * re + ax.y * im);
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int d = 32; d >= 1; d >>= 1) {
-    const double s2 = __shfl_xor(s, d, 64), c2 = __shfl_xor(c, d, 64);
-    dd_merge(s, c, s2, c2);
+    double t1 = __shfl_xor(s, d, 64), t2 = __shfl_xor(c, d, 64);
+    dd_merge(s, c, t1, t2);
+    if constexpr (!STORE) {
+      t1 = __shfl_xor(s2, d, 64);
+      t2 = __shfl_xor(c2, d, 64);
+      dd_merge(s2, c2, t1, t2);
+    }
   }
-  if (lane == 0) { red_s[wave] = s; red_c[wave] = c; }
+  if (lane == 0) {
+    red_s[wave] = s;
+    red_c[wave] = c;
+    red_s2[wave] = s2;
+    red_c2[wave] = c2;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double ss = red_s[0], cc = red_c[0];
-    for (int q = 1; q < kThreads / 64; ++q) dd_merge(ss, cc, red_s[q], red_c[q]);
-    partials[2 * blockIdx.x] = ss;
-    partials[2 * blockIdx.x + 1] = cc;
+    double ss = red_s[0], cc = red_c[0], ss2 = red_s2[0], cc2 = red_c2[0];
+    for (int q = 1; q < kThreads / 64; ++q) {
+      dd_merge(ss, cc, red_s[q], red_c[q]);
+      dd_merge(ss2, cc2, red_s2[q], red_c2[q]);
+    }
+    if constexpr (STORE) {
+      partials[2 * blockIdx.x] = ss;
+      partials[2 * blockIdx.x + 1] = cc;
+    } else {
+      partials[4 * blockIdx.x] = ss;
+      partials[4 * blockIdx.x + 1] = cc;
+      partials[4 * blockIdx.x + 2] = ss2;
+      partials[4 * blockIdx.x + 3] = cc2;
+    }
   }
 }
 
@@ -1048,7 +1077,15 @@ hipError_t launch_grad_reduce(const double* partials, uint64_t nblk, uint32_t np
 
 hipError_t launch_pauli_sum(const double2* a, double2* out, uint64_t n, const PauliGroup* groups, uint32_t ng,
                             const PauliTermZ* terms, double* partials, unsigned grid, hipStream_t st) {
-  hipLaunchKernelGGL(pauli_sum_kernel, dim3(grid), dim3(kThreads), 0, st, a, out, n, groups, ng, terms, partials);
+  hipLaunchKernelGGL(pauli_sum_kernel<true>, dim3(grid), dim3(kThreads), 0, st, a, out, n, groups, ng, terms, partials);
+  return hipGetLastError();
+}
+
+// <a|H|a>: partials receive four doubles per block (re and im, compensated)
+hipError_t launch_pauli_expect(const double2* a, uint64_t n, const PauliGroup* groups, uint32_t ng,
+                               const PauliTermZ* terms, double* partials, unsigned grid, hipStream_t st) {
+  hipLaunchKernelGGL(pauli_sum_kernel<false>, dim3(grid), dim3(kThreads), 0, st, a, (double2*)nullptr, n, groups, ng,
+                     terms, partials);
   return hipGetLastError();
 }
 

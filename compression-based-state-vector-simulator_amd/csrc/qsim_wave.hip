@@ -1065,12 +1065,14 @@ __global__ __launch_bounds__(kThreads, 2) void wave_adjoint_kernel(const PassArg
 // reads the Pauli partner of each element from there, whatever the flip mask.
 // Two LDS tiles alternate between consecutive points, so one barrier per
 // point keeps the psi wave from overwriting a tile the lambda wave still reads.
-__global__ __launch_bounds__(128, 2) void wave_pair_adjoint_kernel(const PassArgs* __restrict__ pa,
+template <int KV>
+__global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kernel(const PassArgs* __restrict__ pa,
                                                                    double2* __restrict__ psi,
                                                                    double2* __restrict__ lam,
                                                                    double* __restrict__ partials, uint64_t tile0,
                                                                    uint64_t ntiles) {
-  __shared__ __align__(16) double2 pbuf[2][1 << kWaveTileBits];
+  constexpr int RB = KV == 16 ? 4 : 3;
+  __shared__ __align__(16) double2 pbuf[2][64 * KV];
   const QC PassArgs& a = *(const QC PassArgs*)(pa);
   const QC Op* ops = (const QC Op*)(a.ops);
   const QC PhaseTerm* terms = (const QC PhaseTerm*)(a.terms);
@@ -1080,17 +1082,18 @@ __global__ __launch_bounds__(128, 2) void wave_pair_adjoint_kernel(const PassArg
   const bool is_lam = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) != 0;
   const uint64_t wt = tile0 + blockIdx.x;
   if (wt >= ntiles) return;  // uniform over the workgroup
-  const TileMapConst tm{(uint32_t)kWaveTileBits, 6u, (uint32_t)kRegBits, a.E};
+  const TileMapConst tm{(uint32_t)(6 + RB), 6u, (uint32_t)RB, a.E};
   const uint64_t base = tm.base(wt);
-  const uint64_t e0 = 1ull << a.E[0], e1 = 1ull << a.E[1], e2 = 1ull << a.E[2], e3 = 1ull << a.E[3];
+  const uint64_t e0 = 1ull << a.E[0], e1 = 1ull << a.E[1], e2 = 1ull << a.E[2];
+  const uint64_t e3 = RB > 3 ? 1ull << a.E[3] : 0ull;
   auto roff = [&](int v) -> uint64_t {
     return ((v & 1) ? e0 : 0ull) | ((v & 2) ? e1 : 0ull) | ((v & 4) ? e2 : 0ull) | ((v & 8) ? e3 : 0ull);
   };
   const WaveCtx cx{lane, base};
   QG double* sv = (QG double*)(is_lam ? lam : psi) + 2 * (base + lane);
-  double2 amp[kV];
+  double2 amp[KV];
 #pragma unroll
-  for (int v = 0; v < kV; ++v) amp[v] = make_double2(sv[2 * roff(v)], sv[2 * roff(v) + 1]);
+  for (int v = 0; v < KV; ++v) amp[v] = make_double2(sv[2 * roff(v)], sv[2 * roff(v) + 1]);
   uint32_t pb = 0;
   for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
     const QC Op& op = ops[o];
@@ -1103,7 +1106,7 @@ __global__ __launch_bounds__(128, 2) void wave_pair_adjoint_kernel(const PassArg
       pb ^= 1u;
       if (!is_lam) {
 #pragma unroll
-        for (int v = 0; v < kV; ++v) xb[64 * v + lane] = amp[v];
+        for (int v = 0; v < KV; ++v) xb[64 * v + lane] = amp[v];
       }
       __syncthreads();
       if (is_lam) {
@@ -1117,7 +1120,7 @@ __global__ __launch_bounds__(128, 2) void wave_pair_adjoint_kernel(const PassArg
         double acc0 = 0.0, acc1 = 0.0;
         if ((kr & 1u) == 0u) {
 #pragma unroll
-          for (int v = 0; v < kV; ++v) {
+          for (int v = 0; v < KV; ++v) {
             const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
             const double2 x = xb[q], l = amp[v];
             const uint32_t sg = lsgn ^ ((uint32_t)(__popc((((uint32_t)v << 6) ^ (fx & ~63u)) & zt) & 1) << 31);
@@ -1128,7 +1131,7 @@ __global__ __launch_bounds__(128, 2) void wave_pair_adjoint_kernel(const PassArg
           }
         } else {
 #pragma unroll
-          for (int v = 0; v < kV; ++v) {
+          for (int v = 0; v < KV; ++v) {
             const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
             const double2 x = xb[q], l = amp[v];
             const uint32_t sg = lsgn ^ ((uint32_t)(__popc((((uint32_t)v << 6) ^ (fx & ~63u)) & zt) & 1) << 31);
@@ -1149,19 +1152,25 @@ __global__ __launch_bounds__(128, 2) void wave_pair_adjoint_kernel(const PassArg
     }
   }
 #pragma unroll
-  for (int v = 0; v < kV; ++v) {
+  for (int v = 0; v < KV; ++v) {
     sv[2 * roff(v)] = amp[v].x;
     sv[2 * roff(v) + 1] = amp[v].y;
   }
 }
 
-// ntiles counts 1024-amplitude tiles of one state; one workgroup per tile
-hipError_t launch_wave_pair_adjoint(const PassArgs* d_args, double2* psi, double2* lam, double* partials,
-                                    uint64_t ntiles, hipStream_t st) {
+// ntiles counts tiles of 2^tile_bits amplitudes (10: 16 rows per wave, 9: 8
+// rows) of one state; one workgroup per tile
+hipError_t launch_wave_pair_adjoint(int tile_bits, const PassArgs* d_args, double2* psi, double2* lam,
+                                    double* partials, uint64_t ntiles, hipStream_t st) {
+  if (tile_bits != 10 && tile_bits != 9) return hipErrorInvalidValue;
   for (uint64_t t0 = 0; t0 < ntiles; t0 += kMaxLaunchBlocks) {
     const uint64_t cnt = std::min<uint64_t>(ntiles - t0, kMaxLaunchBlocks);
-    hipLaunchKernelGGL(wave_pair_adjoint_kernel, dim3((unsigned)cnt), dim3(128), 0, st, d_args, psi, lam, partials,
-                       t0, ntiles);
+    if (tile_bits == 10)
+      hipLaunchKernelGGL(wave_pair_adjoint_kernel<16>, dim3((unsigned)cnt), dim3(128), 0, st, d_args, psi, lam,
+                         partials, t0, ntiles);
+    else
+      hipLaunchKernelGGL(wave_pair_adjoint_kernel<8>, dim3((unsigned)cnt), dim3(128), 0, st, d_args, psi, lam,
+                         partials, t0, ntiles);
   }
   return hipGetLastError();
 }

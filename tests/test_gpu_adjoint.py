@@ -151,7 +151,7 @@ def test_adjoint_engines_agree(n, rng, monkeypatch):
     obs = _mixed_observable(n, rng)
     e0, g0 = O.backprop(c, obs)
     got = {}
-    for eng in ("", "wave9", "lds"):
+    for eng in ("", "pair9", "wave9", "lds"):
         monkeypatch.setenv("QOMPSIM_ADJ_ENGINE", eng)
         got[eng] = adjoint_gradient(c, obs)
     for eng, (e, g) in got.items():
