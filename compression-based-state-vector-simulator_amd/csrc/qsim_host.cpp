@@ -2296,6 +2296,114 @@ int qsim_cs_block_bytes(qsim_cs* cs, uint64_t* out) {
   API_END
 }
 
+// ---- compressed-store checkpoint files (no reference counterpart)
+namespace qsim {
+struct CkptHeader {
+  char magic[8];  // "QSIMCKP1"
+  uint32_t version, n, B, nsub;
+  double eps_rel, eps_abs, zero_thr;
+  int32_t guard, zero_mode;
+  uint64_t nb, cpb, cap_words, next_stage, cur_bytes;
+};
+static_assert(sizeof(CkptHeader) == 96, "checkpoint header layout");
+
+// device <-> file through one host staging buffer of at most 256 MiB
+static void dev_to_file(std::FILE* f, const void* dev, uint64_t bytes) {
+  const uint64_t chunk = 256ull << 20;
+  std::vector<char> h((size_t)std::min<uint64_t>(std::max<uint64_t>(bytes, 1), chunk));
+  for (uint64_t o = 0; o < bytes; o += chunk) {
+    const uint64_t c = std::min(chunk, bytes - o);
+    HIPCHECK(hipMemcpy(h.data(), (const char*)dev + o, c, hipMemcpyDeviceToHost));
+    if (std::fwrite(h.data(), 1, c, f) != c) throw std::runtime_error("checkpoint: write failed");
+  }
+}
+static void file_to_dev(std::FILE* f, void* dev, uint64_t bytes) {
+  const uint64_t chunk = 256ull << 20;
+  std::vector<char> h((size_t)std::min<uint64_t>(std::max<uint64_t>(bytes, 1), chunk));
+  for (uint64_t o = 0; o < bytes; o += chunk) {
+    const uint64_t c = std::min(chunk, bytes - o);
+    if (std::fread(h.data(), 1, c, f) != c) throw std::runtime_error("checkpoint: file is truncated");
+    HIPCHECK(hipMemcpy((char*)dev + o, h.data(), c, hipMemcpyHostToDevice));
+  }
+}
+struct FileCloser {
+  std::FILE* f;
+  ~FileCloser() {
+    if (f) std::fclose(f);
+  }
+};
+}  // namespace qsim
+
+int qsim_cs_save_checkpoint(qsim_cs* cs, const char* path, uint64_t next_stage) {
+  API_BEGIN
+  require(cs != nullptr && path != nullptr, "null argument");
+  HIPCHECK(hipStreamSynchronize(stream()));
+  const int c = cs->cur;
+  CkptHeader h{};
+  std::memcpy(h.magic, "QSIMCKP1", 8);
+  h.version = 1;
+  h.n = cs->n;
+  h.B = cs->B;
+  h.nsub = cs->nsub[c];
+  h.eps_rel = cs->cfg.eps_rel;
+  h.eps_abs = cs->cfg.eps_abs;
+  h.zero_thr = cs->cfg.zero_threshold;
+  h.guard = cs->cfg.guard_bits;
+  h.zero_mode = cs->cfg.zero_mode;
+  h.nb = cs->nb;
+  h.cpb = cs->cpb;
+  h.cap_words = cs->cap_words[c];
+  h.next_stage = next_stage;
+  h.cur_bytes = cs->cur_bytes;
+  std::vector<uint64_t> sb = cs->h_subbase[c];
+  sb.resize((size_t)h.nsub + 1, 0);
+  FileCloser fc{std::fopen(path, "wb")};
+  if (!fc.f) throw std::runtime_error(std::string("checkpoint: cannot open ") + path);
+  if (std::fwrite(&h, sizeof(h), 1, fc.f) != 1) throw std::runtime_error("checkpoint: write failed");
+  dev_to_file(fc.f, cs->null_flags.p, cs->nb);
+  dev_to_file(fc.f, cs->desc[c].p, cs->nb * cs->cpb * 8);
+  if (std::fwrite(sb.data(), 8, sb.size(), fc.f) != sb.size()) throw std::runtime_error("checkpoint: write failed");
+  dev_to_file(fc.f, cs->cursors[c].p, (uint64_t)h.nsub * 16 * 8);
+  dev_to_file(fc.f, cs->arena[c].p, (h.cap_words + 8) * 8);
+  if (std::fflush(fc.f) != 0) throw std::runtime_error("checkpoint: write failed");
+  API_END
+}
+
+int qsim_cs_load_checkpoint(qsim_cs* cs, const char* path, uint64_t* next_stage) {
+  API_BEGIN
+  require(cs != nullptr && path != nullptr, "null argument");
+  FileCloser fc{std::fopen(path, "rb")};
+  if (!fc.f) throw std::runtime_error(std::string("checkpoint: cannot open ") + path);
+  CkptHeader h{};
+  if (std::fread(&h, sizeof(h), 1, fc.f) != 1) throw std::runtime_error("checkpoint: file is truncated");
+  if (std::memcmp(h.magic, "QSIMCKP1", 8) != 0 || h.version != 1)
+    throw std::invalid_argument("checkpoint: not a compressed-store checkpoint (magic/version)");
+  if (h.n != cs->n || h.B != cs->B || h.nb != cs->nb || h.cpb != cs->cpb)
+    throw std::invalid_argument("checkpoint: qubit count or block size differs from this store");
+  if (h.eps_rel != cs->cfg.eps_rel || h.eps_abs != cs->cfg.eps_abs || h.guard != cs->cfg.guard_bits ||
+      h.zero_mode != cs->cfg.zero_mode || h.zero_thr != cs->cfg.zero_threshold)
+    throw std::invalid_argument("checkpoint: codec settings differ from this store");
+  if (h.nsub == 0 || h.nsub > (uint32_t)kMaxSubArenas || (h.nsub & (h.nsub - 1)))
+    throw std::invalid_argument("checkpoint: bad sub-arena count");
+  HIPCHECK(hipStreamSynchronize(stream()));
+  const int c = cs->cur;
+  file_to_dev(fc.f, cs->null_flags.p, cs->nb);
+  file_to_dev(fc.f, cs->desc[c].p, cs->nb * cs->cpb * 8);
+  std::vector<uint64_t> sb((size_t)h.nsub + 1);
+  if (std::fread(sb.data(), 8, sb.size(), fc.f) != sb.size()) throw std::runtime_error("checkpoint: file is truncated");
+  if (sb.back() > h.cap_words) throw std::invalid_argument("checkpoint: sub-arena table exceeds the arena");
+  cs->nsub[c] = h.nsub;
+  cs->h_subbase[c] = sb;
+  cs_upload_subbase(cs, c);
+  file_to_dev(fc.f, cs->cursors[c].p, (uint64_t)h.nsub * 16 * 8);
+  if (cs->cap_words[c] < h.cap_words) cs_alloc_arena(cs, c, h.cap_words);
+  file_to_dev(fc.f, cs->arena[c].p, (h.cap_words + 8) * 8);
+  HIPCHECK(hipMemcpy(cs->h_null.data(), cs->null_flags.p, cs->nb, hipMemcpyDeviceToHost));
+  cs->cur_bytes = h.cur_bytes;
+  if (next_stage) *next_stage = h.next_stage;
+  API_END
+}
+
 int qsim_cs_bytes_history(const qsim_cs* cs, uint64_t* out, uint64_t cap, uint64_t* n) {
   API_BEGIN
   *n = cs->bytes_hist.size();

@@ -258,6 +258,18 @@ class CompressedQuantumState:
         call("qsim_cs_sampling_blockwise", self._h, n, int(random_seed) & 0xFFFFFFFF, ptr(out))
         return [int(x) for x in out[:n]]
 
+    def save_checkpoint(self, path, next_stage=0):
+        """Write the compressed store (descriptors, null flags, arena) to
+        `path`; `next_stage` is kept for resuming a staged run."""
+        call("qsim_cs_save_checkpoint", self._h, os.fsencode(path), int(next_stage))
+
+    def load_checkpoint(self, path):
+        """Replace the store by a checkpoint written by save_checkpoint from a
+        store of the same shape and codec settings; returns its next_stage."""
+        ns = ctypes.c_uint64()
+        call("qsim_cs_load_checkpoint", self._h, os.fsencode(path), ctypes.byref(ns))
+        return int(ns.value)
+
     def get_null_bitmap(self):
         out = np.empty(self.nblocks, dtype=np.uint8)
         call("qsim_cs_null_bitmap", self._h, ptr(out))

@@ -163,6 +163,15 @@ int qsim_cs_read_vector(qsim_cs* cs, double* host_amplitudes, int normalise);
 int qsim_cs_read_block(qsim_cs* cs, uint64_t block, double* host_amplitudes);
 int qsim_cs_set_basis(qsim_cs* cs, uint64_t index);
 int qsim_cs_load(qsim_cs* cs, const double* host_amplitudes);
+/* Compressed-store checkpoint files (no reference counterpart; the
+ * checkpoint format of a long run, so that it can resume): the null flags,
+ * the chunk descriptors, the arena's sub-arena table and cursors and the
+ * payload arena of the current store, behind a header with the store's shape
+ * and codec settings.  next_stage is stored for the caller (the first stage
+ * still to run).  Loading requires a store of the same qubit count, block
+ * size and codec settings (status -2 otherwise). */
+int qsim_cs_save_checkpoint(qsim_cs* cs, const char* path, uint64_t next_stage);
+int qsim_cs_load_checkpoint(qsim_cs* cs, const char* path, uint64_t* next_stage);
 int qsim_cs_null_bitmap(const qsim_cs* cs, uint8_t* out);
 /* sampling_blockwise (compressed_state.hpp:1600-1681): n_shots basis indices
  * into out, drawn with seed as the reference does in one process (block

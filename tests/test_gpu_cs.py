@@ -298,3 +298,64 @@ def test_sampling_blockwise_zero_mass():
     cs.load(np.zeros(1 << 10, dtype=complex))
     with pytest.raises(ValueError):
         cs.sampling_blockwise(4, 1)
+
+
+def test_checkpoint_file_resume_is_exact(tmp_path):
+    """Run half the stages, save, restore into a fresh store, run the rest:
+    bit-identical to an uninterrupted run (vector, bitmap, bytes)."""
+    n, B = 18, 10
+    c = W.qft_on_product(n, seed=5)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    kw = dict(eps_rel=1e-3, eps_abs=1e-7, guard_bits=1, verbose=False)
+    full = CompressedQuantumState(n, B, **kw)
+    full.partition_and_register_circuit(c, 4)
+    full.apply_circuit()
+    ns = full.get_num_stages()
+    half = ns // 2
+    a = CompressedQuantumState(n, B, **kw)
+    a.partition_and_register_circuit(c, 4)
+    a.run_stages(0, half)
+    path = str(tmp_path / "store.qckp")
+    a.save_checkpoint(path, next_stage=half)
+    b = CompressedQuantumState(n, B, **kw)
+    b.partition_and_register_circuit(c, 4)
+    assert b.load_checkpoint(path) == half
+    assert np.array_equal(b.get_raw_vector(), a.get_raw_vector())
+    b.run_stages(half, ns)
+    assert np.array_equal(b.get_raw_vector(), full.get_raw_vector())
+    assert np.array_equal(b.get_null_bitmap(), full.get_null_bitmap())
+    assert sum(b.get_block_bytes()) == sum(full.get_block_bytes())
+
+
+def test_checkpoint_file_rejects_other_store(tmp_path):
+    n = 12
+    a = CompressedQuantumState(n, 6, verbose=False)
+    path = str(tmp_path / "s.qckp")
+    a.save_checkpoint(path)
+    with pytest.raises(ValueError, match="block size"):
+        CompressedQuantumState(n, 7, verbose=False).load_checkpoint(path)
+    with pytest.raises(ValueError, match="codec"):
+        CompressedQuantumState(n, 6, eps_rel=1e-4, verbose=False).load_checkpoint(path)
+    bad = tmp_path / "bad.qckp"
+    bad.write_bytes(b"not a checkpoint" * 8)
+    with pytest.raises(ValueError, match="magic"):
+        a.load_checkpoint(str(bad))
+
+
+def test_compressed_tile_shapes_agree(monkeypatch):
+    """Single-pass stages on 512-amplitude (default) and 1024-amplitude tiles:
+    same stage results up to the rounding of the phase factors, same bitmap."""
+    n, B = 16, 10
+    c = W.qft_on_product(n, seed=4)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    out = {}
+    for bits in ("9", "10"):
+        monkeypatch.setenv("QOMPSIM_WAVE_TILE_BITS", bits)
+        cs = CompressedQuantumState(n, B, eps_rel=1e-12, eps_abs=1e-12, guard_bits=2, verbose=False)
+        cs.partition_and_register_circuit(c, 4)
+        cs.apply_circuit()
+        out[bits] = (cs.get_raw_vector(), cs.get_null_bitmap())
+    assert np.max(np.abs(out["9"][0] - out["10"][0])) < 1e-11
+    assert np.array_equal(out["9"][1], out["10"][1])
+    exact = O.apply_circuit(O.zero_state(n), c.gate_list)
+    assert abs(np.vdot(exact, out["9"][0])) > 1 - 1e-10
