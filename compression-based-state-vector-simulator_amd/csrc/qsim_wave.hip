@@ -219,11 +219,11 @@ __device__ __forceinline__ void stage_or(uint64_t* stg, uint32_t bit, uint64_t c
   atomicOr(reinterpret_cast<unsigned long long*>(&stg[qw + 1]), (unsigned long long)((c >> 1) >> (63u - sh)));
 }
 
-// Quantise the half chunk held by this lane (chunk layout; the partner lane
-// lane^1 holds the other half) and replace amp by the codes (zig-zag integers
-// of rint(x * 2^-e), or the raw bit patterns when the chunk is stored
-// verbatim).  Returns the chunk's w | (e & 0xffff) << 8, the same in both
-// lanes of the pair.
+// Quantise the part of a chunk held by this lane (chunk layout: a chunk
+// spans 32/KV lanes, two for KV = 16 and four for KV = 8) and replace amp by
+// the codes (zig-zag integers of rint(x * 2^-e), or the raw bit patterns when
+// the chunk is stored verbatim).  Returns the chunk's w | (e & 0xffff) << 8,
+// the same in every lane of the chunk.
 //
 // e is the minimum over the chunk's nonzero doubles of
 // ilogb(2*fma(eps_rel,|x|,eps_abs)) minus the guard bits.  That function is
@@ -946,19 +946,16 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
   return x;
 }
 
-// Wave-tile form of the adjoint pass (ParametricQuantumCircuit::
-// backprop_inner_product, src/vqcsim/parametric_circuit.cpp:180-279): the
-// wave holds the same 1024-amplitude tile of psi and lambda in registers and
-// applies every dense / elementwise op of the pass (the inverted gates) to
-// both.  An OP_GRAD op stages psi in the wave's LDS buffer, reads the Pauli
-// partner of each element, and adds Re sum conj(lambda) (i P psi) over the
-// tile, compensated, to partials[(gidx * ntiles + tile) * 2 + {0, 1}].
-// The tile is 512 amplitudes of one state: lanes on qubits 0..5 and three
-// register qubits E[0..2].  Register rows 0..7 hold psi and rows 8..15 the
-// same positions of lambda, so the fourth row bit selects the state and no
-// gate ever acts on it: every dense / elementwise op runs once over all 16
-// rows with the forward kernel's code and register footprint, sharing
-// coefficient loads and phase factors between psi and lambda.
+// Single-wave form of the adjoint pass (ParametricQuantumCircuit::
+// backprop_inner_product, src/vqcsim/parametric_circuit.cpp:180-279) on
+// 512-amplitude tiles: lanes on qubits 0..5 and three register qubits
+// E[0..2].  Register rows 0..7 hold psi and rows 8..15 the same positions of
+// lambda, so the fourth row bit selects the state and no gate ever acts on it:
+// every dense / elementwise op (the inverted gates) runs once over all 16 rows
+// with the forward kernel's code and register footprint.  An OP_GRAD op forms
+// Re sum conj(lambda) (i P psi) over the tile from the Pauli partners (row
+// swaps, lane shuffles, or the wave's LDS buffer for mixed flips) and writes it
+// to partials[(gidx * ntiles + tile) * 2].
 __global__ __launch_bounds__(kThreads, 2) void wave_adjoint_kernel(const PassArgs* __restrict__ pa,
                                                                    double2* __restrict__ psi,
                                                                    double2* __restrict__ lam,
