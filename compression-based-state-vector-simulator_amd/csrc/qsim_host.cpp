@@ -1989,6 +1989,81 @@ int qsim_cs_destroy(qsim_cs* cs) {
   API_END
 }
 
+namespace qsim {
+// Commutation-aware stage packing (B-aware reordering; no reference
+// counterpart, the role the optimizer's swap insertion plays for ranks,
+// circuit_optimizer.cpp:834-943): a stage takes, in circuit order, every
+// remaining gate whose qubits at or above B fit the stage's high set (at most
+// max(I, 2) qubits, the partition_circuit threshold) and that commutes with
+// every earlier gate left for a later stage; the rest wait.  Gates keep their
+// circuit order inside a stage, and a gate only overtakes gates it commutes
+// with, so the product of the stages equals the circuit.
+static std::vector<std::vector<uint32_t>> partition_commuting(const std::vector<HGate>& gs, uint32_t B,
+                                                              uint32_t inner_size) {
+  const uint32_t thr = std::max<uint32_t>(inner_size, 2u);
+  std::vector<std::vector<uint32_t>> hi(gs.size());
+  for (size_t i = 0; i < gs.size(); ++i) {
+    for (auto q : gs[i].t)
+      if (q >= B && std::find(hi[i].begin(), hi[i].end(), q) == hi[i].end()) hi[i].push_back(q);
+    for (auto q : gs[i].c)
+      if (q >= B && std::find(hi[i].begin(), hi[i].end(), q) == hi[i].end()) hi[i].push_back(q);
+    if (hi[i].size() > thr) throw std::invalid_argument("gate touches more high qubits than the stage threshold");
+  }
+  std::vector<std::vector<uint32_t>> stages;
+  std::vector<uint32_t> rem(gs.size());
+  std::iota(rem.begin(), rem.end(), 0u);
+  while (!rem.empty()) {
+    std::vector<uint32_t> H, cur, deferred;
+    for (uint32_t i : rem) {
+      bool ok = true;
+      for (uint32_t d : deferred)
+        if (!gates_commute(gs[i], gs[d])) { ok = false; break; }
+      if (ok) {
+        std::vector<uint32_t> H2 = H;
+        for (auto q : hi[i])
+          if (std::find(H2.begin(), H2.end(), q) == H2.end()) H2.push_back(q);
+        if (H2.size() <= thr) {
+          H = H2;
+          cur.push_back(i);
+          continue;
+        }
+      }
+      deferred.push_back(i);
+    }
+    if (cur.empty()) throw std::logic_error("stage packing made no progress");
+    stages.push_back(cur);
+    rem.swap(deferred);
+  }
+  return stages;
+}
+}  // namespace qsim
+
+int qsim_partition_commuting(uint32_t n_qubits, uint32_t block_size, uint32_t inner_size, const qsim_gate* gates,
+                             uint64_t n_gates, const double* pool, uint64_t pool_len, uint32_t* stage_of_gate,
+                             uint32_t* n_stages) {
+  API_BEGIN
+  require(stage_of_gate != nullptr || n_gates == 0, "null argument");
+  require(block_size <= n_qubits, "block_size above qubit_count");
+  auto gs = parse_gates(gates, n_gates, pool, pool_len);
+  check_qubits(gs, n_qubits);
+  auto st = partition_commuting(gs, block_size, inner_size);
+  for (uint32_t s = 0; s < st.size(); ++s)
+    for (auto g : st[s]) stage_of_gate[g] = s;
+  if (n_stages) *n_stages = (uint32_t)st.size();
+  API_END
+}
+
+int qsim_cs_partition_commuting_and_register(qsim_cs* cs, const qsim_gate* gates, uint64_t n_gates, const double* pool,
+                                             uint64_t pool_len, uint32_t inner_size) {
+  API_BEGIN
+  auto gs = parse_gates(gates, n_gates, pool, pool_len);
+  check_qubits(gs, cs->n);
+  auto st = partition_commuting(gs, cs->B, inner_size);
+  cs->gates = gs;
+  cs_register(cs, std::move(gs), st, nullptr, nullptr);
+  API_END
+}
+
 int qsim_cs_partition_and_register(qsim_cs* cs, const qsim_gate* gates, uint64_t n_gates, const double* pool,
                                    uint64_t pool_len, uint32_t inner_size) {
   API_BEGIN

@@ -120,6 +120,8 @@ _SIGS = {
     "qsim_cs_load": (ctypes.c_int, [_vp, _vp]),
     "qsim_cs_null_bitmap": (ctypes.c_int, [_vp, _vp]),
     "qsim_cs_sampling_blockwise": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp]),
+    "qsim_partition_commuting": (ctypes.c_int, [_u32, _u32, _u32, _vp, _u64, _vp, _u64, _vp, _P(_u32)]),
+    "qsim_cs_partition_commuting_and_register": (ctypes.c_int, [_vp, _vp, _u64, _vp, _u64, _u32]),
     "qsim_cs_save_checkpoint": (ctypes.c_int, [_vp, ctypes.c_char_p, _u64]),
     "qsim_cs_load_checkpoint": (ctypes.c_int, [_vp, ctypes.c_char_p, _P(_u64)]),
     "qsim_cs_block_bytes": (ctypes.c_int, [_vp, _vp]),

@@ -82,6 +82,21 @@ class QuantumState:
         return f"QuantumState(qubit_count={self._n})"
 
 
+def partition_commuting(circuit, block_size, inner_size):
+    """Stages (lists of gate indices, each in circuit order) of the
+    commutation-aware packing; host-side planning only."""
+    gates = list(circuit.gate_list)
+    rec, pool = _lib.pack_gates(gates)
+    out = np.zeros(max(1, len(gates)), dtype=np.uint32)
+    ns = ctypes.c_uint32()
+    call("qsim_partition_commuting", int(circuit.get_qubit_count()), int(block_size), int(inner_size), ptr(rec),
+         len(gates), ptr(pool), pool.size, ptr(out), ctypes.byref(ns))
+    stages = [[] for _ in range(ns.value)]
+    for i in range(len(gates)):
+        stages[int(out[i])].append(i)
+    return stages
+
+
 def inner_product(state_bra, state_ket):
     """state::inner_product (src/cppsim/state.cpp:9-31): <bra|ket>."""
     re, im = ctypes.c_double(), ctypes.c_double()
@@ -160,10 +175,15 @@ class CompressedQuantumState:
     def nblocks(self):
         return 1 << (self._n - self._B)
 
-    def partition_and_register_circuit(self, circuit, inner_size, verbose=0):
+    def partition_and_register_circuit(self, circuit, inner_size, verbose=0, reorder=False):
+        """partition_circuit + make_block_groups + registration
+        (compressed_state.hpp:673-726).  reorder=True packs stages with the
+        commutation-aware rule instead (qsim_partition_commuting): fewer
+        stages, the same circuit."""
         gates = list(circuit.gate_list)
         rec, pool = _lib.pack_gates(gates)
-        call("qsim_cs_partition_and_register", self._h, ptr(rec), len(gates), ptr(pool), pool.size, int(inner_size))
+        fn = "qsim_cs_partition_commuting_and_register" if reorder else "qsim_cs_partition_and_register"
+        call(fn, self._h, ptr(rec), len(gates), ptr(pool), pool.size, int(inner_size))
         self._gates = gates
         if verbose > 0:
             sizes = [len(s) for s in self.get_stages()]

@@ -145,6 +145,18 @@ int qsim_cs_destroy(qsim_cs* cs);
 /* partition_and_register_circuit (compressed_state.hpp:673-726) */
 int qsim_cs_partition_and_register(qsim_cs* cs, const qsim_gate* gates, uint64_t n_gates, const double* pool,
                                    uint64_t pool_len, uint32_t inner_size);
+/* Commutation-aware stage packing (B-aware reordering; no reference
+ * counterpart): stages take, in circuit order, every remaining gate whose
+ * qubits at or above block_size fit the stage's high set (at most
+ * max(inner_size, 2)) and that commutes with every earlier gate left for a
+ * later stage.  qsim_partition_commuting only plans (host code, no GPU):
+ * stage_of_gate[i] receives gate i's stage.  The _and_register form plans and
+ * registers like qsim_cs_partition_and_register. */
+int qsim_partition_commuting(uint32_t n_qubits, uint32_t block_size, uint32_t inner_size, const qsim_gate* gates,
+                             uint64_t n_gates, const double* pool, uint64_t pool_len, uint32_t* stage_of_gate,
+                             uint32_t* n_stages);
+int qsim_cs_partition_commuting_and_register(qsim_cs* cs, const qsim_gate* gates, uint64_t n_gates,
+                                             const double* pool, uint64_t pool_len, uint32_t inner_size);
 /* register_circuit_stages (compressed_state.hpp:464-507): stage s holds gate
  * ids stage_gate_ids[stage_offsets[s] .. stage_offsets[s+1]); block_to_group
  * holds n_stages x 2^(n-B) group ids; num_groups[s] groups per stage. */

@@ -359,3 +359,24 @@ def test_compressed_tile_shapes_agree(monkeypatch):
     assert np.array_equal(out["9"][1], out["10"][1])
     exact = O.apply_circuit(O.zero_state(n), c.gate_list)
     assert abs(np.vdot(exact, out["9"][0])) > 1 - 1e-10
+
+
+def test_commuting_partition_runs_compressed():
+    """Stages from the commutation-aware packing: fewer stages, and the
+    compressed run keeps the fidelity bar against the exact state."""
+    n, B = 16, 8
+    c = W.qft_on_product(n, seed=6)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    ref = CompressedQuantumState(n, B, verbose=False)
+    ref.partition_and_register_circuit(c, 4)
+    cs = CompressedQuantumState(n, B, verbose=False)
+    cs.partition_and_register_circuit(c, 4, reorder=True)
+    assert cs.get_num_stages() < ref.get_num_stages()
+    cs.apply_circuit()
+    exact = O.apply_circuit(O.zero_state(n), c.gate_list)
+    assert abs(np.vdot(exact, cs.get_vector())) > 0.99999
+    # near-lossless codec: the reordered stages reproduce the exact state
+    cs2 = CompressedQuantumState(n, B, eps_rel=1e-12, eps_abs=1e-12, guard_bits=2, verbose=False)
+    cs2.partition_and_register_circuit(c, 4, reorder=True)
+    cs2.apply_circuit()
+    assert np.max(np.abs(cs2.get_vector() - exact)) < 1e-9

@@ -89,3 +89,39 @@ def test_python_merge_matches_optimizer_unitary(rng):
 def test_qft_gate_count():
     c = W.qft_on_product(32, seed=2)
     assert c.get_gate_count() == 608
+
+
+def test_commuting_partition_plan_is_valid_and_shorter():
+    """Host-side stage packing: every stage's high set fits max(I, 2), each
+    stage lists gates in circuit order, it needs no more stages than
+    partition_circuit, and applying the stages in turn equals the circuit
+    (oracle, random state)."""
+    import numpy as np
+
+    import pyoracle as O
+    from qompsim import QuantumCircuitOptimizer
+    from qompsim import workloads as W
+    from qompsim.state import partition_commuting
+
+    c = W.qft_on_product(32, seed=2)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    st = partition_commuting(c, 20, 4)
+    assert sorted(i for s in st for i in s) == list(range(len(c.gate_list)))
+    for s in st:
+        assert s == sorted(s)
+        hi = set()
+        for i in s:
+            g = c.gate_list[i]
+            hi |= {q for q in list(g.get_target_index_list()) + list(g.get_control_index_list()) if q >= 20}
+        assert len(hi) <= 4
+    assert len(st) <= 24  # partition_circuit gives 29 on this circuit
+    n = 11
+    c2 = W.qft_on_product(n, seed=3)
+    QuantumCircuitOptimizer().optimize(c2, 2)
+    st2 = partition_commuting(c2, 6, 2)
+    rng = np.random.default_rng(2)
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    v /= np.linalg.norm(v)
+    ref = O.apply_circuit(v, c2.gate_list)
+    got = O.apply_circuit(v, [c2.gate_list[i] for s in st2 for i in s])
+    assert np.max(np.abs(got - ref)) < 1e-12
