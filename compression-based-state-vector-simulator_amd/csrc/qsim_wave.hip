@@ -758,8 +758,11 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   QG uint64_t* arena_out = (QG uint64_t*)(dy.arena_out);
   QG uint8_t* nonzero = (QG uint8_t*)(dy.nonzero);
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const uint64_t wt = dy.tile0 + (uint64_t)blockIdx.x * (kThreads / 64) + wave;
-  if (wt >= ntiles) return;
+  // The fused DEC->ENC form runs a persistent grid (wave w of group b takes
+  // tiles tile0 + 4 (b + k gridDim) + w); the other forms take one tile per
+  // wave.
+  constexpr bool kPersistent = IN == IN_DEC && OUT == OUT_ENC;
+  auto tile_body = [&](const uint64_t wt) {
   const uint32_t L = a.L;
   const uint64_t slot = wt >> (L - TB);
   const uint64_t tau = wt & ((1ull << (L - TB)) - 1ull);
@@ -898,6 +901,14 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
     } else if (g == 0) {
       desc_out[(uint64_t)mbid * a.cpb + mci] = desc_pack(0, 0, 0);
     }
+  }
+  };
+  const uint64_t wt0 = dy.tile0 + (uint64_t)blockIdx.x * (kThreads / 64) + wave;
+  if constexpr (kPersistent) {
+    const uint64_t stride = (uint64_t)gridDim.x * (kThreads / 64);
+    for (uint64_t wt = wt0; wt < ntiles; wt += stride) tile_body(wt);
+  } else {
+    if (wt0 < ntiles) tile_body(wt0);
   }
 }
 
@@ -1185,16 +1196,27 @@ hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* p
   return hipGetLastError();
 }
 
+// DEC->ENC: one launch, a persistent grid of at most kWaveGroups work-groups
+// (each wave walks its tiles with a stride of the grid).  Other forms: one
+// tile per wave, launches split at kMaxLaunchBlocks work-groups.
 template <int IN, int OUT, int KV>
 static hipError_t launch_wave_mode(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
   constexpr uint64_t per_block = kThreads / 64;
-  const uint64_t maxb = kMaxLaunchBlocks;
-  PassDyn d = dy;
-  for (uint64_t t0 = 0; t0 < ntiles; t0 += maxb * per_block) {
-    const uint64_t cnt = std::min<uint64_t>(ntiles - t0, maxb * per_block);
-    d.tile0 = dy.tile0 + t0;
-    hipLaunchKernelGGL((wave_kernel<IN, OUT, KV>), dim3((unsigned)((cnt + per_block - 1) / per_block)), dim3(kThreads), 0,
-                       st, d_args, d, dy.tile0 + ntiles);
+  if constexpr (IN == IN_DEC && OUT == OUT_ENC) {
+    constexpr uint64_t kWaveGroups = 8192;
+    const uint64_t groups = std::min<uint64_t>((ntiles + per_block - 1) / per_block, kWaveGroups);
+    if (groups == 0) return hipSuccess;
+    hipLaunchKernelGGL((wave_kernel<IN, OUT, KV>), dim3((unsigned)groups), dim3(kThreads), 0, st, d_args, dy,
+                       dy.tile0 + ntiles);
+  } else {
+    const uint64_t maxb = kMaxLaunchBlocks;
+    PassDyn d = dy;
+    for (uint64_t t0 = 0; t0 < ntiles; t0 += maxb * per_block) {
+      const uint64_t cnt = std::min<uint64_t>(ntiles - t0, maxb * per_block);
+      d.tile0 = dy.tile0 + t0;
+      hipLaunchKernelGGL((wave_kernel<IN, OUT, KV>), dim3((unsigned)((cnt + per_block - 1) / per_block)),
+                         dim3(kThreads), 0, st, d_args, d, d.tile0 + cnt);
+    }
   }
   return hipGetLastError();
 }
