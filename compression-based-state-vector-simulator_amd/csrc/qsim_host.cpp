@@ -858,6 +858,44 @@ struct CStage {
   uint32_t num_groups = 0;
   std::unique_ptr<Program> prog;
 };
+
+// Region store (QSIM_STORE_REGIONS): payload words are offsets into one
+// virtual reservation: one always-mapped unit, then two halves.  Half h
+// holds 2^k regions; region r spans `span` bytes at
+// unit + h * half_bytes + r * span and holds the blocks whose
+// key qubits read r (key[i] is bit i of r).  Physical units of `unit` bytes
+// are mapped at the front of a region as its payload grows and unmapped into
+// `pool` once the region has been consumed.
+constexpr uint32_t kRegionKeyBits = 4;
+struct RegionStore {
+  hipMemAllocationProp prop{};
+  hipMemAccessDesc acc{};
+  uint64_t unit = 0, half_bytes = 0, va_bytes = 0;
+  char* va = nullptr;
+  std::vector<hipMemGenericAllocationHandle_t> pool, all;
+  struct Half {
+    std::vector<uint32_t> key;
+    uint64_t span = 0;
+    std::vector<std::vector<hipMemGenericAllocationHandle_t>> units;
+    std::vector<uint64_t> used;  // payload words per region
+  } half[2];
+  uint64_t mapped_bytes = 0, peak_mapped_bytes = 0;
+  bool zero_mapped = false;  // the unit at offset 0
+  RegionStore() = default;
+  RegionStore(const RegionStore&) = delete;
+  RegionStore& operator=(const RegionStore&) = delete;
+  ~RegionStore() {
+    if (!va) return;
+    (void)hipDeviceSynchronize();
+    for (int h = 0; h < 2; ++h)
+      for (size_t r = 0; r < half[h].units.size(); ++r)
+        for (size_t u = 0; u < half[h].units[r].size(); ++u)
+          (void)hipMemUnmap(va + unit + h * half_bytes + r * half[h].span + u * unit, unit);
+    if (zero_mapped) (void)hipMemUnmap(va, unit);
+    for (auto hd : all) (void)hipMemRelease(hd);
+    (void)hipMemAddressFree(va, va_bytes);
+  }
+};
 }  // namespace qsim
 
 struct qsim_cs {
@@ -881,6 +919,9 @@ struct qsim_cs {
   Program io;  // zero-op program over one block (load / readout)
   Partials part;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool regions = false;  // QSIM_STORE_REGIONS
+  qsim::RegionStore rs;
+  DBuf slot_region;
 };
 
 namespace qsim {
@@ -981,9 +1022,14 @@ static PassDyn cs_dyn(qsim_cs* cs, int out) {
   PassDyn dy{};
   dy.null_in = cs->null_flags.as<uint8_t>();
   dy.desc_in = cs->desc[cs->cur].as<uint64_t>();
-  dy.arena_in = cs->arena[cs->cur].as<uint64_t>();
   dy.desc_out = cs->desc[out].as<uint64_t>();
-  dy.arena_out = cs->arena[out].as<uint64_t>();
+  if (cs->regions) {
+    dy.arena_in = reinterpret_cast<const uint64_t*>(cs->rs.va);
+    dy.arena_out = reinterpret_cast<uint64_t*>(cs->rs.va);
+  } else {
+    dy.arena_in = cs->arena[cs->cur].as<uint64_t>();
+    dy.arena_out = cs->arena[out].as<uint64_t>();
+  }
   dy.sub_base = cs->subbase[out].as<uint64_t>();
   dy.cursors = cs->cursors[out].as<unsigned long long>();
   dy.sub_mask = cs->nsub[out] - 1;
@@ -1039,7 +1085,9 @@ static void cs_commit(qsim_cs* cs, int out) {
   cs->cur = out;
 }
 
+static uint64_t rs_current_bytes(qsim_cs* cs);
 static uint64_t cs_current_bytes(qsim_cs* cs) {
+  if (cs->regions) return rs_current_bytes(cs);
   const uint64_t words = cs_read_cursor_words(cs, cs->cur, nullptr);
   uint64_t nonnull = 0;
   for (auto v : cs->h_null) nonnull += v ? 0 : 1;
@@ -1068,8 +1116,11 @@ static uint64_t cs_blocks_per_batch(qsim_cs* cs) {
   return std::max<uint64_t>(1, std::min<uint64_t>(cs->nb, cs->scratch_amps >> cs->B));
 }
 
+static void cs_load_regions(qsim_cs* cs, const double* host, const double2* dev, uint64_t basis);
+
 // Replace the whole state by a host vector (nullptr: basis state `basis`).
 static void cs_load(qsim_cs* cs, const double* host, uint64_t basis) {
+  if (cs->regions) return cs_load_regions(cs, host, nullptr, basis);
   const int out = 1 - cs->cur;
   for (int attempt = 0;; ++attempt) {
     HIPCHECK(launch_fill_u8(cs->null_flags.as<uint8_t>(), cs->nb, 1, stream()));
@@ -1105,6 +1156,7 @@ static void cs_load(qsim_cs* cs, const double* host, uint64_t basis) {
 // Replace the whole state by a device vector of 2^n amplitudes: blocks are
 // encoded straight from it (block b = amplitudes [b 2^B, (b+1) 2^B)).
 static void cs_load_device(qsim_cs* cs, const double2* dev) {
+  if (cs->regions) return cs_load_regions(cs, nullptr, dev, 0);
   const int out = 1 - cs->cur;
   std::vector<uint32_t> ids(cs->nb);
   std::iota(ids.begin(), ids.end(), 0u);
@@ -1213,7 +1265,9 @@ static std::vector<std::vector<uint32_t>> partition(qsim_cs* cs, uint32_t inner_
   return stages;
 }
 
+static void cs_run_stage_regions(qsim_cs* cs, uint32_t si);
 static void cs_run_stage(qsim_cs* cs, uint32_t si) {
+  if (cs->regions) return cs_run_stage_regions(cs, si);
   CStage& S = cs->stages[si];
   Program& P = *S.prog;
   const uint32_t k = (uint32_t)S.H.size();
@@ -1280,6 +1334,373 @@ static void cs_run_stage(qsim_cs* cs, uint32_t si) {
   HIPCHECK(hipEventElapsedTime(&ms, cs->ev0, cs->ev1));
   cs_commit(cs, out);
   const uint64_t bytes_out = cs_current_bytes(cs);
+  cs->cur_bytes = bytes_out;
+  cs->bytes_hist.push_back(bytes_out);
+  cs->timers[1] += ms * 1e-3;
+  cs->stats.push_back({ms * 1e-3, (double)bytes_in, (double)bytes_out, (double)(nact << L), (double)P.passes.size()});
+}
+
+// ------------------------------------------------------------ region store
+static void rs_init(qsim_cs* cs) {
+  RegionStore& R = cs->rs;
+  int dev = 0, vmm = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  HIPCHECK(hipDeviceGetAttribute(&vmm, hipDeviceAttributeVirtualMemoryManagementSupported, dev));
+  if (!vmm) throw std::runtime_error("region store: the device does not support virtual memory management");
+  R.prop.type = hipMemAllocationTypePinned;
+  R.prop.location.type = hipMemLocationTypeDevice;
+  R.prop.location.id = dev;
+  R.acc.location = R.prop.location;
+  R.acc.flags = hipMemAccessFlagsProtReadWrite;
+  size_t gran = 0;
+  HIPCHECK(hipMemGetAllocationGranularity(&gran, &R.prop, hipMemAllocationGranularityMinimum));
+  if (gran == 0) gran = 4096;
+  const uint64_t raw = cs->dim * 16;
+  // about 4096 units per state: 256 MiB at 36 qubits, 64 KiB at 24
+  uint64_t unit = 64ull << 10;
+  while (unit < (256ull << 20) && unit < raw / 4096) unit <<= 1;
+  R.unit = (unit + gran - 1) / gran * gran;
+  // a half holds 2^k regions of at most raw / 2^k + 2 units each
+  R.half_bytes = (raw + (2ull << kRegionKeyBits) * R.unit + R.unit - 1) / R.unit * R.unit;
+  R.va_bytes = R.unit + 2 * R.half_bytes;
+  if (R.va_bytes / 8 >= (1ull << 40)) throw std::invalid_argument("region store: state too large");
+  HIPCHECK(hipMemAddressReserve((void**)&R.va, R.va_bytes, R.unit, nullptr, 0));
+  // The first unit stays mapped: descriptors of chunks without payload hold
+  // offset 0, and the decoders read a few words there before masking them.
+  hipMemGenericAllocationHandle_t hd;
+  HIPCHECK(hipMemCreate(&hd, R.unit, &R.prop, 0));
+  R.all.push_back(hd);
+  HIPCHECK(hipMemMap(R.va, R.unit, 0, hd, 0));
+  HIPCHECK(hipMemSetAccess(R.va, R.unit, &R.acc, 1));
+  R.zero_mapped = true;
+  HIPCHECK(hipMemsetAsync(R.va, 0, R.unit, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+}
+
+static char* rs_region_ptr(qsim_cs* cs, int h, uint32_t r) {
+  return cs->rs.va + cs->rs.unit + (uint64_t)h * cs->rs.half_bytes + (uint64_t)r * cs->rs.half[h].span;
+}
+
+// Unmap region r of half h into the pool.  Waits for the stream first: the
+// region's payload may still be read by queued work.
+static void rs_release_region(qsim_cs* cs, int h, uint32_t r) {
+  RegionStore& R = cs->rs;
+  auto& u = R.half[h].units[r];
+  if (!u.empty()) {
+    HIPCHECK(hipStreamSynchronize(stream()));
+    char* base = rs_region_ptr(cs, h, r);
+    for (size_t i = 0; i < u.size(); ++i) {
+      HIPCHECK(hipMemUnmap(base + i * R.unit, R.unit));
+      R.pool.push_back(u[i]);
+    }
+    R.mapped_bytes -= u.size() * R.unit;
+    u.clear();
+  }
+  R.half[h].used[r] = 0;
+}
+
+static void rs_release_half(qsim_cs* cs, int h) {
+  for (uint32_t r = 0; r < cs->rs.half[h].units.size(); ++r) rs_release_region(cs, h, r);
+}
+
+// Empty half h and lay it out as 2^|key| regions keyed by `key`.
+static void rs_set_layout(qsim_cs* cs, int h, const std::vector<uint32_t>& key) {
+  RegionStore& R = cs->rs;
+  rs_release_half(cs, h);
+  auto& H = R.half[h];
+  H.key = key;
+  H.span = (((cs->dim * 16) >> key.size()) + 2 * R.unit - 1) / R.unit * R.unit;
+  H.units.assign(1u << key.size(), {});
+  H.used.assign(1u << key.size(), 0);
+}
+
+// Map units at the front of region r until it holds `words` payload words
+// plus the decoder's 64 bytes of padding (bounded by the region's span).
+static void rs_map(qsim_cs* cs, int h, uint32_t r, uint64_t words) {
+  RegionStore& R = cs->rs;
+  auto& H = R.half[h];
+  const uint64_t need = std::min<uint64_t>((words + 8) * 8, H.span);
+  auto& u = H.units[r];
+  char* base = rs_region_ptr(cs, h, r);
+  while (u.size() * R.unit < need) {
+    hipMemGenericAllocationHandle_t hd;
+    if (!R.pool.empty()) {
+      hd = R.pool.back();
+      R.pool.pop_back();
+    } else {
+      const hipError_t e = hipMemCreate(&hd, R.unit, &R.prop, 0);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw std::runtime_error(std::string("region store: out of device memory (") + hipGetErrorString(e) + ")");
+      }
+      R.all.push_back(hd);
+    }
+    char* p = base + u.size() * R.unit;
+    HIPCHECK(hipMemMap(p, R.unit, 0, hd, 0));
+    HIPCHECK(hipMemSetAccess(p, R.unit, &R.acc, 1));
+    u.push_back(hd);
+    R.mapped_bytes += R.unit;
+    R.peak_mapped_bytes = std::max(R.peak_mapped_bytes, R.mapped_bytes);
+  }
+}
+
+static uint32_t key_value(const qsim_cs* cs, uint64_t block, const std::vector<uint32_t>& key) {
+  uint32_t v = 0;
+  for (size_t i = 0; i < key.size(); ++i) v |= (uint32_t)((block >> (key[i] - cs->B)) & 1u) << i;
+  return v;
+}
+
+// Up to kRegionKeyBits qubits >= B outside `excl`, the highest first.
+static std::vector<uint32_t> rs_pick_key(const qsim_cs* cs, const std::set<uint32_t>& excl) {
+  std::vector<uint32_t> key;
+  for (uint32_t q = cs->n; q-- > cs->B && key.size() < kRegionKeyBits;)
+    if (!excl.count(q)) key.push_back(q);
+  std::sort(key.begin(), key.end());
+  return key;
+}
+
+// One encode into half h: maps est[r] more words per region, then runs
+// `launch` until no tile overflows.  Sub-arena 2r of the launch is region
+// r's free space [used, mapped - 8 words); its cursor counts the region's
+// demand, tiles that do not fit included, so a retry maps exactly that.
+template <class F>
+static void rs_encode(qsim_cs* cs, int h, const std::vector<uint64_t>& est, F&& launch, std::vector<uint64_t>* demand) {
+  auto& H = cs->rs.half[h];
+  const uint32_t K = (uint32_t)H.used.size();
+  for (uint32_t r = 0; r < K; ++r) rs_map(cs, h, r, H.used[r] + est[r]);
+  std::vector<unsigned long long> cur((size_t)2 * K * 16);
+  std::vector<uint64_t> sb((size_t)2 * K + 1, 0);
+  for (int attempt = 0;; ++attempt) {
+    for (uint32_t r = 0; r < K; ++r) {
+      const uint64_t w0 = (uint64_t)(rs_region_ptr(cs, h, r) - cs->rs.va) / 8;
+      const uint64_t cap = H.units[r].size() * cs->rs.unit / 8;
+      sb[2 * r] = w0 + H.used[r];
+      sb[2 * r + 1] = std::max(sb[2 * r], w0 + (cap >= 8 ? cap - 8 : 0));
+    }
+    upload(cs->subbase[h], sb.data(), sb.size() * sizeof(uint64_t));
+    HIPCHECK(launch_fill_u64(cs->cursors[h].as<unsigned long long>(), (uint64_t)2 * K * 16, 0ull, stream()));
+    HIPCHECK(launch_fill_u8(cs->overflow.as<uint8_t>(), sizeof(uint32_t), 0, stream()));
+    launch();
+    uint32_t ov = 0;
+    HIPCHECK(hipMemcpyAsync(&ov, cs->overflow.p, sizeof(ov), hipMemcpyDeviceToHost, stream()));
+    HIPCHECK(hipMemcpyAsync(cur.data(), cs->cursors[h].p, cur.size() * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, stream()));
+    HIPCHECK(hipStreamSynchronize(stream()));
+    if (!ov) {
+      for (uint32_t r = 0; r < K; ++r) {
+        H.used[r] += cur[(size_t)2 * r * 16];
+        if (demand) (*demand)[r] = cur[(size_t)2 * r * 16];
+      }
+      return;
+    }
+    if (attempt > 8) throw std::runtime_error("region store: allocation did not converge");
+    for (uint32_t r = 0; r < K; ++r) {
+      const uint64_t d = cur[(size_t)2 * r * 16];
+      rs_map(cs, h, r, H.used[r] + d + d / 16 + 64);
+    }
+  }
+}
+
+static uint64_t rs_current_bytes(qsim_cs* cs) {
+  uint64_t words = 0, nonnull = 0;
+  for (auto u : cs->rs.half[cs->cur].used) words += u;
+  for (auto v : cs->h_null) nonnull += v ? 0 : 1;
+  return 8 * (words + nonnull * cs->cpb);
+}
+
+// Load for the region store: the old state is dropped first, the new one is
+// laid out for the first registered stage.  host / dev: the full vector;
+// otherwise the basis state `basis`.
+static void cs_load_regions(qsim_cs* cs, const double* host, const double2* dev, uint64_t basis) {
+  const int out = 1 - cs->cur;
+  rs_release_half(cs, cs->cur);
+  std::set<uint32_t> excl;
+  if (!cs->stages.empty()) excl.insert(cs->stages[0].H.begin(), cs->stages[0].H.end());
+  const std::vector<uint32_t> key = rs_pick_key(cs, excl);
+  rs_set_layout(cs, out, key);
+  const uint32_t K = 1u << key.size();
+  HIPCHECK(launch_fill_u8(cs->null_flags.as<uint8_t>(), cs->nb, 1, stream()));
+  HIPCHECK(launch_fill_u8(cs->nonzero.as<uint8_t>(), cs->nb, 0, stream()));
+  const bool whole = host || dev;
+  std::vector<uint32_t> ids;
+  if (whole) {
+    ids.resize(cs->nb);
+    std::iota(ids.begin(), ids.end(), 0u);
+  } else {
+    ids.push_back((uint32_t)(basis >> cs->B));
+  }
+  std::vector<uint32_t> sr(ids.size());
+  for (size_t i = 0; i < ids.size(); ++i) sr[i] = key_value(cs, ids[i], key);
+  upload(cs->slot_region, sr.data(), sr.size() * sizeof(uint32_t));
+  upload(cs->bmap, ids.data(), ids.size() * sizeof(uint32_t));
+  // first guess: a quarter of the raw size of a full load, a little for a basis state
+  std::vector<uint64_t> est(K, whole ? ((cs->dim * 2) >> key.size()) / 4 : 64);
+  rs_encode(cs, out, est, [&] {
+    PassDyn dy = cs_dyn(cs, out);
+    dy.slot_region = cs->slot_region.as<uint32_t>();
+    dy.bmap = cs->bmap.as<uint32_t>();
+    if (dev) {
+      dy.raw = (double*)dev;
+      HIPCHECK(launch_program_pass(cs->io, 0, IN_RAW, OUT_ENC, dy, ids.size(), stream()));
+    } else if (host) {
+      dy.raw = cs->scratch.as<double>();
+      const uint64_t bpb = cs_blocks_per_batch(cs);
+      for (uint64_t b0 = 0; b0 < cs->nb; b0 += bpb) {
+        const uint64_t cnt = std::min(bpb, cs->nb - b0);
+        HIPCHECK(hipMemcpyAsync(cs->scratch.p, host + 2 * (b0 << cs->B), (cnt << cs->B) * 16, hipMemcpyHostToDevice,
+                                stream()));
+        dy.bmap = cs->bmap.as<uint32_t>() + b0;
+        dy.slot_region = cs->slot_region.as<uint32_t>() + b0;
+        HIPCHECK(launch_program_pass(cs->io, 0, IN_RAW, OUT_ENC, dy, cnt, stream()));
+        HIPCHECK(hipStreamSynchronize(stream()));
+      }
+    } else {
+      dy.raw = cs->scratch.as<double>();
+      HIPCHECK(launch_fill_basis(cs->scratch.as<double2>(), 1ull << cs->B, basis & ((1ull << cs->B) - 1), stream()));
+      HIPCHECK(launch_program_pass(cs->io, 0, IN_RAW, OUT_ENC, dy, 1, stream()));
+    }
+  }, nullptr);
+  cs_commit(cs, out);
+  cs->cur_bytes = rs_current_bytes(cs);
+}
+
+// One stage on the region store.  Active groups run in batches keyed by
+// qubits the stage does not touch, ordered so that the input's key qubits
+// are the most significant: after batch j, every input region whose last
+// live block belonged to a batch <= j is unmapped.  The output is routed to
+// regions keyed by qubits neither this stage nor the next one touches (the
+// first stage follows the last), so the next stage consumes it region by
+// region.  Peak: the state, plus the regions in flight, plus what the
+// running batch writes.
+static void cs_run_stage_regions(qsim_cs* cs, uint32_t si) {
+  CStage& S = cs->stages[si];
+  Program& P = *S.prog;
+  RegionStore& R = cs->rs;
+  const uint32_t k = (uint32_t)S.H.size();
+  const uint32_t L = cs->B + k;
+  const uint64_t G = 1ull << k;
+  const int in = cs->cur, out = 1 - cs->cur;
+  const uint64_t bytes_in = cs->cur_bytes;
+  const std::set<uint32_t> T(S.H.begin(), S.H.end());
+  // consumer key: free qubits in the low bits, the input key's untouched
+  // qubits (in their order) in the high bits
+  const auto& kin = R.half[in].key;
+  std::vector<uint32_t> compat, ckey;
+  for (auto q : kin)
+    if (!T.count(q)) compat.push_back(q);
+  for (uint32_t q = cs->n; q-- > cs->B && ckey.size() + compat.size() < kRegionKeyBits;)
+    if (!T.count(q) && std::find(compat.begin(), compat.end(), q) == compat.end()) ckey.push_back(q);
+  std::reverse(ckey.begin(), ckey.end());
+  ckey.insert(ckey.end(), compat.begin(), compat.end());
+  const CStage& N = cs->stages[si + 1 < cs->stages.size() ? si + 1 : 0];
+  std::set<uint32_t> excl = T;
+  excl.insert(N.H.begin(), N.H.end());
+  const std::vector<uint32_t> okey = rs_pick_key(cs, excl);
+  const uint32_t Kout = 1u << okey.size(), nbatch = 1u << ckey.size();
+
+  // per-block payload words of the input (descriptor widths)
+  std::vector<unsigned long long> bb(cs->nb);
+  cs->bytes_tmp.ensure(cs->nb * 8);
+  HIPCHECK(launch_block_bytes(cs->desc[in].as<uint64_t>(), cs->null_flags.as<uint8_t>(), cs->nb, cs->cpb,
+                              cs->bytes_tmp.as<unsigned long long>(), stream()));
+  HIPCHECK(hipMemcpyAsync(bb.data(), cs->bytes_tmp.p, cs->nb * 8, hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+
+  // active groups (build_stage_group_execution_plan, compressed_state.hpp:797-837) by batch
+  const uint64_t ng = 1ull << S.O.size();
+  std::vector<std::vector<uint64_t>> bgroups(nbatch);
+  {
+    std::vector<uint8_t> active(ng, 0);
+    const auto b2g = canonical_b2g(cs, S);
+    for (uint64_t b = 0; b < cs->nb; ++b)
+      if (!cs->h_null[b]) active[b2g[b]] = 1;
+    for (uint64_t g = 0; g < ng; ++g) {
+      if (!active[g]) continue;
+      uint64_t obits = 0;
+      for (size_t i = 0; i < S.O.size(); ++i) obits |= ((g >> i) & 1ull) << (S.O[i] - cs->B);
+      bgroups[key_value(cs, obits, ckey)].push_back(obits);
+    }
+  }
+  std::vector<int64_t> last_use(R.half[in].used.size(), -1);
+  std::vector<uint32_t> bm, sr;
+  std::vector<uint64_t> slot0(nbatch + 1, 0);
+  std::vector<uint64_t> est_in((size_t)nbatch * Kout, 0);
+  for (uint32_t j = 0; j < nbatch; ++j) {
+    for (uint64_t obits : bgroups[j]) {
+      const uint32_t ro = key_value(cs, obits, okey);
+      for (uint64_t jj = 0; jj < G; ++jj) {
+        uint64_t b = obits;
+        for (uint32_t i = 0; i < k; ++i) b |= ((jj >> i) & 1ull) << (S.H[i] - cs->B);
+        bm.push_back((uint32_t)b);
+        if (!cs->h_null[b]) {
+          last_use[key_value(cs, b, kin)] = j;
+          est_in[(size_t)j * Kout + ro] += bb[b] / 8 - cs->cpb;
+        }
+      }
+      sr.push_back(ro);
+    }
+    slot0[j + 1] = sr.size();
+  }
+  const uint64_t nact = sr.size();
+  if (nact == 0 || P.passes.empty()) {
+    cs->bytes_hist.push_back(bytes_in);
+    cs->stats.push_back({0.0, (double)bytes_in, (double)bytes_in, 0.0, 0.0});
+    return;
+  }
+  upload(cs->bmap, bm.data(), bm.size() * sizeof(uint32_t));
+  upload(cs->slot_region, sr.data(), sr.size() * sizeof(uint32_t));
+  HIPCHECK(launch_fill_u8(cs->nonzero.as<uint8_t>(), cs->nb, 0, stream()));
+  rs_set_layout(cs, out, okey);
+  for (uint32_t r = 0; r < last_use.size(); ++r)
+    if (last_use[r] < 0) rs_release_region(cs, in, r);
+
+  HIPCHECK(hipEventRecord(cs->ev0, stream()));
+  double ratio = 1.25;
+  std::vector<uint64_t> est(Kout), dem(Kout);
+  for (uint32_t j = 0; j < nbatch; ++j) {
+    const uint64_t a0 = slot0[j], a1 = slot0[j + 1];
+    if (a1 > a0) {
+      uint64_t ein = 0;
+      for (uint32_t r = 0; r < Kout; ++r) {
+        const uint64_t e = est_in[(size_t)j * Kout + r];
+        ein += e;
+        est[r] = (uint64_t)((double)e * ratio) + 64;
+      }
+      rs_encode(cs, out, est, [&] {
+        PassDyn dy = cs_dyn(cs, out);
+        if (P.passes.size() == 1) {
+          dy.bmap = cs->bmap.as<uint32_t>() + a0 * G;
+          dy.slot_region = cs->slot_region.as<uint32_t>() + a0;
+          HIPCHECK(launch_program_pass(P, 0, IN_DEC, OUT_ENC, dy, a1 - a0, stream()));
+        } else {
+          const uint64_t gpb = std::max<uint64_t>(1, cs->scratch_amps >> L);
+          dy.raw = cs->scratch.as<double>();
+          for (uint64_t b0 = a0; b0 < a1; b0 += gpb) {
+            const uint64_t cnt = std::min(gpb, a1 - b0);
+            dy.bmap = cs->bmap.as<uint32_t>() + b0 * G;
+            dy.slot_region = cs->slot_region.as<uint32_t>() + b0;
+            for (size_t p = 0; p < P.passes.size(); ++p) {
+              const int im = p == 0 ? IN_DEC : IN_RAW;
+              const int om = p + 1 == P.passes.size() ? OUT_ENC : OUT_RAW;
+              HIPCHECK(launch_program_pass(P, p, im, om, dy, cnt, stream()));
+            }
+          }
+        }
+      }, &dem);
+      uint64_t dsum = 0;
+      for (auto d : dem) dsum += d;
+      if (ein > 0) ratio = std::min(64.0, std::max(1.0, (double)dsum / (double)ein)) * 1.1;
+    }
+    for (uint32_t r = 0; r < last_use.size(); ++r)
+      if (last_use[r] == (int64_t)j) rs_release_region(cs, in, r);
+  }
+  HIPCHECK(hipEventRecord(cs->ev1, stream()));
+  rs_release_half(cs, in);
+  float ms = 0.f;
+  HIPCHECK(hipEventSynchronize(cs->ev1));
+  HIPCHECK(hipEventElapsedTime(&ms, cs->ev0, cs->ev1));
+  cs_commit(cs, out);
+  const uint64_t bytes_out = rs_current_bytes(cs);
   cs->cur_bytes = bytes_out;
   cs->bytes_hist.push_back(bytes_out);
   cs->timers[1] += ms * 1e-3;
@@ -1936,6 +2357,7 @@ int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** 
   require(c.eps_abs >= 0.0, "eps_abs must be >= 0");
   require(c.guard_bits >= 0 && c.guard_bits <= 20, "guard_bits out of range");
   require(c.zero_mode == QSIM_ZERO_EXACT || c.zero_mode == QSIM_ZERO_THRESHOLD, "bad zero_mode");
+  require(c.store_mode == QSIM_STORE_DOUBLE || c.store_mode == QSIM_STORE_REGIONS, "bad store_mode");
   auto cs = std::make_unique<qsim_cs>();
   cs->n = n;
   cs->B = B;
@@ -1944,9 +2366,12 @@ int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** 
   cs->dim = 1ull << n;
   cs->cpb = 1ull << (B - kChunkAmpsLog2);
   const uint64_t raw = cs->dim * 16;
+  cs->regions = c.store_mode == QSIM_STORE_REGIONS;
+  if (cs->regions) rs_init(cs.get());
   for (int w = 0; w < 2; ++w) {
     cs->desc[w].alloc(cs->nb * cs->cpb * 8);
     cs->cursors[w].alloc((size_t)kMaxSubArenas * 16 * 8);
+    if (cs->regions) continue;
     // initial capacity; an overflowing stage regrows its output arena to the
     // measured demand (QOMPSIM_ARENA_KIB sets the start, for tests)
     uint64_t init = std::min<uint64_t>(std::max<uint64_t>(raw / 16, 8ull << 20), 4ull << 30);
@@ -2411,6 +2836,7 @@ struct FileCloser {
 
 int qsim_cs_save_checkpoint(qsim_cs* cs, const char* path, uint64_t next_stage) {
   API_BEGIN
+  if (cs->regions) throw std::invalid_argument("checkpoint files need the double-buffered store");
   require(cs != nullptr && path != nullptr, "null argument");
   HIPCHECK(hipStreamSynchronize(stream()));
   const int c = cs->cur;
@@ -2446,6 +2872,7 @@ int qsim_cs_save_checkpoint(qsim_cs* cs, const char* path, uint64_t next_stage) 
 
 int qsim_cs_load_checkpoint(qsim_cs* cs, const char* path, uint64_t* next_stage) {
   API_BEGIN
+  if (cs->regions) throw std::invalid_argument("checkpoint files need the double-buffered store");
   require(cs != nullptr && path != nullptr, "null argument");
   FileCloser fc{std::fopen(path, "rb")};
   if (!fc.f) throw std::runtime_error(std::string("checkpoint: cannot open ") + path);
@@ -2491,6 +2918,18 @@ int qsim_cs_max_sv_bytes(const qsim_cs* cs, uint64_t* out) {
   uint64_t m = 0;
   for (auto v : cs->bytes_hist) m = std::max(m, v);
   *out = m;
+  API_END
+}
+
+int qsim_cs_store_bytes(const qsim_cs* cs, uint64_t* out3) {
+  API_BEGIN
+  if (cs->regions) {
+    out3[0] = cs->rs.mapped_bytes;
+    out3[1] = cs->rs.peak_mapped_bytes;
+  } else {
+    out3[0] = out3[1] = cs->arena[0].bytes + cs->arena[1].bytes;
+  }
+  out3[2] = cs->desc[0].bytes + cs->desc[1].bytes;
   API_END
 }
 

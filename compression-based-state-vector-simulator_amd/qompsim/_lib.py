@@ -16,6 +16,7 @@ MAX_GATE_QUBITS = 8
 COMMUTE_X, COMMUTE_Y, COMMUTE_Z = 1, 2, 4
 GATE_PARAMETRIC = 1
 ZERO_EXACT, ZERO_THRESHOLD = 0, 1
+STORE_DOUBLE, STORE_REGIONS = 0, 1
 TIMER_KEYS = ("decomp", "gate", "comp", "fused_swap", "allgather", "hmalloc", "dmalloc", "h2d", "d2h", "dfree")
 
 GATE_DTYPE = np.dtype(
@@ -45,7 +46,7 @@ class CsConfig(ctypes.Structure):
         ("zero_threshold", ctypes.c_double),
         ("scratch_bytes", ctypes.c_uint64),
         ("tile_bits", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("store_mode", ctypes.c_uint32),
     ]
 
 
@@ -127,6 +128,7 @@ _SIGS = {
     "qsim_cs_block_bytes": (ctypes.c_int, [_vp, _vp]),
     "qsim_cs_bytes_history": (ctypes.c_int, [_vp, _vp, _u64, _P(_u64)]),
     "qsim_cs_max_sv_bytes": (ctypes.c_int, [_vp, _P(_u64)]),
+    "qsim_cs_store_bytes": (ctypes.c_int, [_vp, _vp]),
     "qsim_cs_stage_stats": (ctypes.c_int, [_vp, _vp, _u64, _P(_u64)]),
     "qsim_cs_times": (ctypes.c_int, [_vp, _vp]),
     "qsim_cs_norm_squared": (ctypes.c_int, [_vp, _P(_dbl)]),

@@ -126,8 +126,17 @@ class CompressedQuantumState:
     """Block-compressed state in HBM (CompressedQuantumState(qubit_count, block_size))."""
 
     def __init__(self, qubit_count, block_size, *, eps_rel=None, eps_abs=None, guard_bits=None,
-                 zero_mode=None, zero_threshold=None, scratch_bytes=None, tile_bits=None, verbose=True):
+                 zero_mode=None, zero_threshold=None, scratch_bytes=None, tile_bits=None, store="double",
+                 verbose=True):
+        """store: "double" (two whole arenas) or "regions" (payload regions
+        mapped on demand and freed as a stage consumes them: about one
+        compressed state of peak memory instead of two; no checkpoint
+        files)."""
         cfg = default_codec_config()
+        stores = {"double": _lib.STORE_DOUBLE, "regions": _lib.STORE_REGIONS}
+        if store not in stores:
+            raise ValueError(f"store must be one of {sorted(stores)}")
+        cfg.store_mode = stores[store]
         if eps_rel is not None:
             cfg.eps_rel = eps_rel
         if eps_abs is not None:
@@ -302,6 +311,13 @@ class CompressedQuantumState:
         out = np.empty(self.nblocks, dtype=np.uint64)
         call("qsim_cs_block_bytes", self._h, ptr(out))
         return out
+
+    def get_store_bytes(self):
+        """Device bytes of the payload store: {"arena": held now, "arena_peak":
+        most held at once, "descriptors": the two descriptor tables}."""
+        out = np.zeros(3, dtype=np.uint64)
+        call("qsim_cs_store_bytes", self._h, ptr(out))
+        return {"arena": int(out[0]), "arena_peak": int(out[1]), "descriptors": int(out[2])}
 
     def get_bytes_history(self):
         n = ctypes.c_uint64()

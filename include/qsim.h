@@ -135,8 +135,22 @@ typedef struct qsim_cs_config {
   double zero_threshold;   /* ALL_ZERO_THRESHOLD for QSIM_ZERO_THRESHOLD */
   uint64_t scratch_bytes;  /* HBM staging for multi-pass stages (0 = default) */
   uint32_t tile_bits;      /* LDS tile log2 amplitudes (0 = default) */
-  uint32_t reserved;
+  uint32_t store_mode;     /* QSIM_STORE_DOUBLE (default) or QSIM_STORE_REGIONS */
 } qsim_cs_config;
+
+/* Payload store of a compressed state (no reference counterpart: the
+ * reference keeps one host buffer per block).
+ * QSIM_STORE_DOUBLE: two whole arenas, a stage reads one and writes the
+ *   other (peak about twice the compressed state).
+ * QSIM_STORE_REGIONS: the payload lives in up to 16 regions of a virtual
+ *   address reservation, backed by physical units mapped on demand.  A stage
+ *   consumes its input region by region and routes its output into the
+ *   regions the next stage will consume; each input region's units are
+ *   unmapped and reused as soon as it has been consumed (peak about one
+ *   compressed state plus one region).  Checkpoint files are not supported
+ *   in this mode. */
+#define QSIM_STORE_DOUBLE 0
+#define QSIM_STORE_REGIONS 1
 
 typedef struct qsim_cs qsim_cs;
 int qsim_cs_default_config(qsim_cs_config* cfg);
@@ -197,6 +211,10 @@ int qsim_cs_block_bytes(qsim_cs* cs, uint64_t* out);
  * initial state.  *n receives the history length. */
 int qsim_cs_bytes_history(const qsim_cs* cs, uint64_t* out, uint64_t cap, uint64_t* n);
 int qsim_cs_max_sv_bytes(const qsim_cs* cs, uint64_t* out);
+/* Device memory of the payload store (no reference counterpart): out3[0] the
+ * arena bytes held now (region store: mapped units), out3[1] the most held
+ * at once, out3[2] the descriptor tables. */
+int qsim_cs_store_bytes(const qsim_cs* cs, uint64_t* out3);
 /* Per-stage measurement of the last run: for each executed stage
  * {gpu_seconds, bytes_in, bytes_out, decompressed_amplitudes, n_passes}. */
 int qsim_cs_stage_stats(const qsim_cs* cs, double* out5, uint64_t cap, uint64_t* n);

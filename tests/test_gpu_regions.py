@@ -1,0 +1,131 @@
+"""Region store (store="regions", QSIM_STORE_REGIONS): the stage loop with its
+payload in regions mapped on demand and released as they are consumed.  The
+codec, the gates and the null bookkeeping are those of the double-buffered
+store, so every result must be bit-identical to it; only where the payload
+sits, and how much device memory it holds, differ."""
+import numpy as np
+import pytest
+
+from gpu_util import rand_state, random_circuit
+from qompsim import CompressedQuantumState, QuantumCircuit, QuantumCircuitOptimizer
+from qompsim import workloads as W
+from qompsim.observable import Observable
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(n, B, **kw):
+    kw.setdefault("eps_rel", 1e-3)
+    kw.setdefault("eps_abs", 1e-7)
+    return (CompressedQuantumState(n, B, store="double", verbose=False, **kw),
+            CompressedQuantumState(n, B, store="regions", verbose=False, **kw))
+
+
+def _tfim_step(n, first):
+    """One Trotter step of the transverse-field Ising chain (the first one
+    starts with X on qubit 1)."""
+    c = QuantumCircuit(n)
+    if first:
+        c.add_X_gate(1)
+    for i in range(n - 1):
+        c.add_multi_Pauli_rotation_gate([i, i + 1], [3, 3], 0.2)
+    for q in range(n):
+        c.add_RX_gate(q, 0.3)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    return c
+
+
+def _same(a, b):
+    assert np.array_equal(a.get_raw_vector().view(np.uint64), b.get_raw_vector().view(np.uint64))
+    assert np.array_equal(a.get_null_bitmap(), b.get_null_bitmap())
+    assert np.array_equal(a.get_block_bytes(), b.get_block_bytes())
+    assert a.get_bytes_history() == b.get_bytes_history()
+
+
+@pytest.mark.parametrize("n,B,I,seed", [(16, 6, 4, 1), (18, 8, 3, 2), (14, 5, 2, 3)])
+def test_regions_match_double_store(n, B, I, seed):
+    c = random_circuit(n, 150, np.random.default_rng(seed), max_k=2)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    a, b = _pair(n, B)
+    for cs in (a, b):
+        cs.partition_and_register_circuit(c, I)
+        cs.apply_circuit()
+    _same(a, b)
+
+
+def test_regions_loaded_state_qft(rng):
+    n, B = 16, 7
+    v = rand_state(n, rng)
+    v[: 3 << B] = 0  # three null blocks at load
+    c = W.qft_on_product(n, seed=4)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    a, b = _pair(n, B)
+    for cs in (a, b):
+        cs.load(v)
+        cs.partition_and_register_circuit(c, 4)
+        cs.apply_circuit()
+    _same(a, b)
+
+
+def test_regions_multipass_lds_engine(monkeypatch):
+    """Stages of several passes through the staging buffer (64-amplitude LDS
+    tiles) write their last pass into the regions too."""
+    monkeypatch.setenv("QOMPSIM_ENGINE", "lds")
+    n, B = 13, 6
+    c = random_circuit(n, 120, np.random.default_rng(77), max_k=2)
+    a, b = _pair(n, B, tile_bits=6)
+    for cs in (a, b):
+        cs.partition_and_register_circuit(c, 3)
+        cs.apply_circuit()
+    assert max(b.get_stage_passes()) > 1
+    _same(a, b)
+
+
+def test_regions_successive_circuits_and_readouts():
+    """Trotter steps registered one after another (the first stage of each
+    step consumes the last stage's layout), then the read-out paths."""
+    n, B = 18, 8
+    a, b = _pair(n, B)
+    for s in range(3):
+        c = _tfim_step(n, s == 0)
+        for cs in (a, b):
+            cs.partition_and_register_circuit(c, 4)
+            cs.apply_circuit()
+        _same(a, b)
+    obs = Observable(n)
+    obs.add_operator(1.0, "Z 0")
+    obs.add_operator(0.5, "X 3 X 12")
+    assert a.get_expectation_value(obs) == b.get_expectation_value(obs)
+    assert a.get_squared_norm() == b.get_squared_norm()
+    assert a.sampling_blockwise(500, 9) == b.sampling_blockwise(500, 9)
+
+
+def test_regions_basis_and_reset():
+    a, b = _pair(12, 6)
+    for cs in (a, b):
+        cs.set_computational_basis(2345)
+    _same(a, b)
+    assert b.get_nonnull_blocks() == [2345 >> 6]
+    st = b.get_store_bytes()
+    assert st["arena"] > 0 and st["arena_peak"] >= st["arena"]
+
+
+def test_regions_reject_checkpoint_files(tmp_path):
+    _, b = _pair(12, 6)
+    with pytest.raises(ValueError, match="double-buffered"):
+        b.save_checkpoint(str(tmp_path / "x.ckpt"), 0)
+
+
+def test_regions_peak_below_two_states():
+    """Two Trotter steps at 24 qubits: the region store's largest mapping
+    stays well below the two whole arenas a double-buffered stage needs
+    (input and output of the largest stage)."""
+    n, B = 24, 10
+    b = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, store="regions", verbose=False)
+    for s in range(2):
+        c = _tfim_step(n, s == 0)
+        b.partition_and_register_circuit(c, 4)
+        b.apply_circuit()
+    big = max(b.get_bytes_history())
+    st = b.get_store_bytes()
+    assert st["arena_peak"] < 1.6 * big, (st, big)
