@@ -211,7 +211,7 @@ def adjoint_bench(n=28, layers=8, reps=3):
     }
 
 
-def tfim_bench(n=36, B=22, inner=4, steps=12, n_ref=32):
+def tfim_bench(n=36, B=22, inner=4, steps=12, n_ref=32, store="regions"):
     """Transverse-field Ising Trotter evolution on the compressed store: X on
     qubit 1, then per step RZZ(0.2) on the open chain and RX(0.3) on every
     qubit.  Each step is registered and run as its own circuit so the
@@ -222,7 +222,7 @@ def tfim_bench(n=36, B=22, inner=4, steps=12, n_ref=32):
     from qompsim import workloads as W
     from qompsim.observable import Observable
 
-    cs = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, verbose=False)
+    cs = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, store=store, verbose=False)
     raw = 16.0 * 2.0 ** n
     per_step, gates, t_run, oom = [], 0, 0.0, None
     for s in range(steps):
@@ -248,12 +248,14 @@ def tfim_bench(n=36, B=22, inner=4, steps=12, n_ref=32):
         t_run += dt
         hist = cs.get_bytes_history()
         nrm = cs.get_squared_norm()
+        st = cs.get_store_bytes()
         per_step.append({"step": s + 1, "ms": dt * 1e3, "stages": cs.get_num_stages(),
                          "compressed_GiB": hist[-1] / 2**30, "peak_GiB": max(hist) / 2**30,
                          "ratio": raw / hist[-1], "norm_drift": nrm - 1.0,
-                         "nonnull_blocks": len(cs.get_nonnull_blocks())})
+                         "nonnull_blocks": len(cs.get_nonnull_blocks()),
+                         "arena_peak_GiB": st["arena_peak"] / 2**30, "descriptors_GiB": st["descriptors"] / 2**30})
         log(f"tfim-{n} step {s + 1}: {dt * 1e3:.0f} ms, {hist[-1] / 2**30:.2f} GiB ({raw / hist[-1]:.1f}x), "
-            f"norm-1 {nrm - 1:.2e}")
+            f"arena peak {st['arena_peak'] / 2**30:.2f} GiB, norm-1 {nrm - 1:.2e}")
     z0 = Observable(n)
     z0.add_operator(1.0, "Z 0")
     zc = cs.get_expectation_value(z0) / cs.get_squared_norm() if oom is None else None
@@ -270,6 +272,9 @@ def tfim_bench(n=36, B=22, inner=4, steps=12, n_ref=32):
     del sv
     return {
         "workload": f"TFIM-{n} Trotter, {steps} steps, compressed (B={B}, I={inner}, eps_rel 1e-3, eps_abs 1e-7)",
+        "store": store,
+        "arena_peak_GiB": max(p["arena_peak_GiB"] for p in per_step) if per_step else None,
+        "descriptors_GiB": per_step[-1]["descriptors_GiB"] if per_step else None,
         "steps_completed": done,
         "stopped_by": oom,
         "gates": gates,
