@@ -25,6 +25,13 @@ __device__ __forceinline__ uint64_t ins0_64(uint64_t x, uint32_t pos) {
   return lo | ((x >> pos) << (pos + 1));
 }
 
+// Allocation piece of a tile in the region store: the tile index folded so
+// that pieces mix tiles of every low-bit pattern (payload sizes can follow
+// the tile index's low bits, which select fixed amplitude positions).
+__device__ __forceinline__ uint32_t region_piece(uint64_t t, uint32_t mask) {
+  return (uint32_t)(t ^ (t >> 6) ^ (t >> 12) ^ (t >> 18) ^ (t >> 24)) & mask;
+}
+
 template <class EPtr>
 struct TileMapT {
   uint32_t t, c, nE;

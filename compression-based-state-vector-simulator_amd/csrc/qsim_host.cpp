@@ -867,6 +867,7 @@ struct CStage {
 // are mapped at the front of a region as its payload grows and unmapped into
 // `pool` once the region has been consumed.
 constexpr uint32_t kRegionKeyBits = 4;
+constexpr uint32_t kRegionSubs = 64;  // allocation cursors per region
 struct RegionStore {
   hipMemAllocationProp prop{};
   hipMemAccessDesc acc{};
@@ -877,10 +878,14 @@ struct RegionStore {
     std::vector<uint32_t> key;
     uint64_t span = 0;
     std::vector<std::vector<hipMemGenericAllocationHandle_t>> units;
-    std::vector<uint64_t> used;  // payload words per region
+    std::vector<uint64_t> used;     // words from the region start to the end of its last piece
+    std::vector<uint64_t> payload;  // payload words per region (used less the gaps)
   } half[2];
   uint64_t mapped_bytes = 0, peak_mapped_bytes = 0;
   bool zero_mapped = false;  // the unit at offset 0
+  // counters of the current stage (QOMPSIM_REGION_LOG=1 prints them)
+  uint64_t n_launch = 0, n_retry = 0, n_map = 0, n_unmap = 0;
+  double t_map = 0.0, t_unmap = 0.0, t_wait = 0.0;
   RegionStore() = default;
   RegionStore(const RegionStore&) = delete;
   RegionStore& operator=(const RegionStore&) = delete;
@@ -1032,7 +1037,7 @@ static PassDyn cs_dyn(qsim_cs* cs, int out) {
   }
   dy.sub_base = cs->subbase[out].as<uint64_t>();
   dy.cursors = cs->cursors[out].as<unsigned long long>();
-  dy.sub_mask = cs->nsub[out] - 1;
+  dy.sub_mask = cs->regions ? kRegionSubs - 1 : cs->nsub[out] - 1;
   dy.overflow = cs->overflow.as<uint32_t>();
   dy.nonzero = cs->nonzero.as<uint8_t>();
   return dy;
@@ -1356,12 +1361,13 @@ static void rs_init(qsim_cs* cs) {
   HIPCHECK(hipMemGetAllocationGranularity(&gran, &R.prop, hipMemAllocationGranularityMinimum));
   if (gran == 0) gran = 4096;
   const uint64_t raw = cs->dim * 16;
-  // about 4096 units per state: 256 MiB at 36 qubits, 64 KiB at 24
+  // about 1024 units per raw state (256 MiB from 34 qubits up, 64 KiB at 22):
+  // unmapping costs about 0.25 ms a unit whatever its size
   uint64_t unit = 64ull << 10;
-  while (unit < (256ull << 20) && unit < raw / 4096) unit <<= 1;
+  while (unit < (256ull << 20) && unit < raw / 1024) unit <<= 1;
   R.unit = (unit + gran - 1) / gran * gran;
-  // a half holds 2^k regions of at most raw / 2^k + 2 units each
-  R.half_bytes = (raw + (2ull << kRegionKeyBits) * R.unit + R.unit - 1) / R.unit * R.unit;
+  // a half holds 2^k regions of at most 9/8 raw / 2^k + 2 units each
+  R.half_bytes = (raw + raw / 8 + (2ull << kRegionKeyBits) * R.unit + R.unit - 1) / R.unit * R.unit;
   R.va_bytes = R.unit + 2 * R.half_bytes;
   if (R.va_bytes / 8 >= (1ull << 40)) throw std::invalid_argument("region store: state too large");
   HIPCHECK(hipMemAddressReserve((void**)&R.va, R.va_bytes, R.unit, nullptr, 0));
@@ -1381,22 +1387,27 @@ static char* rs_region_ptr(qsim_cs* cs, int h, uint32_t r) {
   return cs->rs.va + cs->rs.unit + (uint64_t)h * cs->rs.half_bytes + (uint64_t)r * cs->rs.half[h].span;
 }
 
-// Unmap region r of half h into the pool.  Waits for the stream first: the
-// region's payload may still be read by queued work.
-static void rs_release_region(qsim_cs* cs, int h, uint32_t r) {
+// Unmap region r of half h into the pool.  With `wait`, waits for the
+// stream first (queued work may still read the region); without, the caller
+// knows that no queued work reads it.
+static void rs_release_region(qsim_cs* cs, int h, uint32_t r, bool wait = true) {
   RegionStore& R = cs->rs;
   auto& u = R.half[h].units[r];
   if (!u.empty()) {
-    HIPCHECK(hipStreamSynchronize(stream()));
+    if (wait) HIPCHECK(hipStreamSynchronize(stream()));
+    const auto t0 = std::chrono::steady_clock::now();
     char* base = rs_region_ptr(cs, h, r);
     for (size_t i = 0; i < u.size(); ++i) {
       HIPCHECK(hipMemUnmap(base + i * R.unit, R.unit));
       R.pool.push_back(u[i]);
     }
     R.mapped_bytes -= u.size() * R.unit;
+    R.n_unmap += u.size();
     u.clear();
+    R.t_unmap += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   R.half[h].used[r] = 0;
+  R.half[h].payload[r] = 0;
 }
 
 static void rs_release_half(qsim_cs* cs, int h) {
@@ -1409,19 +1420,27 @@ static void rs_set_layout(qsim_cs* cs, int h, const std::vector<uint32_t>& key) 
   rs_release_half(cs, h);
   auto& H = R.half[h];
   H.key = key;
-  H.span = (((cs->dim * 16) >> key.size()) + 2 * R.unit - 1) / R.unit * R.unit;
+  // An encode lays out at most the raw size of its blocks (capped first
+  // estimate) or exactly their demand (retry), so a region's pieces never
+  // pass its raw size; the span adds 1/8 and a unit of margin.
+  const uint64_t rr = (cs->dim * 16) >> key.size();
+  H.span = (rr + rr / 8 + 2 * R.unit - 1) / R.unit * R.unit;
   H.units.assign(1u << key.size(), {});
   H.used.assign(1u << key.size(), 0);
+  H.payload.assign(1u << key.size(), 0);
 }
 
 // Map units at the front of region r until it holds `words` payload words
-// plus the decoder's 64 bytes of padding (bounded by the region's span).
+// plus the decoder's 64 bytes of padding.
 static void rs_map(qsim_cs* cs, int h, uint32_t r, uint64_t words) {
   RegionStore& R = cs->rs;
   auto& H = R.half[h];
-  const uint64_t need = std::min<uint64_t>((words + 8) * 8, H.span);
+  const uint64_t need = (words + 8) * 8;
+  if (need > H.span) throw std::logic_error("region store: pieces past the region's span");
   auto& u = H.units[r];
   char* base = rs_region_ptr(cs, h, r);
+  if (u.size() * R.unit >= need) return;
+  const auto t0 = std::chrono::steady_clock::now();
   while (u.size() * R.unit < need) {
     hipMemGenericAllocationHandle_t hd;
     if (!R.pool.empty()) {
@@ -1441,7 +1460,9 @@ static void rs_map(qsim_cs* cs, int h, uint32_t r, uint64_t words) {
     u.push_back(hd);
     R.mapped_bytes += R.unit;
     R.peak_mapped_bytes = std::max(R.peak_mapped_bytes, R.mapped_bytes);
+    ++R.n_map;
   }
+  R.t_map += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 static uint32_t key_value(const qsim_cs* cs, uint64_t block, const std::vector<uint32_t>& key) {
@@ -1459,51 +1480,79 @@ static std::vector<uint32_t> rs_pick_key(const qsim_cs* cs, const std::set<uint3
   return key;
 }
 
-// One encode into half h: maps est[r] more words per region, then runs
-// `launch` until no tile overflows.  Sub-arena 2r of the launch is region
-// r's free space [used, mapped - 8 words); its cursor counts the region's
-// demand, tiles that do not fit included, so a retry maps exactly that.
-template <class F>
-static void rs_encode(qsim_cs* cs, int h, const std::vector<uint64_t>& est, F&& launch, std::vector<uint64_t>* demand) {
+// One encode into half h.  Region r's next est[r] words are cut into
+// kRegionSubs pieces, one allocation cursor each (sub-arena r kRegionSubs + i
+// of the launch; a tile t takes i = t mod kRegionSubs), so the waves working
+// on one group spread their atomics.  Pieces are laid out in order; what a
+// piece leaves unused stays a gap.  On overflow the cursors hold every
+// piece's demand (tiles that did not fit included) and the retry lays the
+// pieces out at exactly that demand.  `during` runs once, while the first
+// attempt's kernels are in flight.
+template <class F, class D>
+static void rs_encode(qsim_cs* cs, int h, const std::vector<uint64_t>& est, F&& launch, D&& during,
+                      std::vector<uint64_t>* demand) {
   auto& H = cs->rs.half[h];
-  const uint32_t K = (uint32_t)H.used.size();
-  for (uint32_t r = 0; r < K; ++r) rs_map(cs, h, r, H.used[r] + est[r]);
-  std::vector<unsigned long long> cur((size_t)2 * K * 16);
-  std::vector<uint64_t> sb((size_t)2 * K + 1, 0);
+  const uint32_t K = (uint32_t)H.used.size(), M = kRegionSubs, S = K * M;
+  std::vector<uint64_t> piece((size_t)S), w0(K);
+  for (uint32_t r = 0; r < K; ++r) {
+    w0[r] = (uint64_t)(rs_region_ptr(cs, h, r) - cs->rs.va) / 8;
+    for (uint32_t i = 0; i < M; ++i) piece[(size_t)r * M + i] = (est[r] + M - 1) / M;
+  }
+  std::vector<unsigned long long> cur((size_t)2 * S * 16);
+  std::vector<uint64_t> sb((size_t)2 * S + 1, 0);
+  bool ran = false;
   for (int attempt = 0;; ++attempt) {
     for (uint32_t r = 0; r < K; ++r) {
-      const uint64_t w0 = (uint64_t)(rs_region_ptr(cs, h, r) - cs->rs.va) / 8;
-      const uint64_t cap = H.units[r].size() * cs->rs.unit / 8;
-      sb[2 * r] = w0 + H.used[r];
-      sb[2 * r + 1] = std::max(sb[2 * r], w0 + (cap >= 8 ? cap - 8 : 0));
+      uint64_t w = w0[r] + H.used[r];
+      for (uint32_t i = 0; i < M; ++i) {
+        const size_t s = (size_t)r * M + i;
+        sb[2 * s] = w;
+        w += piece[s];
+        sb[2 * s + 1] = w;
+      }
+      rs_map(cs, h, r, w - w0[r]);
     }
     upload(cs->subbase[h], sb.data(), sb.size() * sizeof(uint64_t));
-    HIPCHECK(launch_fill_u64(cs->cursors[h].as<unsigned long long>(), (uint64_t)2 * K * 16, 0ull, stream()));
+    HIPCHECK(launch_fill_u64(cs->cursors[h].as<unsigned long long>(), (uint64_t)2 * S * 16, 0ull, stream()));
     HIPCHECK(launch_fill_u8(cs->overflow.as<uint8_t>(), sizeof(uint32_t), 0, stream()));
     launch();
+    ++cs->rs.n_launch;
+    if (!ran) {
+      ran = true;
+      during();
+    }
     uint32_t ov = 0;
     HIPCHECK(hipMemcpyAsync(&ov, cs->overflow.p, sizeof(ov), hipMemcpyDeviceToHost, stream()));
     HIPCHECK(hipMemcpyAsync(cur.data(), cs->cursors[h].p, cur.size() * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, stream()));
+    const auto t0 = std::chrono::steady_clock::now();
     HIPCHECK(hipStreamSynchronize(stream()));
+    cs->rs.t_wait += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!ov) {
       for (uint32_t r = 0; r < K; ++r) {
-        H.used[r] += cur[(size_t)2 * r * 16];
-        if (demand) (*demand)[r] = cur[(size_t)2 * r * 16];
+        uint64_t tot = 0, end = H.used[r];
+        for (uint32_t i = 0; i < M; ++i) {
+          const size_t s = (size_t)r * M + i;
+          const uint64_t d = cur[2 * s * 16];
+          tot += d;
+          if (d) end = sb[2 * s] + d - w0[r];
+        }
+        H.used[r] = end;
+        H.payload[r] += tot;
+        if (demand) (*demand)[r] = tot;
       }
       return;
     }
     if (attempt > 8) throw std::runtime_error("region store: allocation did not converge");
-    for (uint32_t r = 0; r < K; ++r) {
-      const uint64_t d = cur[(size_t)2 * r * 16];
-      rs_map(cs, h, r, H.used[r] + d + d / 16 + 64);
-    }
+    ++cs->rs.n_retry;
+    // the kernels are deterministic: a retry asks for the same sizes
+    for (size_t s = 0; s < S; ++s) piece[s] = cur[2 * s * 16];
   }
 }
 
 static uint64_t rs_current_bytes(qsim_cs* cs) {
   uint64_t words = 0, nonnull = 0;
-  for (auto u : cs->rs.half[cs->cur].used) words += u;
+  for (auto u : cs->rs.half[cs->cur].payload) words += u;
   for (auto v : cs->h_null) nonnull += v ? 0 : 1;
   return 8 * (words + nonnull * cs->cpb);
 }
@@ -1559,7 +1608,7 @@ static void cs_load_regions(qsim_cs* cs, const double* host, const double2* dev,
       HIPCHECK(launch_fill_basis(cs->scratch.as<double2>(), 1ull << cs->B, basis & ((1ull << cs->B) - 1), stream()));
       HIPCHECK(launch_program_pass(cs->io, 0, IN_RAW, OUT_ENC, dy, 1, stream()));
     }
-  }, nullptr);
+  }, [] {}, nullptr);
   cs_commit(cs, out);
   cs->cur_bytes = rs_current_bytes(cs);
 }
@@ -1624,10 +1673,11 @@ static void cs_run_stage_regions(qsim_cs* cs, uint32_t si) {
   std::vector<int64_t> last_use(R.half[in].used.size(), -1);
   std::vector<uint32_t> bm, sr;
   std::vector<uint64_t> slot0(nbatch + 1, 0);
-  std::vector<uint64_t> est_in((size_t)nbatch * Kout, 0);
+  std::vector<uint64_t> est_in((size_t)nbatch * Kout, 0), raw_out((size_t)nbatch * Kout, 0);
   for (uint32_t j = 0; j < nbatch; ++j) {
     for (uint64_t obits : bgroups[j]) {
       const uint32_t ro = key_value(cs, obits, okey);
+      raw_out[(size_t)j * Kout + ro] += G * cs->cpb * 64;
       for (uint64_t jj = 0; jj < G; ++jj) {
         uint64_t b = obits;
         for (uint32_t i = 0; i < k; ++i) b |= ((jj >> i) & 1ull) << (S.H[i] - cs->B);
@@ -1654,17 +1704,28 @@ static void cs_run_stage_regions(qsim_cs* cs, uint32_t si) {
   for (uint32_t r = 0; r < last_use.size(); ++r)
     if (last_use[r] < 0) rs_release_region(cs, in, r);
 
+  R.n_launch = R.n_retry = R.n_map = R.n_unmap = 0;
+  R.t_map = R.t_unmap = R.t_wait = 0.0;
+  const auto tw0 = std::chrono::steady_clock::now();
   HIPCHECK(hipEventRecord(cs->ev0, stream()));
   double ratio = 1.25;
   std::vector<uint64_t> est(Kout), dem(Kout);
+  // input regions consumed by the batches so far: unmapped while the next
+  // batch runs (no later batch reads them)
+  std::vector<uint32_t> pending;
+  auto release_pending = [&] {
+    for (auto r : pending) rs_release_region(cs, in, r, false);
+    pending.clear();
+  };
   for (uint32_t j = 0; j < nbatch; ++j) {
     const uint64_t a0 = slot0[j], a1 = slot0[j + 1];
+    if (a1 == a0) release_pending();
     if (a1 > a0) {
       uint64_t ein = 0;
       for (uint32_t r = 0; r < Kout; ++r) {
         const uint64_t e = est_in[(size_t)j * Kout + r];
         ein += e;
-        est[r] = (uint64_t)((double)e * ratio) + 64;
+        est[r] = std::min<uint64_t>((uint64_t)((double)e * ratio) + 64, raw_out[(size_t)j * Kout + r]);
       }
       rs_encode(cs, out, est, [&] {
         PassDyn dy = cs_dyn(cs, out);
@@ -1686,21 +1747,31 @@ static void cs_run_stage_regions(qsim_cs* cs, uint32_t si) {
             }
           }
         }
-      }, &dem);
+      }, release_pending, &dem);
       uint64_t dsum = 0;
       for (auto d : dem) dsum += d;
       if (ein > 0) ratio = std::min(64.0, std::max(1.0, (double)dsum / (double)ein)) * 1.1;
     }
     for (uint32_t r = 0; r < last_use.size(); ++r)
-      if (last_use[r] == (int64_t)j) rs_release_region(cs, in, r);
+      if (last_use[r] == (int64_t)j) pending.push_back(r);
   }
   HIPCHECK(hipEventRecord(cs->ev1, stream()));
+  release_pending();
   rs_release_half(cs, in);
   float ms = 0.f;
   HIPCHECK(hipEventSynchronize(cs->ev1));
   HIPCHECK(hipEventElapsedTime(&ms, cs->ev0, cs->ev1));
   cs_commit(cs, out);
   const uint64_t bytes_out = rs_current_bytes(cs);
+  if (const char* e = std::getenv("QOMPSIM_REGION_LOG"); e && *e == '1') {
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - tw0).count();
+    std::fprintf(stderr,
+                 "[regions] stage %u: %u batches, %llu launches, %llu retries, map %llu units %.1f ms, unmap %llu "
+                 "units %.1f ms, wait %.1f ms, events %.1f ms, wall %.1f ms, mapped %.2f GiB\n",
+                 si, nbatch, (unsigned long long)R.n_launch, (unsigned long long)R.n_retry,
+                 (unsigned long long)R.n_map, R.t_map * 1e3, (unsigned long long)R.n_unmap, R.t_unmap * 1e3,
+                 R.t_wait * 1e3, ms, wall * 1e3, R.mapped_bytes / 1073741824.0);
+  }
   cs->cur_bytes = bytes_out;
   cs->bytes_hist.push_back(bytes_out);
   cs->timers[1] += ms * 1e-3;

@@ -130,7 +130,8 @@ struct PassDyn {
   uint64_t* desc_out;
   uint64_t* arena_out;
   const uint64_t* sub_base; // sub-arena s spans words [sub_base[s], sub_base[s + 1])
-  const uint32_t* slot_region;  // region store: group slot -> output region r (sub-arena 2r); else null
+  const uint32_t* slot_region;  // region store: group slot -> output region r (sub-arenas r (sub_mask + 1) + i,
+                                //   bounds at sub_base[2s], sub_base[2s + 1]); else null
   unsigned long long* cursors;  // [sub_mask + 1] (stride 16 words)
   uint32_t sub_mask;        // number of sub-arenas - 1 (power of two)
   uint32_t pad_;

@@ -298,7 +298,9 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(const PassArgs* __restri
       }
       if (threadIdx.x < NCH) cs->pre[threadIdx.x] = woff + incl - v;
       if (threadIdx.x == 0) {
-        const uint32_t sub = dy.slot_region ? 2u * dy.slot_region[slot] : (blockIdx.x & dy.sub_mask);
+        const uint32_t sub = dy.slot_region
+                                 ? 2u * (dy.slot_region[slot] * (dy.sub_mask + 1u) + region_piece(blockIdx.x, dy.sub_mask))
+                                 : (blockIdx.x & dy.sub_mask);
         unsigned long long off = 0;
         uint32_t skip = 0;
         if (tot > 0) {

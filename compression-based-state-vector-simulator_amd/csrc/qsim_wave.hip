@@ -866,8 +866,10 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
     const uint32_t loff = incl - w;
     const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     // one allocation per wave tile; its latency is hidden behind the packing.
-    // The region store routes a group's tiles to its output region's cursor.
-    const uint32_t sub = dy.slot_region ? 2u * dy.slot_region[slot] : (uint32_t)(wt & dy.sub_mask);
+    // The region store routes a group's tiles to its output region's
+    // sub-arenas (pairs of bounds: sub-arena s at sub_base[2s], [2s + 1]).
+    const uint32_t sub = dy.slot_region ? 2u * (dy.slot_region[slot] * (dy.sub_mask + 1u) + region_piece(wt, dy.sub_mask))
+                                        : (uint32_t)(wt & dy.sub_mask);
     unsigned long long aoff = 0;
     if (lane == 0 && tot > 0) aoff = atomicAdd(&dy.cursors[sub * 16], (unsigned long long)tot);
     uint64_t* st = wbuf[wave];
