@@ -2132,7 +2132,64 @@ static bool adjoint_pair9() {
 }
 constexpr uint32_t kAdjWaveTile = kWaveTile;
 
-static void plan_reverse(Program& P, const std::vector<LGate>& lg, uint32_t n, uint32_t tb, bool wave) {
+// A gradient point reads psi and lambda without changing them, and a gate
+// that commutes with the point's Pauli string P leaves <lambda|P|psi> as it is
+// when applied to both states, so a point may move back over such gates and
+// over other points.  Each point can sit anywhere from the first earlier gate
+// it does not commute with up to its own place; the points are gathered at as
+// few places as possible (greedy stabbing of those intervals, taken by their
+// right ends), so that runs of adjacent points share one exchange of the psi
+// tile in the paired kernel.  Commutation uses the scheduler's mask test.
+// QOMPSIM_ADJ_CLUSTER=0 keeps the order [GRAD(i), inverse(g_i)].
+static std::vector<LGate> cluster_gradient_points(const std::vector<LGate>& lg) {
+  const char* e = std::getenv("QOMPSIM_ADJ_CLUSTER");
+  if (e && *e == '0') return lg;
+  const size_t n = lg.size();
+  auto qmask = [](const std::vector<uint32_t>& v) {
+    uint64_t m = 0;
+    for (auto q : v) m |= 1ull << q;
+    return m;
+  };
+  std::vector<uint64_t> ndm(n), allm(n);
+  for (size_t i = 0; i < n; ++i) {
+    ndm[i] = lg[i].diag ? 0ull : qmask(lg[i].N);
+    allm[i] = lg[i].diag ? qmask(lg[i].Q) : (qmask(lg[i].N) | qmask(lg[i].S));
+  }
+  std::vector<size_t> left(n, 0), points;
+  for (size_t i = 0; i < n; ++i) {
+    if (lg[i].grad < 0) continue;
+    points.push_back(i);
+    size_t j = i;
+    while (j > 0) {
+      const size_t k = j - 1;
+      if (lg[k].grad < 0 && ((ndm[k] & allm[i]) || (allm[k] & ndm[i]))) break;
+      j = k;
+    }
+    left[i] = j;
+  }
+  // anchor[i]: the place point i moves to (the place of its cluster's first point)
+  std::vector<size_t> anchor(n, SIZE_MAX);
+  for (size_t a = 0; a < points.size(); ++a) {
+    const size_t p = points[a];
+    if (anchor[p] != SIZE_MAX) continue;
+    for (size_t b = a; b < points.size(); ++b)
+      if (anchor[points[b]] == SIZE_MAX && left[points[b]] <= p) anchor[points[b]] = p;
+  }
+  std::vector<LGate> out;
+  out.reserve(n);
+  for (size_t i = 0; i < n; ++i) {
+    if (lg[i].grad < 0) {
+      out.push_back(lg[i]);
+    } else if (anchor[i] == i) {
+      for (size_t b : points)
+        if (anchor[b] == i) out.push_back(lg[b]);
+    }
+  }
+  return out;
+}
+
+static void plan_reverse(Program& P, const std::vector<LGate>& lg0, uint32_t n, uint32_t tb, bool wave) {
+  const std::vector<LGate> lg = cluster_gradient_points(lg0);
   const uint32_t wt = adjoint_wave_tile();
   if (wave && n >= wt) plan_program_engine(P, lg, n, wt, 6, false, false, true, wt);
   else plan_program_engine(P, lg, n, tb, std::min<uint32_t>(5, tb), false, false, false);

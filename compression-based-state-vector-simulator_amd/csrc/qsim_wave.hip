@@ -1072,10 +1072,11 @@ __global__ __launch_bounds__(kThreads, 2) void wave_adjoint_kernel(const PassArg
 // Paired form on the forward engine's 1024-amplitude tiles: a workgroup of
 // two waves holds one tile of psi (wave 0) and the same positions of lambda
 // (wave 1), 16 rows each, and both waves run every dense / elementwise op.
-// At a gradient point the psi wave parks its tile in LDS and the lambda wave
-// reads the Pauli partner of each element from there, whatever the flip mask.
-// Two LDS tiles alternate between consecutive points, so one barrier per
-// point keeps the psi wave from overwriting a tile the lambda wave still reads.
+// At the first of a run of adjacent gradient points the psi wave parks its
+// tile in LDS, and for every point of the run the lambda wave reads the Pauli
+// partner of each element from there, whatever the flip mask.  Two LDS tiles
+// alternate between runs, so one barrier per run keeps the psi wave from
+// overwriting a tile the lambda wave still reads.
 template <int KV>
 __global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kernel(const PassArgs* __restrict__ pa,
                                                                    double2* __restrict__ psi,
@@ -1106,6 +1107,7 @@ __global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kerne
 #pragma unroll
   for (int v = 0; v < KV; ++v) amp[v] = make_double2(sv[2 * roff(v)], sv[2 * roff(v) + 1]);
   uint32_t pb = 0;
+  double2* xb = pbuf[0];
   for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
     const QC Op& op = ops[o];
     if (op.kind == OP_DENSE) {
@@ -1113,13 +1115,16 @@ __global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kerne
     } else if (op.kind == OP_ELEM) {
       wave_elem(op, amp, cx, terms, diags, coefs);
     } else {
-      double2* xb = pbuf[pb];
-      pb ^= 1u;
-      if (!is_lam) {
+      // adjacent points see the same psi tile: it is parked once per run
+      if (o == a.op_begin || ops[o - 1].kind != OP_GRAD) {
+        xb = pbuf[pb];
+        pb ^= 1u;
+        if (!is_lam) {
 #pragma unroll
-        for (int v = 0; v < KV; ++v) xb[64 * v + lane] = amp[v];
+          for (int v = 0; v < KV; ++v) xb[64 * v + lane] = amp[v];
+        }
+        __syncthreads();
       }
-      __syncthreads();
       if (is_lam) {
         // Re(i^kr conj(l) x) times the Z/Y sign (-1)^pc(q & zt) of the
         // partner index q: i^kr picks Re(conj(l) x) or Im(conj(l) x) and an
