@@ -733,16 +733,50 @@ static uint32_t compressed_wave_bits() {
   return (e && std::atoi(e) == (int)kWaveTile) ? kWaveTile : kWaveTileSmall;
 }
 
-static void plan_program(Program& P, const std::vector<LGate>& lg, uint32_t L, uint32_t tmax, uint32_t cmin,
+// An exact SWAP: two non-diagonal targets, no selector, the 4x4 permutation.
+static bool is_swap_gate(const LGate& g) {
+  if (g.grad >= 0 || g.diag || g.N.size() != 2 || !g.S.empty() || g.mats.size() != 16) return false;
+  static const int perm[4] = {0, 2, 1, 3};
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c)
+      if (g.mats[r * 4 + c] != cplx(c == perm[r] ? 1.0 : 0.0, 0.0)) return false;
+  return true;
+}
+
+static void plan_program(Program& P, const std::vector<LGate>& lg0, uint32_t L, uint32_t tmax, uint32_t cmin,
                          bool at_least_one_pass, bool codec_ends = false) {
-  const bool wave = wave_engine_ok(lg, L);
-  plan_program_engine(P, lg, L, tmax, cmin, at_least_one_pass, codec_ends, wave);
+  // Trailing SWAP gates on qubits at or above the chunk bits of a compressed
+  // program move whole chunks: they become the last (encoding) pass's chunk
+  // permutation instead of gates (wave engine; QOMPSIM_STORE_SWAPS=0 keeps
+  // them as gates).
+  std::vector<LGate> lg = lg0;
+  std::vector<std::pair<uint32_t, uint32_t>> swaps;
+  const char* es = std::getenv("QOMPSIM_STORE_SWAPS");
+  if (codec_ends && wave_engine_ok(lg0, L) && !(es && *es == '0')) {
+    while (!lg.empty() && swaps.size() < (size_t)kMaxStoreSwaps && is_swap_gate(lg.back()) &&
+           lg.back().N[0] >= (uint32_t)kChunkAmpsLog2 && lg.back().N[1] >= (uint32_t)kChunkAmpsLog2) {
+      swaps.emplace_back(lg.back().N[0], lg.back().N[1]);
+      lg.pop_back();
+    }
+    std::reverse(swaps.begin(), swaps.end());
+  }
+  const bool wave = wave_engine_ok(lg0, L);
+  plan_program_engine(P, lg, L, tmax, cmin, at_least_one_pass || !swaps.empty(), codec_ends, wave);
   if (wave && codec_ends && P.passes.size() == 1 && compressed_wave_bits() == kWaveTileSmall) {
     Program Q;
-    plan_program_engine(Q, lg, L, tmax, cmin, at_least_one_pass, codec_ends, true, kWaveTileSmall);
+    plan_program_engine(Q, lg, L, tmax, cmin, at_least_one_pass || !swaps.empty(), codec_ends, true, kWaveTileSmall);
     if (Q.passes.size() == 1) {
       P.reset();
-      plan_program_engine(P, lg, L, tmax, cmin, at_least_one_pass, codec_ends, true, kWaveTileSmall);
+      plan_program_engine(P, lg, L, tmax, cmin, at_least_one_pass || !swaps.empty(), codec_ends, true,
+                          kWaveTileSmall);
+    }
+  }
+  if (!swaps.empty()) {
+    PassArgs& last = P.passes.back();
+    last.nswap = (uint32_t)swaps.size();
+    for (size_t i = 0; i < swaps.size(); ++i) {
+      last.swp[2 * i] = (uint8_t)swaps[i].first;
+      last.swp[2 * i + 1] = (uint8_t)swaps[i].second;
     }
   }
   if (std::getenv("QOMPSIM_PLAN_LOG")) {
@@ -755,7 +789,7 @@ static void plan_program(Program& P, const std::vector<LGate>& lg, uint32_t L, u
       }
       fprintf(stderr, "[plan]   engine=%s t=%u c=%u E=", a.engine == ENGINE_WAVE ? "wave" : "lds", a.t, a.c);
       for (uint32_t i = 0; i < a.nE; ++i) fprintf(stderr, "%u%s", a.E[i], i + 1 < a.nE ? "," : "");
-      fprintf(stderr, " dense=%u elem=%u terms=%u maxk=%u\n", nd, ne, nt, a.maxk);
+      fprintf(stderr, " dense=%u elem=%u terms=%u maxk=%u swaps=%u\n", nd, ne, nt, a.maxk, a.nswap);
     }
   }
 }
@@ -764,6 +798,7 @@ static void plan_program(Program& P, const std::vector<LGate>& lg, uint32_t L, u
 static hipError_t launch_program_pass(const Program& P, size_t i, int in_mode, int out_mode, const PassDyn& dy,
                                       uint64_t groups, hipStream_t st) {
   const PassArgs& a = P.passes[i];
+  if (a.nswap && (a.engine != ENGINE_WAVE || out_mode != OUT_ENC)) return hipErrorInvalidValue;
   if (a.engine == ENGINE_WAVE) {
     if (a.t != kWaveTile && a.t != kWaveTileSmall) return hipErrorInvalidValue;
     return launch_wave((int)a.t, in_mode, out_mode, P.dargs(i), dy, groups << (P.L - a.t), st);

@@ -25,6 +25,7 @@ constexpr int kMaxSel = 6;         // selector qubits (diagonal targets + contro
 constexpr int kMaxTileBits = 13;   // 2^13 complex128 = 128 KiB of LDS
 constexpr int kThreads = 256;      // work-group size of the pass kernels
 constexpr int kChunkAmpsLog2 = 5;  // 32 amplitudes = 64 doubles per codec chunk
+constexpr int kMaxStoreSwaps = 16;     // SWAP gates folded into one ENC store
 constexpr int kSubArenas = 64;         // allocation granularity of arena capacities
 constexpr int kMaxSubArenas = 65536;   // allocation cursors per arena (the launch uses a power of two)
 constexpr int kMaxLocalBits = 48;
@@ -118,6 +119,11 @@ struct PassArgs {
   uint32_t ovf_hi;         // high word of the smallest |x| whose 2*fma(eps_rel,|x|,eps_abs)
                            // overflows (0x7ff00000 when none is finite)
   double p_abs;            // 2^e_abs
+  // chunk permutation of an ENC store: the chunk at local position x is
+  // written to position s_(n-1)(...s_0(x)), s_i swapping local bits
+  // swp[2i] and swp[2i + 1] (all >= kChunkAmpsLog2): trailing SWAP gates
+  uint32_t nswap;
+  uint8_t swp[2 * kMaxStoreSwaps];
 };
 
 // Per-launch buffers of a pass (passed by value).

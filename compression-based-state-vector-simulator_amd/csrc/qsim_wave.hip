@@ -789,6 +789,21 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
     mci = (uint32_t)((loc & ((1ull << a.B) - 1ull)) >> kChunkAmpsLog2);
     mnull = null_in[mbid];
     if constexpr (IN == IN_DEC) mdesc = mnull ? 0ull : desc_in[(uint64_t)mbid * a.cpb + mci];
+    if constexpr (OUT == OUT_ENC) {
+      // the store's chunk permutation: from here on the metadata is that of
+      // the chunk's output position
+      if (a.nswap) {
+        uint64_t lo = loc;
+        for (uint32_t i = 0; i < a.nswap; ++i) {
+          const uint32_t p0 = a.swp[2 * i], p1 = a.swp[2 * i + 1];
+          const uint64_t d = ((lo >> p0) ^ (lo >> p1)) & 1ull;
+          lo ^= (d << p0) | (d << p1);
+        }
+        mbid = bmap[(slot << a.khi) + (lo >> a.B)];
+        mci = (uint32_t)((lo & ((1ull << a.B) - 1ull)) >> kChunkAmpsLog2);
+        mnull = null_in[mbid];
+      }
+    }
   }
 
   // ---- load
