@@ -380,3 +380,56 @@ def test_commuting_partition_runs_compressed():
     cs2.partition_and_register_circuit(c, 4, reorder=True)
     cs2.apply_circuit()
     assert np.max(np.abs(cs2.get_vector() - exact)) < 1e-9
+
+
+def test_trailing_swaps_folded_into_encode(monkeypatch):
+    """Trailing SWAPs on qubits >= 5 become the encoding pass's chunk
+    permutation: fewer passes, and the same state, null flags and bytes as
+    applying them as gates (QOMPSIM_STORE_SWAPS=0)."""
+    n, B = 18, 10
+    c = W.qft_on_product(n, seed=6)  # ends with n/2 SWAPs, some on qubits 0..4
+    QuantumCircuitOptimizer().optimize(c, 2)
+    res = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("QOMPSIM_STORE_SWAPS", mode)
+        cs = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, verbose=False)
+        cs.partition_and_register_circuit(c, 4)
+        cs.apply_circuit()
+        res[mode] = (cs.get_raw_vector(), cs.get_null_bitmap(), cs.get_bytes_history(), sum(cs.get_stage_passes()))
+    assert np.array_equal(res["1"][0].view(np.uint64), res["0"][0].view(np.uint64))
+    assert np.array_equal(res["1"][1], res["0"][1])
+    assert res["1"][2] == res["0"][2]
+    assert res["1"][3] < res["0"][3]
+    exact = O.apply_circuit(O.zero_state(n), c.gate_list)
+    assert abs(np.vdot(exact, res["1"][0] / np.linalg.norm(res["1"][0]))) > 0.9999
+
+
+def test_swap_only_stage_moves_blocks_and_null_flags():
+    """A stage of SWAPs between a block qubit and an in-block qubit >= 5 on a
+    state with null blocks: the data and the null flags follow the swapped
+    positions."""
+    n, B = 14, 8
+    x = np.zeros(1 << n, dtype=np.complex128)
+    rng = np.random.default_rng(12)
+    x[: 3 << B] = rng.normal(size=3 << B) + 1j * rng.normal(size=3 << B)  # blocks 0..2, rest null
+    x /= np.linalg.norm(x)
+    c = QuantumCircuit(n)
+    c.add_gate(gate.SWAP(6, 9))
+    c.add_gate(gate.SWAP(7, 12))
+    cs = CompressedQuantumState(n, B, eps_rel=1e-12, eps_abs=1e-14, verbose=False)
+    cs.load(x)
+    before = cs.get_raw_vector()
+    cs.partition_and_register_circuit(c, 4)
+    assert cs.get_stage_passes() == [1]
+    cs.apply_circuit()
+    idx = np.arange(1 << n)
+    def swap_bits(v, a, b):
+        d = ((v >> a) ^ (v >> b)) & 1
+        return v ^ ((d << a) | (d << b))
+    perm = swap_bits(swap_bits(idx, 6, 9), 7, 12)
+    expect = np.empty_like(before)
+    expect[perm] = before
+    # one more decode/encode round trip of each chunk: within the codec bound
+    assert np.max(np.abs(cs.get_raw_vector() - expect)) < 1e-12
+    nz = sorted({int(b) for b in (perm[np.flatnonzero(before)] >> B)})
+    assert cs.get_nonnull_blocks() == sorted(set(nz) | {0, 1, 2})
