@@ -788,7 +788,12 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
     mbid = bmap[(slot << a.khi) + (loc >> a.B)];
     mci = (uint32_t)((loc & ((1ull << a.B) - 1ull)) >> kChunkAmpsLog2);
     mnull = null_in[mbid];
-    if constexpr (IN == IN_DEC) mdesc = mnull ? 0ull : desc_in[(uint64_t)mbid * a.cpb + mci];
+    if constexpr (IN == IN_DEC) {
+      // the descriptor table spans every block, so the descriptor load need
+      // not wait for the null flag (a null block's stale entry is discarded)
+      const uint64_t d = desc_in[(uint64_t)mbid * a.cpb + mci];
+      mdesc = mnull ? 0ull : d;
+    }
     if constexpr (OUT == OUT_ENC) {
       // the store's chunk permutation: from here on the metadata is that of
       // the chunk's output position

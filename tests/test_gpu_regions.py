@@ -129,3 +129,17 @@ def test_regions_peak_below_two_states():
     big = max(b.get_bytes_history())
     st = b.get_store_bytes()
     assert st["arena_peak"] < 1.6 * big, (st, big)
+
+
+def test_regions_match_double_store_qft26_packed():
+    """QFT-26 with commutation-aware stage packing (multi-pass stages, folded
+    SWAPs, 1024 blocks): both stores give the same bits."""
+    n, B = 26, 16
+    c = W.qft_on_product(n, seed=8)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    a, b = _pair(n, B)
+    for cs in (a, b):
+        cs.partition_and_register_circuit(c, 4, reorder=True)
+        cs.apply_circuit()
+    _same(a, b)
+    assert max(b.get_stage_passes()) > 1
