@@ -289,6 +289,61 @@ def tfim_bench(n=36, B=22, inner=4, steps=12, n_ref=32, store="regions"):
     }
 
 
+def ghz_bench(n=20, B=12, inner=4, reps=20):
+    """GHZ-20 (H, then a CNOT chain) after QuantumCircuitOptimizer(block_size=2):
+    19 gates, 3 stages of 256 blocks.  Two codec settings: the CPU path's
+    contract (pure absolute bound 1e-7, all-zero threshold 1e-6) and the GPU
+    path's (eps_rel 1e-3, eps_abs 1e-7, exact zeros).  Each is timed over
+    `reps` runs from |0...0>, with the final null bitmap against the expected
+    non-null blocks {0, 15, 127, 255}, the fidelity against the uncompressed
+    GPU run, and the oracle's stage loop timed on the host."""
+    import numpy as np
+
+    from qompsim import CompressedQuantumState, QuantumCircuitOptimizer, QuantumState, _lib
+    from qompsim import workloads as W
+
+    c = W.ghz(n)
+    gates_spec = c.get_gate_count()
+    QuantumCircuitOptimizer().optimize(c, 2)
+    sv = QuantumState(n)
+    c.update_quantum_state(sv)
+    expect = [0, 15, 127, 255]
+    out = {"workload": f"GHZ-{n} (B={B}, I={inner}), {gates_spec} gates, {c.get_gate_count()} after optimize",
+           "gates": gates_spec}
+    for name, kw in (("cpu_contract", dict(eps_rel=0.0, eps_abs=1e-7, zero_mode="threshold", zero_threshold=1e-6)),
+                     ("gpu_contract", dict(eps_rel=1e-3, eps_abs=1e-7, zero_mode="exact"))):
+        cs = CompressedQuantumState(n, B, verbose=False, **kw)
+        cs.partition_and_register_circuit(c, inner)
+        cs.apply_circuit()
+        times = []
+        for _ in range(reps):
+            cs.set_zero_state()
+            _lib.call("qsim_synchronize")
+            t0 = time.perf_counter()
+            cs.apply_circuit()
+            _lib.call("qsim_synchronize")
+            times.append(time.perf_counter() - t0)
+        t = float(np.median(times))
+        rec = {"stages": cs.get_num_stages(), "ms": t * 1e3, "gates_per_s": gates_spec / t,
+               "nonnull_blocks": cs.get_nonnull_blocks(), "bitmap_ok": cs.get_nonnull_blocks() == expect,
+               "fidelity": cs.fidelity_with(sv)}
+        try:
+            import pyoracle as O
+
+            oc = O.OracleCompressedState(n, B, kw["eps_rel"], kw["eps_abs"], 1, kw["zero_mode"],
+                                         kw.get("zero_threshold", 1e-6))
+            oc.partition_and_register_circuit(c, inner)
+            t0 = time.perf_counter()
+            oc.apply_circuit()
+            rec["oracle_host_ms"] = (time.perf_counter() - t0) * 1e3
+            rec["oracle_bitmap_equal"] = bool(np.array_equal(cs.get_null_bitmap(), oc.null_bitmap()))
+        except ImportError:
+            pass
+        out[name] = rec
+        log(f"ghz-{n} {name}: {t * 1e3:.2f} ms, blocks {rec['nonnull_blocks']}, fidelity {rec['fidelity']:.10f}")
+    return out
+
+
 def qaoa_batch_bench(count=512, p=4, reps=3):
     """512 QAOA MaxCut instances on seeded random 3-regular graphs, n
     round-robin over 12..24, p = 4; energies and the 8-parameter adjoint
@@ -361,7 +416,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-reorder", action="store_true",
                     help="skip the extra QFT run with commutation-aware stage packing")
-    ap.add_argument("--workload", choices=["qft32", "tfim36", "hea28-adjoint", "qaoa-batch"], default="qft32",
+    ap.add_argument("--workload", choices=["qft32", "ghz20", "tfim36", "hea28-adjoint", "qaoa-batch"],
+                    default="qft32",
                     help="qft32 is the headline bench line; the others print their parity/measurement record")
     ap.add_argument("--cpu-qubits", type=int, default=25)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -372,6 +428,9 @@ def main():
               "MI355X; the multi-GPU exchange is design-only)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "ghz20":
+        print(json.dumps({"workload_record": "ghz20", **ghz_bench()}))
+        return
     if args.workload == "tfim36":
         print(json.dumps({"workload_record": "tfim36", **tfim_bench(n=args.qubits if args.qubits != 32 else 36)}))
         return
