@@ -143,3 +143,19 @@ def test_regions_match_double_store_qft26_packed():
         cs.apply_circuit()
     _same(a, b)
     assert max(b.get_stage_passes()) > 1
+
+
+@pytest.mark.parametrize("n,B", [(10, 10), (12, 9), (13, 10)])
+def test_regions_few_block_qubits(n, B):
+    """Fewer block qubits than key bits (down to a single block, one region):
+    the same bits as the double-buffered store."""
+    c = random_circuit(n, 80, np.random.default_rng(n * 31 + B), max_k=2)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    a, b = _pair(n, B)
+    for cs in (a, b):
+        cs.partition_and_register_circuit(c, 2)
+        cs.apply_circuit()
+    _same(a, b)
+    b.set_computational_basis(5)
+    a.set_computational_basis(5)
+    _same(a, b)
