@@ -1617,8 +1617,16 @@ static void cs_load_regions(qsim_cs* cs, const double* host, const double2* dev,
   for (size_t i = 0; i < ids.size(); ++i) sr[i] = key_value(cs, ids[i], key);
   upload(cs->slot_region, sr.data(), sr.size() * sizeof(uint32_t));
   upload(cs->bmap, ids.data(), ids.size() * sizeof(uint32_t));
-  // first guess: a quarter of the raw size of a full load, a little for a basis state
-  std::vector<uint64_t> est(K, whole ? ((cs->dim * 2) >> key.size()) / 4 : 64);
+  // first guess: a quarter of the raw size of a full load (at most a quarter
+  // of the free device memory in all), a little for a basis state; a retry
+  // maps the measured demand
+  uint64_t guess = 64;
+  if (whole) {
+    size_t free_b = 0, total_b = 0;
+    HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+    guess = std::min<uint64_t>(((cs->dim * 2) >> key.size()) / 4, (uint64_t)free_b / 4 / 8 / K);
+  }
+  std::vector<uint64_t> est(K, guess);
   rs_encode(cs, out, est, [&] {
     PassDyn dy = cs_dyn(cs, out);
     dy.slot_region = cs->slot_region.as<uint32_t>();
