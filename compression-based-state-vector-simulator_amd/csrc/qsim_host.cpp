@@ -965,6 +965,25 @@ struct qsim_cs {
 };
 
 namespace qsim {
+// Device bytes the store holds now: descriptor tables, staging buffer and
+// arenas (region store: mapped units, the offset-0 unit included).
+static uint64_t cs_held_bytes(const qsim_cs* cs) {
+  uint64_t b = cs->desc[0].bytes + cs->desc[1].bytes + cs->scratch.bytes;
+  if (cs->regions) b += cs->rs.mapped_bytes + (cs->rs.zero_mapped ? cs->rs.unit : 0);
+  else b += cs->arena[0].bytes + cs->arena[1].bytes;
+  return b;
+}
+
+// Throws when holding `extra` more bytes (and releasing `freed`) would pass
+// the configured budget.
+static void cs_budget_check(const qsim_cs* cs, uint64_t extra, uint64_t freed, const char* what) {
+  if (!cs->cfg.hbm_budget) return;
+  const uint64_t after = cs_held_bytes(cs) + extra - std::min(freed, cs_held_bytes(cs));
+  if (after > cs->cfg.hbm_budget)
+    throw std::runtime_error(std::string("HBM budget of ") + std::to_string(cs->cfg.hbm_budget) +
+                             " bytes exceeded (" + what + " would hold " + std::to_string(after) + ")");
+}
+
 static uint64_t words_for_cap(uint64_t bytes) {
   uint64_t w = (bytes + 7) / 8;
   w = (w + kSubArenas - 1) / kSubArenas * kSubArenas;
@@ -974,6 +993,7 @@ static uint64_t words_for_cap(uint64_t bytes) {
 // The decoder reads whole dwords around a code field and may touch up to 8
 // bytes past a chunk's payload, so every arena keeps 64 bytes of padding.
 static void cs_alloc_arena(qsim_cs* cs, int which, uint64_t words) {
+  cs_budget_check(cs, (words + 8) * 8, cs->arena[which].bytes, "arena growth");
   cs->arena[which].alloc((words + 8) * 8);
   cs->cap_words[which] = words;
 }
@@ -1049,7 +1069,9 @@ static bool cs_check_overflow(qsim_cs* cs, int out) {
     HIPCHECK(hipMemGetInfo(&free_b, &total_b));
     const uint64_t held = cs->cap_words[out] * 8;
     const uint64_t want = words_for_cap(equal_words * 8) * 8;
-    if (want <= held + free_b / 2) cs_alloc_arena(cs, out, want / 8);
+    const bool in_budget =
+        !cs->cfg.hbm_budget || cs_held_bytes(cs) - cs->arena[out].bytes + want + 64 <= cs->cfg.hbm_budget;
+    if (want <= held + free_b / 2 && in_budget) cs_alloc_arena(cs, out, want / 8);
     else if (hb[ns] > cs->cap_words[out]) cs_alloc_arena(cs, out, words_for_cap(hb[ns] * 8));
   }
   const uint64_t slack = cs->cap_words[out] - hb[ns];
@@ -1273,6 +1295,7 @@ static void build_stage_program(qsim_cs* cs, CStage& S) {
   S.prog->upload_all();
   const uint64_t need = (1ull << L);
   if (S.prog->passes.size() > 1 && cs->scratch_amps < need) {
+    cs_budget_check(cs, need * 16, cs->scratch.bytes, "staging buffer growth");
     cs->scratch.alloc(need * 16);
     cs->scratch_amps = need;
   }
@@ -1478,6 +1501,7 @@ static void rs_map(qsim_cs* cs, int h, uint32_t r, uint64_t words) {
   const auto t0 = std::chrono::steady_clock::now();
   while (u.size() * R.unit < need) {
     hipMemGenericAllocationHandle_t hd;
+    cs_budget_check(cs, R.unit, 0, "region mapping");
     if (!R.pool.empty()) {
       hd = R.pool.back();
       R.pool.pop_back();
@@ -2538,22 +2562,6 @@ int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** 
   cs->cpb = 1ull << (B - kChunkAmpsLog2);
   const uint64_t raw = cs->dim * 16;
   cs->regions = c.store_mode == QSIM_STORE_REGIONS;
-  if (cs->regions) rs_init(cs.get());
-  for (int w = 0; w < 2; ++w) {
-    cs->desc[w].alloc(cs->nb * cs->cpb * 8);
-    cs->cursors[w].alloc((size_t)kMaxSubArenas * 16 * 8);
-    if (cs->regions) continue;
-    // initial capacity; an overflowing stage regrows its output arena to the
-    // measured demand (QOMPSIM_ARENA_KIB sets the start, for tests)
-    uint64_t init = std::min<uint64_t>(std::max<uint64_t>(raw / 16, 8ull << 20), 4ull << 30);
-    if (const char* e = std::getenv("QOMPSIM_ARENA_KIB")) init = std::max<uint64_t>(1, std::atoll(e)) << 10;
-    cs_alloc_arena(cs.get(), w, words_for_cap(init));
-  }
-  cs->null_flags.alloc(cs->nb);
-  cs->nonzero.alloc(cs->nb);
-  cs->overflow.alloc(16);
-  cs->cstats.alloc(16);
-  cs->h_null.assign(cs->nb, 1);
   uint64_t sbytes = c.scratch_bytes;
   if (sbytes == 0) {
     const char* e = std::getenv("QOMPSIM_SCRATCH_GIB");
@@ -2561,8 +2569,33 @@ int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** 
   }
   sbytes = std::min<uint64_t>(sbytes, raw);
   sbytes = std::max<uint64_t>(sbytes, (1ull << B) * 16);
+  // the fixed part of the budget: two descriptor tables and the staging buffer
+  const uint64_t fixed = 2 * cs->nb * cs->cpb * 8 + sbytes;
+  if (c.hbm_budget && fixed >= c.hbm_budget)
+    throw std::invalid_argument("hbm_budget " + std::to_string(c.hbm_budget) +
+                                " is below the descriptor tables and staging buffer (" + std::to_string(fixed) +
+                                " bytes)");
   cs->scratch.alloc(sbytes);
   cs->scratch_amps = sbytes / 16;
+  if (cs->regions) rs_init(cs.get());
+  for (int w = 0; w < 2; ++w) {
+    cs->desc[w].alloc(cs->nb * cs->cpb * 8);
+    cs->cursors[w].alloc((size_t)kMaxSubArenas * 16 * 8);
+  }
+  if (!cs->regions) {
+    // initial capacity (within the budget); an overflowing stage regrows its
+    // output arena to the measured demand (QOMPSIM_ARENA_KIB sets the start,
+    // for tests)
+    uint64_t init = std::min<uint64_t>(std::max<uint64_t>(raw / 16, 8ull << 20), 4ull << 30);
+    if (const char* e = std::getenv("QOMPSIM_ARENA_KIB")) init = std::max<uint64_t>(1, std::atoll(e)) << 10;
+    if (c.hbm_budget) init = std::min<uint64_t>(init, (c.hbm_budget - fixed) / 4);
+    for (int w = 0; w < 2; ++w) cs_alloc_arena(cs.get(), w, words_for_cap(std::max<uint64_t>(init, 512)));
+  }
+  cs->null_flags.alloc(cs->nb);
+  cs->nonzero.alloc(cs->nb);
+  cs->overflow.alloc(16);
+  cs->cstats.alloc(16);
+  cs->h_null.assign(cs->nb, 1);
   HIPCHECK(hipEventCreate(&cs->ev0));
   HIPCHECK(hipEventCreate(&cs->ev1));
   // io program: one block, no ops, exact zero detection

@@ -47,6 +47,7 @@ class CsConfig(ctypes.Structure):
         ("scratch_bytes", ctypes.c_uint64),
         ("tile_bits", ctypes.c_uint32),
         ("store_mode", ctypes.c_uint32),
+        ("hbm_budget", ctypes.c_uint64),
     ]
 
 

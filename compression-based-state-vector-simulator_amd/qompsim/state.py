@@ -127,16 +127,20 @@ class CompressedQuantumState:
 
     def __init__(self, qubit_count, block_size, *, eps_rel=None, eps_abs=None, guard_bits=None,
                  zero_mode=None, zero_threshold=None, scratch_bytes=None, tile_bits=None, store="double",
-                 verbose=True):
+                 hbm_budget=None, verbose=True):
         """store: "double" (two whole arenas) or "regions" (payload regions
         mapped on demand and freed as a stage consumes them: about one
         compressed state of peak memory instead of two; no checkpoint
-        files)."""
+        files).  hbm_budget: device bytes the store may hold (descriptors,
+        staging buffer, arenas or mapped regions); growth past it raises
+        RuntimeError."""
         cfg = default_codec_config()
         stores = {"double": _lib.STORE_DOUBLE, "regions": _lib.STORE_REGIONS}
         if store not in stores:
             raise ValueError(f"store must be one of {sorted(stores)}")
         cfg.store_mode = stores[store]
+        if hbm_budget is not None:
+            cfg.hbm_budget = int(hbm_budget)
         if eps_rel is not None:
             cfg.eps_rel = eps_rel
         if eps_abs is not None:

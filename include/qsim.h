@@ -136,6 +136,11 @@ typedef struct qsim_cs_config {
   uint64_t scratch_bytes;  /* HBM staging for multi-pass stages (0 = default) */
   uint32_t tile_bits;      /* LDS tile log2 amplitudes (0 = default) */
   uint32_t store_mode;     /* QSIM_STORE_DOUBLE (default) or QSIM_STORE_REGIONS */
+  uint64_t hbm_budget;     /* device bytes the store may hold: descriptor tables, staging
+                              buffer and arenas (region store: mapped units); 0 = no
+                              limit.  Growth past it fails with -1 (RuntimeError); the
+                              double-buffered store keeps its state, the region store's
+                              state must be reloaded. */
 } qsim_cs_config;
 
 /* Payload store of a compressed state (no reference counterpart: the

@@ -433,3 +433,34 @@ def test_swap_only_stage_moves_blocks_and_null_flags():
     assert np.max(np.abs(cs.get_raw_vector() - expect)) < 1e-12
     nz = sorted({int(b) for b in (perm[np.flatnonzero(before)] >> B)})
     assert cs.get_nonnull_blocks() == sorted(set(nz) | {0, 1, 2})
+
+
+@pytest.mark.parametrize("store", ["double", "regions"])
+def test_hbm_budget(store):
+    """hbm_budget bounds what the store holds: below the fixed descriptor and
+    staging bytes the constructor refuses; a budget the run fits in gives the
+    unbudgeted result; one it does not fit in raises RuntimeError."""
+    from qompsim import _lib
+
+    n, B = 18, 10
+    c = W.qft_on_product(n, seed=9)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    kw = dict(eps_rel=1e-3, eps_abs=1e-7, scratch_bytes=1 << 22, store=store, verbose=False)
+    with pytest.raises(ValueError, match="budget"):
+        CompressedQuantumState(n, B, hbm_budget=1 << 20, **kw)
+    ref = CompressedQuantumState(n, B, **kw)
+    ref.partition_and_register_circuit(c, 4)
+    ref.apply_circuit()
+    big = CompressedQuantumState(n, B, hbm_budget=64 << 20, **kw)
+    big.partition_and_register_circuit(c, 4)
+    big.apply_circuit()
+    assert np.array_equal(big.get_raw_vector().view(np.uint64), ref.get_raw_vector().view(np.uint64))
+    st = big.get_store_bytes()
+    assert st["arena_peak"] + st["descriptors"] + (1 << 22) <= 64 << 20
+    fixed = 2 * (1 << (n - B)) * (1 << (B - 5)) * 8 + (1 << 22)  # descriptor tables + staging
+    with pytest.raises(_lib.QsimError, match="budget"):
+        # 256 KiB of payload room: the region store's first mapping of its
+        # regions, or the double store's arenas during the run, do not fit
+        small = CompressedQuantumState(n, B, hbm_budget=fixed + (256 << 10), **kw)
+        small.partition_and_register_circuit(c, 4)
+        small.apply_circuit()
