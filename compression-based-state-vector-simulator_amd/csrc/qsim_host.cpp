@@ -2266,14 +2266,14 @@ static void plan_reverse(Program& P, const std::vector<LGate>& lg0, uint32_t n, 
 static uint64_t reverse_tiles(const Program& P, size_t p, uint32_t n) { return (1ull << n) >> P.passes[p].t; }
 
 static hipError_t launch_reverse_pass(const Program& P, size_t p, uint32_t n, double2* psi, double2* lam,
-                                      double* partials) {
+                                      double* partials, hipStream_t st) {
   const uint64_t ntiles = reverse_tiles(P, p, n);
   if (P.passes[p].engine == ENGINE_WAVE) {
     if (P.passes[p].t == kWaveTile || adjoint_pair9())
-      return launch_wave_pair_adjoint((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, stream());
-    return launch_wave_adjoint((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, stream());
+      return launch_wave_pair_adjoint((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, st);
+    return launch_wave_adjoint((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, st);
   }
-  return launch_adjoint_pass((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, stream());
+  return launch_adjoint_pass((int)P.passes[p].t, P.dargs(p), psi, lam, partials, ntiles, st);
 }
 
 static uint32_t adjoint_tile_bits(uint32_t n) {
@@ -2371,7 +2371,7 @@ static void reverse_sweep(qsim_sv* sv, double2* psi_w, double2* lam, const std::
   sv->adj_stage.ensure((uint64_t)std::max<uint32_t>(1, maxpts) * kGradReduceChunks * 2 * sizeof(double));
   std::vector<double> hout(2 * std::max<uint32_t>(1, maxpts));
   for (size_t pi = 0; pi < P.passes.size(); ++pi) {
-    HIPCHECK(launch_reverse_pass(P, pi, sv->n, psi_w, lam, sv->adj_part.as<double>()));
+    HIPCHECK(launch_reverse_pass(P, pi, sv->n, psi_w, lam, sv->adj_part.as<double>(), stream()));
     const uint32_t np = (uint32_t)params[pi].size();
     if (np == 0) continue;
     HIPCHECK(launch_grad_reduce(sv->adj_part.as<double>(), reverse_tiles(P, pi, sv->n), np, sv->adj_out.as<double>(),
@@ -3236,13 +3236,29 @@ struct qsim_batch {
   std::map<uint32_t, qsim::TuneChoice> tune;
   uint32_t maxn = 0;
   uint64_t maxparams = 0;
-  qsim::DBuf psi, psiw, lam, part, ptout, stage, acc;
+  // Instances run on a few independent lanes (lane 0 on the library stream,
+  // the others on streams of default priority forked from it and joined back),
+  // each with its own working states, so that small instances overlap.
+  struct Lane {
+    qsim::DBuf psi, psiw, lam, part, ptout, stage;
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<size_t> order;  // instances of this lane
+  };
+  std::vector<std::unique_ptr<Lane>> lanes;
+  hipEvent_t start = nullptr;
+  qsim::DBuf acc;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool ready = false;  // planned and buffers sized
   ~qsim_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (size_t l = 0; l < lanes.size(); ++l) {
+      if (lanes[l]->done) (void)hipEventDestroy(lanes[l]->done);
+      if (l > 0 && lanes[l]->st) (void)hipStreamDestroy(lanes[l]->st);
+    }
+    if (start) (void)hipEventDestroy(start);
   }
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -3256,6 +3272,12 @@ namespace qsim {
 static uint64_t acc_stride(const qsim_batch* b) { return 1 + b->maxparams; }
 
 // size the shared buffers for the planned instances
+static uint32_t batch_lanes() {
+  const char* e = std::getenv("QOMPSIM_BATCH_LANES");
+  const int v = e ? std::atoi(e) : 4;
+  return (uint32_t)std::min(std::max(v, 1), 8);
+}
+
 static void batch_prepare(qsim_batch* b) {
   uint64_t maxdim = 1, maxpart = 1, maxpts = 1;
   for (auto& I : b->inst) {
@@ -3269,48 +3291,80 @@ static void batch_prepare(qsim_batch* b) {
       maxpts = std::max<uint64_t>(maxpts, I->map_cnt[p]);
     }
   }
-  b->psi.ensure(maxdim * 16);
-  b->psiw.ensure(maxdim * 16);
-  b->lam.ensure(maxdim * 16);
-  b->part.ensure(maxpart * 16);
-  b->ptout.ensure(maxpts * 16);
-  b->stage.ensure(std::max<uint64_t>(1, maxpts) * kGradReduceChunks * 16);
+  const uint32_t nl = (uint32_t)std::min<size_t>(batch_lanes(), std::max<size_t>(1, b->inst.size()));
+  while (b->lanes.size() < nl) {
+    auto L = std::make_unique<qsim_batch::Lane>();
+    if (b->lanes.empty()) L->st = stream();
+    else HIPCHECK(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&L->done, hipEventDisableTiming));
+    b->lanes.push_back(std::move(L));
+  }
+  if (!b->start) HIPCHECK(hipEventCreateWithFlags(&b->start, hipEventDisableTiming));
+  // largest instances first, each to the lane with the least work so far
+  std::vector<size_t> idx(b->inst.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return b->inst[x]->n > b->inst[y]->n; });
+  std::vector<double> work(nl, 0.0);
+  for (auto& L : b->lanes) L->order.clear();
+  for (size_t i : idx) {
+    const size_t l = std::min_element(work.begin(), work.begin() + nl) - work.begin();
+    b->lanes[l]->order.push_back(i);
+    work[l] += std::ldexp(1.0, (int)b->inst[i]->n) + 65536.0;  // amplitudes plus a launch-bound floor
+  }
+  for (uint32_t l = 0; l < nl; ++l) {
+    auto& L = *b->lanes[l];
+    L.psi.ensure(maxdim * 16);
+    L.psiw.ensure(maxdim * 16);
+    L.lam.ensure(maxdim * 16);
+    L.part.ensure(maxpart * 16);
+    L.ptout.ensure(maxpts * 16);
+    L.stage.ensure(std::max<uint64_t>(1, maxpts) * kGradReduceChunks * 16);
+  }
   b->acc.ensure(b->inst.size() * acc_stride(b) * 16);
   b->ready = true;
 }
 
 // Enqueue instance i: psi = U|0>, lambda = 2 H psi with <H> into acc[i][0],
 // then the reverse sweep accumulating gradients into acc[i][1 + p].
-static void batch_enqueue(qsim_batch* b, size_t i) {
+static void batch_enqueue(qsim_batch* b, size_t i, qsim_batch::Lane& L) {
   BInst& I = *b->inst[i];
   const uint64_t dim = 1ull << I.n;
-  double2* psi = b->psi.as<double2>();
+  const hipStream_t st = L.st;
+  double2* psi = L.psi.as<double2>();
   double* acc = b->acc.as<double>() + i * acc_stride(b) * 2;
-  HIPCHECK(launch_fill_basis(psi, dim, 0, stream()));
+  HIPCHECK(launch_fill_basis(psi, dim, 0, st));
   for (size_t p = 0; p < I.fwd.passes.size(); ++p) {
     PassDyn dy{};
     dy.raw = reinterpret_cast<double*>(psi);
-    HIPCHECK(launch_program_pass(I.fwd, p, IN_RAW, OUT_RAW, dy, 1, stream()));
+    HIPCHECK(launch_program_pass(I.fwd, p, IN_RAW, OUT_RAW, dy, 1, st));
   }
   const unsigned grid = reduce_grid(dim);
-  HIPCHECK(launch_pauli_sum(psi, b->lam.as<double2>(), dim, I.dG.as<PauliGroup>(), (uint32_t)I.G.size(),
-                            I.dT.as<PauliTermZ>(), b->part.as<double>(), grid, stream()));
-  HIPCHECK(launch_grad_reduce(b->part.as<double>(), grid, 1, acc, nullptr, stream()));
-  HIPCHECK(hipMemcpyAsync(b->psiw.p, psi, dim * 16, hipMemcpyDeviceToDevice, stream()));
+  HIPCHECK(launch_pauli_sum(psi, L.lam.as<double2>(), dim, I.dG.as<PauliGroup>(), (uint32_t)I.G.size(),
+                            I.dT.as<PauliTermZ>(), L.part.as<double>(), grid, st));
+  HIPCHECK(launch_grad_reduce(L.part.as<double>(), grid, 1, acc, nullptr, st));
+  HIPCHECK(hipMemcpyAsync(L.psiw.p, psi, dim * 16, hipMemcpyDeviceToDevice, st));
   for (size_t p = 0; p < I.rev.passes.size(); ++p) {
     const uint64_t ntiles = reverse_tiles(I.rev, p, I.n);
-    HIPCHECK(launch_reverse_pass(I.rev, p, I.n, b->psiw.as<double2>(), b->lam.as<double2>(), b->part.as<double>()));
+    HIPCHECK(launch_reverse_pass(I.rev, p, I.n, L.psiw.as<double2>(), L.lam.as<double2>(), L.part.as<double>(), st));
     const uint32_t np = I.map_cnt[p];
     if (np == 0) continue;
-    HIPCHECK(launch_grad_reduce(b->part.as<double>(), ntiles, np, b->ptout.as<double>(), b->stage.as<double>(), stream()));
-    HIPCHECK(launch_accumulate_points(b->ptout.as<double>(), np, I.dmap.as<int32_t>() + I.map_off[p], acc + 2,
-                                      stream()));
+    HIPCHECK(launch_grad_reduce(L.part.as<double>(), ntiles, np, L.ptout.as<double>(), L.stage.as<double>(), st));
+    HIPCHECK(launch_accumulate_points(L.ptout.as<double>(), np, I.dmap.as<int32_t>() + I.map_off[p], acc + 2, st));
   }
 }
 
+// The whole batch on the library stream: clear the accumulators, fork the
+// other lanes off it, run every lane's instances, join the lanes back.
 static void batch_enqueue_all(qsim_batch* b) {
   HIPCHECK(launch_fill_u64(b->acc.as<unsigned long long>(), b->inst.size() * acc_stride(b) * 2, 0ull, stream()));
-  for (size_t i = 0; i < b->inst.size(); ++i) batch_enqueue(b, i);
+  HIPCHECK(hipEventRecord(b->start, stream()));
+  for (size_t l = 1; l < b->lanes.size(); ++l) HIPCHECK(hipStreamWaitEvent(b->lanes[l]->st, b->start, 0));
+  for (auto& L : b->lanes)
+    for (size_t i : L->order) batch_enqueue(b, i, *L);
+  for (size_t l = 1; l < b->lanes.size(); ++l) {
+    HIPCHECK(hipEventRecord(b->lanes[l]->done, b->lanes[l]->st));
+    HIPCHECK(hipStreamWaitEvent(stream(), b->lanes[l]->done, 0));
+  }
 }
 
 static void batch_read(qsim_batch* b, double* energies, double* grads) {
