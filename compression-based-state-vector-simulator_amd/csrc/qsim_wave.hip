@@ -762,7 +762,27 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   // tiles tile0 + 4 (b + k gridDim) + w); the other forms take one tile per
   // wave.
   constexpr bool kPersistent = IN == IN_DEC && OUT == OUT_ENC;
-  auto tile_body = [&](const uint64_t wt) {
+  // RAW->RAW passes on 1024-amplitude tiles also run persistently and
+  // pipeline their input: the rows of a wave's next tile are copied into the
+  // wave's LDS buffer (unused by this form) by global-to-LDS loads while the
+  // current tile's gates run.
+  constexpr bool kPipeRaw = IN == IN_RAW && OUT == OUT_RAW && KV == 16;
+  static_assert(!kPipeRaw || 2 * XS * sizeof(uint64_t) >= 64 * KV * sizeof(double2), "staging rows must fit");
+  auto stage_rows = [&](const uint64_t wt) {
+    const uint64_t slot = wt >> (a.L - TB);
+    const TileMapConst tm{TB, 6u, (uint32_t)RB, a.E};
+    const uint64_t base = tm.base(wt & ((1ull << (a.L - TB)) - 1ull));
+    const QG double2* src = (const QG double2*)(dy.raw) + (slot << a.L) + base + lane;
+    __attribute__((address_space(3))) double2* dst = (__attribute__((address_space(3))) double2*)(&wbuf[wave][0]);
+#pragma unroll
+    for (int v = 0; v < KV; ++v) {
+      const uint64_t ro = ((v & 1) ? 1ull << a.E[0] : 0ull) | ((v & 2) ? 1ull << a.E[1] : 0ull) |
+                          ((v & 4) ? 1ull << a.E[2] : 0ull) | ((v & 8) ? 1ull << a.E[3] : 0ull);
+      __builtin_amdgcn_global_load_lds((const void*)(src + ro), (__attribute__((address_space(3))) void*)(dst + 64 * v),
+                                       16, 0, 0);
+    }
+  };
+  auto tile_body = [&](const uint64_t wt, auto&& pre) {
   const uint32_t L = a.L;
   const uint64_t slot = wt >> (L - TB);
   const uint64_t tau = wt & ((1ull << (L - TB)) - 1ull);
@@ -812,7 +832,17 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   }
 
   // ---- load
-  if constexpr (IN == IN_RAW) {
+  if constexpr (kPipeRaw) {
+    // this tile's rows were staged by stage_rows; every earlier vector memory
+    // operation of the wave (those loads, the previous tile's stores) done
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const double2* pf = reinterpret_cast<const double2*>(wbuf[wave]);
+#pragma unroll
+    for (int v = 0; v < KV; ++v) amp[v] = pf[64 * v + lane];
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    pre();  // the next tile's rows into the same buffer
+    __builtin_amdgcn_sched_barrier(0);
+  } else if constexpr (IN == IN_RAW) {
     const QG double* src = (const QG double*)(dy.raw) + 2 * ((slot << L) + base + lane);
 #pragma unroll
     for (int v = 0; v < KV; ++v) amp[v] = make_double2(src[2 * roff(v)], src[2 * roff(v) + 1]);
@@ -927,11 +957,18 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   }
   };
   const uint64_t wt0 = dy.tile0 + (uint64_t)blockIdx.x * (kThreads / 64) + wave;
-  if constexpr (kPersistent) {
+  if constexpr (kPipeRaw) {
     const uint64_t stride = (uint64_t)gridDim.x * (kThreads / 64);
-    for (uint64_t wt = wt0; wt < ntiles; wt += stride) tile_body(wt);
+    if (wt0 < ntiles) stage_rows(wt0);
+    for (uint64_t wt = wt0; wt < ntiles; wt += stride)
+      tile_body(wt, [&] {
+        if (wt + stride < ntiles) stage_rows(wt + stride);
+      });
+  } else if constexpr (kPersistent) {
+    const uint64_t stride = (uint64_t)gridDim.x * (kThreads / 64);
+    for (uint64_t wt = wt0; wt < ntiles; wt += stride) tile_body(wt, [] {});
   } else {
-    if (wt0 < ntiles) tile_body(wt0);
+    if (wt0 < ntiles) tile_body(wt0, [] {});
   }
 }
 
@@ -1224,13 +1261,14 @@ hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* p
   return hipGetLastError();
 }
 
-// DEC->ENC: one launch, a persistent grid of at most kWaveGroups work-groups
-// (each wave walks its tiles with a stride of the grid).  Other forms: one
-// tile per wave, launches split at kMaxLaunchBlocks work-groups.
+// DEC->ENC, and RAW->RAW on 1024-amplitude tiles: one launch, a persistent
+// grid of at most kWaveGroups work-groups (each wave walks its tiles with a
+// stride of the grid).  Other forms: one tile per wave, launches split at
+// kMaxLaunchBlocks work-groups.
 template <int IN, int OUT, int KV>
 static hipError_t launch_wave_mode(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
   constexpr uint64_t per_block = kThreads / 64;
-  if constexpr (IN == IN_DEC && OUT == OUT_ENC) {
+  if constexpr ((IN == IN_DEC && OUT == OUT_ENC) || (IN == IN_RAW && OUT == OUT_RAW && KV == 16)) {
     constexpr uint64_t kWaveGroups = 8192;
     const uint64_t groups = std::min<uint64_t>((ntiles + per_block - 1) / per_block, kWaveGroups);
     if (groups == 0) return hipSuccess;
