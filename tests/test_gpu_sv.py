@@ -117,3 +117,19 @@ def test_reductions_match_oracle(rng):
     out = QuantumState(n)
     obs.apply_to_state(None, sa, out)
     assert np.max(np.abs(out.get_vector() - O.apply_pauli_sum(a, obs))) < TOL
+
+
+def test_persistent_raw_passes_many_tiles_per_wave(rng, monkeypatch):
+    """26 qubits: the persistent RAW passes walk several tiles per wave, each
+    tile's rows staged in LDS during the previous tile; the wave engine must
+    agree with the LDS-tile engine (which stages nothing)."""
+    n = 26
+    c = random_circuit(n, 40, rng, max_k=2)
+    psi = rand_state(n, rng)
+    s1, s2 = QuantumState(n), QuantumState(n)
+    s1.load(psi)
+    s2.load(psi)
+    c.update_quantum_state(s1)
+    monkeypatch.setenv("QOMPSIM_ENGINE", "lds")
+    c.update_quantum_state(s2)
+    assert np.max(np.abs(s1.get_vector() - s2.get_vector())) < TOL
