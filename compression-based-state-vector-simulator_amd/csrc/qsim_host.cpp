@@ -485,6 +485,10 @@ struct Program {
 };
 
 static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& lg) {
+  // dense gates with real matrices take the kernels' real-coefficient paths
+  // (QOMPSIM_REAL_DENSE=0 sends them through the complex ones)
+  const char* er = std::getenv("QOMPSIM_REAL_DENSE");
+  const bool real_dense = !(er && er[0] == '0');
   PassArgs a{};
   a.t = pp.t;
   a.c = pp.c;
@@ -606,7 +610,12 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
       op.sel_pos[j] = (uint16_t)(p >= 0 ? p : g.S[j]);
     }
     op.mat_off = (uint32_t)(P.coefs.size() / 2);
-    for (auto& v : g.mats) { P.coefs.push_back(v.real()); P.coefs.push_back(v.imag()); }
+    op.real = real_dense ? 1u : 0u;
+    for (auto& v : g.mats) {
+      P.coefs.push_back(v.real());
+      P.coefs.push_back(v.imag());
+      if (v.imag() != 0.0) op.real = 0;
+    }
     P.ops.push_back(op);
     a.maxk = std::max(a.maxk, op.k);
   }

@@ -66,6 +66,9 @@ struct Op {
   // bits gzb, nY Y factors
   uint32_t gidx, nY;
   uint64_t gfx, gzt, gzb;
+  // OP_DENSE: 1 when every matrix entry has an imaginary part of zero (RY,
+  // H, X, ...); the one-target paths then skip the imaginary products
+  uint32_t real, pad_;
 };
 
 // Pauli-sum application: terms grouped by flip mask; coefficient of a term

@@ -122,6 +122,16 @@ __device__ __forceinline__ double2 mac2(double2 m0, double2 a, double2 m1, doubl
   cmad(re, im, m1.x, m1.y, b.x, b.y);
   return make_double2(re, im);
 }
+// out = m0*a + m1*b for real m0, m1.  The products and sums are those of
+// mac2 without its imaginary-part terms; with a zero imaginary part those
+// terms add a signed zero to a sum that is never -0 (an fma onto +0 or onto a
+// sum of that kind), so both give the same bits.
+__device__ __forceinline__ double2 mac2r(double m0, double2 a, double m1, double2 b) {
+  double re = fma(m0, a.x, 0.0), im = fma(m0, a.y, 0.0);
+  re = fma(m1, b.x, re);
+  im = fma(m1, b.y, im);
+  return make_double2(re, im);
+}
 
 // ------------------------------------------------------------------ codec
 // value of the partner lane (lane ^ 1)
@@ -380,6 +390,18 @@ __device__ __forceinline__ void k1_lane_uniform(const QC Op& op, double2 (&amp)[
     // of a pair, so the matrix entries are wave-uniform
     const double2 m0 = ldc(C, op.mat_off), m1 = ldc(C, op.mat_off + 1), m2 = ldc(C, op.mat_off + 2),
                   m3 = ldc(C, op.mat_off + 3);
+    if (op.real) {
+#pragma unroll
+      for (int v = 0; v < KV; v += 2) {
+        double2 x = amp[v], y = amp[v + 1];
+        swap_half_rows<B>(x, y);
+        double2 x2 = mac2r(m0.x, x, m1.x, y), y2 = mac2r(m2.x, x, m3.x, y);
+        swap_half_rows<B>(x2, y2);
+        amp[v] = x2;
+        amp[v + 1] = y2;
+      }
+      return;
+    }
 #pragma unroll
     for (int v = 0; v < KV; v += 2) {
       double2 x = amp[v], y = amp[v + 1];
@@ -392,6 +414,11 @@ __device__ __forceinline__ void k1_lane_uniform(const QC Op& op, double2 (&amp)[
   } else {
     const uint32_t r = (lane >> B) & 1u;
     const double2 md = ldc(C, op.mat_off + 3u * r), mo = ldc(C, op.mat_off + 1u + r);
+    if (op.real) {
+#pragma unroll
+      for (int v = 0; v < KV; ++v) amp[v] = mac2r(md.x, amp[v], mo.x, lane_xor2<B>(amp[v], lane));
+      return;
+    }
 #pragma unroll
     for (int v = 0; v < KV; ++v) amp[v] = mac2(md, amp[v], mo, lane_xor2<B>(amp[v], lane));
   }
@@ -425,6 +452,16 @@ __device__ __forceinline__ void k1_reg(const QC Op& op, double2 (&amp)[KV], cons
     if (op.ident_mask & 1ull) return;
     const uint64_t mo = op.mat_off;
     const double2 m0 = ldc(C, mo), m1 = ldc(C, mo + 1), m2 = ldc(C, mo + 2), m3 = ldc(C, mo + 3);
+    if (op.real) {
+#pragma unroll
+      for (int v = 0; v < KV; ++v) {
+        if (v & MJ) continue;
+        const double2 a = amp[v], b = amp[v | MJ];
+        amp[v] = mac2r(m0.x, a, m1.x, b);
+        amp[v | MJ] = mac2r(m2.x, a, m3.x, b);
+      }
+      return;
+    }
 #pragma unroll
     for (int v = 0; v < KV; ++v) {
       if (v & MJ) continue;

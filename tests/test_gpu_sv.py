@@ -133,3 +133,35 @@ def test_persistent_raw_passes_many_tiles_per_wave(rng, monkeypatch):
     monkeypatch.setenv("QOMPSIM_ENGINE", "lds")
     c.update_quantum_state(s2)
     assert np.max(np.abs(s1.get_vector() - s2.get_vector())) < TOL
+
+
+def test_real_dense_path_bit_identical(rng, monkeypatch):
+    """Real one-target matrices (RY, H, X, real dense with zero and negative
+    entries) take the wave engine's real-coefficient paths on every lane bit
+    and register slot; they must give the same bits as the complex paths
+    (QOMPSIM_REAL_DENSE=0) and match the oracle."""
+    from qompsim import QuantumCircuit
+    from qompsim import gate
+
+    n = 14
+    c = QuantumCircuit(n)
+    for layer in range(3):
+        for q in range(n):
+            c.add_gate(gate.RY(q, rng.uniform(0, 6.3)))
+            c.add_gate(gate.RZ(q, rng.uniform(0, 6.3)))
+        for q in range(0, n, 3):
+            c.add_gate(gate.H(q))
+            c.add_gate(gate.X((q + 1) % n))
+            c.add_gate(gate.DenseMatrix([(q + 2) % n], np.array([[0.0, -1.0], [1.0, 0.0]])))
+        for q in range(n - 1):
+            c.add_gate(gate.CZ(q, q + 1))
+    psi = rand_state(n, rng)
+    s1, s2 = QuantumState(n), QuantumState(n)
+    s1.load(psi)
+    s2.load(psi)
+    c.update_quantum_state(s1)
+    monkeypatch.setenv("QOMPSIM_REAL_DENSE", "0")
+    c.update_quantum_state(s2)
+    v1, v2 = s1.get_vector(), s2.get_vector()
+    assert np.array_equal(v1.view(np.uint64), v2.view(np.uint64))
+    assert np.max(np.abs(v1 - O.apply_circuit(psi, c.gate_list))) < TOL
