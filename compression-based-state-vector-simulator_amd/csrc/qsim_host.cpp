@@ -485,7 +485,8 @@ struct Program {
 };
 
 static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& lg) {
-  // dense gates with real matrices take the kernels' real-coefficient paths
+  // dense gates with real matrices, or one-target matrices with real diagonal
+  // and imaginary off-diagonal entries, take the kernels' cheaper paths
   // (QOMPSIM_REAL_DENSE=0 sends them through the complex ones)
   const char* er = std::getenv("QOMPSIM_REAL_DENSE");
   const bool real_dense = !(er && er[0] == '0');
@@ -610,12 +611,16 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
       op.sel_pos[j] = (uint16_t)(p >= 0 ? p : g.S[j]);
     }
     op.mat_off = (uint32_t)(P.coefs.size() / 2);
-    op.real = real_dense ? 1u : 0u;
-    for (auto& v : g.mats) {
+    bool all_real = true, rdio = op.k == 1;
+    for (size_t i = 0; i < g.mats.size(); ++i) {
+      const cplx v = g.mats[i];
       P.coefs.push_back(v.real());
       P.coefs.push_back(v.imag());
-      if (v.imag() != 0.0) op.real = 0;
+      if (v.imag() != 0.0) all_real = false;
+      const bool on_diag = (i & 3u) == 0 || (i & 3u) == 3;  // 2x2 entries 0 and 3
+      if (on_diag ? v.imag() != 0.0 : v.real() != 0.0) rdio = false;
     }
+    op.real = !real_dense ? 0u : all_real ? kDenseReal : rdio ? kDenseRealDiagImagOff : 0u;
     P.ops.push_back(op);
     a.maxk = std::max(a.maxk, op.k);
   }

@@ -66,10 +66,16 @@ struct Op {
   // bits gzb, nY Y factors
   uint32_t gidx, nY;
   uint64_t gfx, gzt, gzb;
-  // OP_DENSE: 1 when every matrix entry has an imaginary part of zero (RY,
-  // H, X, ...); the one-target paths then skip the imaginary products
+  // OP_DENSE matrix structure (kDenseReal / kDenseRealDiagImagOff, else 0);
+  // the one-target paths then skip the products with zero parts
   uint32_t real, pad_;
 };
+
+// Dense matrix structures with cheaper one-target paths: every entry real
+// (RY, H, X), or one target with real diagonal and imaginary off-diagonal
+// entries (RX)
+constexpr uint32_t kDenseReal = 1;
+constexpr uint32_t kDenseRealDiagImagOff = 2;
 
 // Pauli-sum application: terms grouped by flip mask; coefficient of a term
 // already includes i^nY and any overall scale

@@ -132,6 +132,20 @@ __device__ __forceinline__ double2 mac2r(double m0, double2 a, double m1, double
   im = fma(m1, b.y, im);
   return make_double2(re, im);
 }
+// out = d*a + (i o)*b and out = (i o)*a + d*b for real d, o, in mac2's order
+// of products (the same bits, by the argument above)
+__device__ __forceinline__ double2 mac2_ri(double d, double2 a, double o, double2 b) {
+  double re = fma(d, a.x, 0.0), im = fma(d, a.y, 0.0);
+  re = fma(-o, b.y, re);
+  im = fma(o, b.x, im);
+  return make_double2(re, im);
+}
+__device__ __forceinline__ double2 mac2_ir(double o, double2 a, double d, double2 b) {
+  double re = fma(-o, a.y, 0.0), im = fma(o, a.x, 0.0);
+  re = fma(d, b.x, re);
+  im = fma(d, b.y, im);
+  return make_double2(re, im);
+}
 
 // ------------------------------------------------------------------ codec
 // value of the partner lane (lane ^ 1)
@@ -390,12 +404,24 @@ __device__ __forceinline__ void k1_lane_uniform(const QC Op& op, double2 (&amp)[
     // of a pair, so the matrix entries are wave-uniform
     const double2 m0 = ldc(C, op.mat_off), m1 = ldc(C, op.mat_off + 1), m2 = ldc(C, op.mat_off + 2),
                   m3 = ldc(C, op.mat_off + 3);
-    if (op.real) {
+    if (op.real == kDenseReal) {
 #pragma unroll
       for (int v = 0; v < KV; v += 2) {
         double2 x = amp[v], y = amp[v + 1];
         swap_half_rows<B>(x, y);
         double2 x2 = mac2r(m0.x, x, m1.x, y), y2 = mac2r(m2.x, x, m3.x, y);
+        swap_half_rows<B>(x2, y2);
+        amp[v] = x2;
+        amp[v + 1] = y2;
+      }
+      return;
+    }
+    if (op.real == kDenseRealDiagImagOff) {
+#pragma unroll
+      for (int v = 0; v < KV; v += 2) {
+        double2 x = amp[v], y = amp[v + 1];
+        swap_half_rows<B>(x, y);
+        double2 x2 = mac2_ri(m0.x, x, m1.y, y), y2 = mac2_ir(m2.y, x, m3.x, y);
         swap_half_rows<B>(x2, y2);
         amp[v] = x2;
         amp[v + 1] = y2;
@@ -414,9 +440,15 @@ __device__ __forceinline__ void k1_lane_uniform(const QC Op& op, double2 (&amp)[
   } else {
     const uint32_t r = (lane >> B) & 1u;
     const double2 md = ldc(C, op.mat_off + 3u * r), mo = ldc(C, op.mat_off + 1u + r);
-    if (op.real) {
+    if (op.real == kDenseReal) {
 #pragma unroll
       for (int v = 0; v < KV; ++v) amp[v] = mac2r(md.x, amp[v], mo.x, lane_xor2<B>(amp[v], lane));
+      return;
+    }
+    if (op.real == kDenseRealDiagImagOff) {
+      // md is the diagonal entry of this lane's row, mo the off-diagonal one
+#pragma unroll
+      for (int v = 0; v < KV; ++v) amp[v] = mac2_ri(md.x, amp[v], mo.y, lane_xor2<B>(amp[v], lane));
       return;
     }
 #pragma unroll
@@ -452,13 +484,23 @@ __device__ __forceinline__ void k1_reg(const QC Op& op, double2 (&amp)[KV], cons
     if (op.ident_mask & 1ull) return;
     const uint64_t mo = op.mat_off;
     const double2 m0 = ldc(C, mo), m1 = ldc(C, mo + 1), m2 = ldc(C, mo + 2), m3 = ldc(C, mo + 3);
-    if (op.real) {
+    if (op.real == kDenseReal) {
 #pragma unroll
       for (int v = 0; v < KV; ++v) {
         if (v & MJ) continue;
         const double2 a = amp[v], b = amp[v | MJ];
         amp[v] = mac2r(m0.x, a, m1.x, b);
         amp[v | MJ] = mac2r(m2.x, a, m3.x, b);
+      }
+      return;
+    }
+    if (op.real == kDenseRealDiagImagOff) {
+#pragma unroll
+      for (int v = 0; v < KV; ++v) {
+        if (v & MJ) continue;
+        const double2 a = amp[v], b = amp[v | MJ];
+        amp[v] = mac2_ri(m0.x, a, m1.y, b);
+        amp[v | MJ] = mac2_ir(m2.y, a, m3.x, b);
       }
       return;
     }

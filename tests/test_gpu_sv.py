@@ -137,7 +137,8 @@ def test_persistent_raw_passes_many_tiles_per_wave(rng, monkeypatch):
 
 def test_real_dense_path_bit_identical(rng, monkeypatch):
     """Real one-target matrices (RY, H, X, real dense with zero and negative
-    entries) take the wave engine's real-coefficient paths on every lane bit
+    entries) and matrices with real diagonal and imaginary off-diagonal
+    entries (RX, i X) take the wave engine's cheaper paths on every lane bit
     and register slot; they must give the same bits as the complex paths
     (QOMPSIM_REAL_DENSE=0) and match the oracle."""
     from qompsim import QuantumCircuit
@@ -153,6 +154,9 @@ def test_real_dense_path_bit_identical(rng, monkeypatch):
             c.add_gate(gate.H(q))
             c.add_gate(gate.X((q + 1) % n))
             c.add_gate(gate.DenseMatrix([(q + 2) % n], np.array([[0.0, -1.0], [1.0, 0.0]])))
+            c.add_gate(gate.RX((q + 1) % n, rng.uniform(0, 6.3)))
+            c.add_gate(gate.DenseMatrix([q], np.array([[0.0, 1j], [1j, 0.0]])))
+            c.add_gate(gate.DenseMatrix([(q + 2) % n], np.array([[0.6, -0.8j], [0.8j, -0.6]])))
         for q in range(n - 1):
             c.add_gate(gate.CZ(q, q + 1))
     psi = rand_state(n, rng)
