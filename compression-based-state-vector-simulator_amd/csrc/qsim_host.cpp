@@ -516,7 +516,50 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
     }
     elem_open = false;
   };
-  for (size_t gi : pp.gates) {
+  // A diagonal gate moves past later dense gates whose non-diagonal targets
+  // avoid its qubits (the two commute), so that the pass's diagonal gates
+  // gather into fewer elementwise ops, each of which evaluates its phase
+  // factors once (RY RZ RY RZ ... on different qubits: one op for all RZ).
+  // A gradient point Re<lambda|iP psi> is read where psi and lambda have been
+  // carried through the same gates, so a diagonal gate D moves past it when D
+  // commutes with P, i.e. avoids P's X/Y qubits.  QOMPSIM_DIAG_GATHER=0 keeps
+  // the order.
+  std::vector<size_t> order;
+  order.reserve(pp.gates.size());
+  {
+    const char* eg = std::getenv("QOMPSIM_DIAG_GATHER");
+    const bool gather = !(eg && eg[0] == '0');
+    std::vector<size_t> run;
+    uint64_t run_q = 0;
+    auto flush = [&]() {
+      order.insert(order.end(), run.begin(), run.end());
+      run.clear();
+      run_q = 0;
+    };
+    for (size_t gi : pp.gates) {
+      const LGate& g = lg[gi];
+      if (!gather) {
+        order.push_back(gi);
+        continue;
+      }
+      if (g.grad >= 0) {
+        if (g.gx & run_q) flush();
+        order.push_back(gi);
+        continue;
+      }
+      if (g.diag) {
+        run.push_back(gi);
+        for (auto q : g.Q) run_q |= 1ull << q;
+        continue;
+      }
+      uint64_t nq = 0;
+      for (auto q : g.N) nq |= 1ull << q;
+      if (nq & run_q) flush();
+      order.push_back(gi);
+    }
+    flush();
+  }
+  for (size_t gi : order) {
     const LGate& g = lg[gi];
     if (g.grad >= 0) {
       close_elem();

@@ -169,3 +169,28 @@ def test_real_dense_path_bit_identical(rng, monkeypatch):
     v1, v2 = s1.get_vector(), s2.get_vector()
     assert np.array_equal(v1.view(np.uint64), v2.view(np.uint64))
     assert np.max(np.abs(v1 - O.apply_circuit(psi, c.gate_list))) < TOL
+
+
+def test_diagonal_gates_gathered_within_passes(rng, monkeypatch):
+    """RY RZ pairs per qubit and a CZ chain (an HEA layer): the diagonal gates
+    of a pass move past the commuting dense gates into one elementwise op.
+    The state must match the original order (QOMPSIM_DIAG_GATHER=0) to
+    rounding and the oracle; the adjoint gradient too."""
+    from qompsim.adjoint import adjoint_gradient
+
+    n = 14
+    c = W.hea(n, layers=3, seed=11)
+    psi = rand_state(n, rng)
+    s1, s2 = QuantumState(n), QuantumState(n)
+    s1.load(psi)
+    s2.load(psi)
+    c.update_quantum_state(s1)
+    obs = W.tfim_observable(n)
+    e1, g1 = adjoint_gradient(c, obs)
+    monkeypatch.setenv("QOMPSIM_DIAG_GATHER", "0")
+    c.update_quantum_state(s2)
+    e2, g2 = adjoint_gradient(c, obs)
+    v1, v2 = s1.get_vector(), s2.get_vector()
+    assert np.max(np.abs(v1 - v2)) < TOL
+    assert np.max(np.abs(v1 - O.apply_circuit(psi, c.gate_list))) < TOL
+    assert abs(e1 - e2) < TOL and np.max(np.abs(np.asarray(g1) - np.asarray(g2))) < TOL
