@@ -773,7 +773,13 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], c
         const double2 f = ((lane & lm) == lv) ? cis_reduced(T.angle) : make_double2(1.0, 0.0);
 #pragma unroll
         for (int v = 0; v < KV; ++v)
-          if (((uint32_t)v & rm) == rv) amp[v] = cmul(amp[v], f);
+          if (((uint32_t)v & rm) == rv) {
+            // the row test is wave-uniform: the fence keeps it a scalar
+            // branch, so only the matching rows (a quarter or less) pay the
+            // multiply instead of every row paying it and a select
+            QSIM_ITER_FENCE();
+            amp[v] = cmul(amp[v], f);
+          }
       }
     }
   }
