@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
+#include <stdlib.h>
 #include "qsim_internal.h"
 
 // Address-space qualifiers: pointers read out of argument structs are
@@ -14,6 +15,24 @@
 #define QC __attribute__((address_space(4)))
 
 namespace qsim {
+
+// ------------------------------------------------------- launch shapes (host)
+// Work-groups per dispatch: launches are split at kMaxLaunchBlocks groups
+// (the dispatch packet's 32-bit work-item count).  QOMPSIM_MAX_LAUNCH_GROUPS
+// lowers the split point, so that multi-dispatch launches can be exercised at
+// small sizes.
+inline uint64_t max_launch_groups() {
+  const char* e = std::getenv("QOMPSIM_MAX_LAUNCH_GROUPS");
+  const long long v = e ? std::atoll(e) : 0;
+  return (v > 0 && (uint64_t)v < kMaxLaunchBlocks) ? (uint64_t)v : kMaxLaunchBlocks;
+}
+// Work-groups of a persistent grid (each wave walks its tiles with the
+// grid's stride); QOMPSIM_PERSISTENT_GROUPS overrides the default.
+inline uint64_t persistent_groups(uint64_t dflt) {
+  const char* e = std::getenv("QOMPSIM_PERSISTENT_GROUPS");
+  const long long v = e ? std::atoll(e) : 0;
+  return (v > 0 && (uint64_t)v < kMaxLaunchBlocks) ? (uint64_t)v : dflt;
+}
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ uint32_t ins0(uint32_t x, uint32_t pos) {

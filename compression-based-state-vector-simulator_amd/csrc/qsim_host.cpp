@@ -2648,7 +2648,7 @@ int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** 
     if (c.hbm_budget) init = std::min<uint64_t>(init, (c.hbm_budget - fixed) / 4);
     for (int w = 0; w < 2; ++w) cs_alloc_arena(cs.get(), w, words_for_cap(std::max<uint64_t>(init, 512)));
   }
-  cs->null_flags.alloc(cs->nb);
+  cs->null_flags.alloc(cs->nb + 8);  // whole dwords are read around a flag
   cs->nonzero.alloc(cs->nb);
   cs->overflow.alloc(16);
   cs->cstats.alloc(16);

@@ -881,7 +881,7 @@ static hipError_t launch_pass_tb(const PassArgs* d_args, const PassDyn& dy, uint
     attr_set = true;
   }
   PassDyn d = dy;
-  const uint64_t maxg = kMaxLaunchBlocks;
+  const uint64_t maxg = max_launch_groups();
   for (uint64_t t0 = 0; t0 < ntiles; t0 += maxg) {
     const uint64_t cnt = (ntiles - t0 < maxg) ? (ntiles - t0) : maxg;
     d.tile0 = dy.tile0 + t0;
@@ -1002,8 +1002,9 @@ static hipError_t launch_adjoint_tb(const PassArgs* d_args, double2* psi, double
     if (e != hipSuccess) return e;
   }
   attr = true;
-  for (uint64_t t0 = 0; t0 < ntiles; t0 += kMaxLaunchBlocks) {
-    const unsigned cnt = (unsigned)std::min<uint64_t>(ntiles - t0, kMaxLaunchBlocks);
+  const uint64_t maxg = max_launch_groups();
+  for (uint64_t t0 = 0; t0 < ntiles; t0 += maxg) {
+    const unsigned cnt = (unsigned)std::min<uint64_t>(ntiles - t0, maxg);
     hipLaunchKernelGGL(adjoint_pass_kernel<TB>, dim3(cnt), dim3(kThreads), shm, st, d_args, psi, lam, partials, ntiles, t0);
   }
   return hipGetLastError();

@@ -177,37 +177,44 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
-// LDS slot of tile position p for the layout transposes: one pad slot per 16
-// positions (slot p + p/16).  Wave-layout accesses (64v + l) become slot
-// 68v + l + l/16 and chunk-layout accesses (KV L + q, q < KV) slot
-// KV L + (KV L)/16 + q: a per-lane base plus an immediate offset, and free of
-// bank conflicts in both directions for KV = 16.
+// LDS slot of tile position p for the layout transposes: slot p + (p >> PS),
+// PS = log2(KV).  Wave-layout accesses (64v + l) become slot RS v + l + (l >>
+// PS) with RS = 64 + 64/KV, chunk-layout accesses (KV L + q, q < KV) slot
+// (KV + 1) L + q: a per-lane base plus an immediate offset, and free of bank
+// conflicts in both directions (the eight lanes of each 128-bit access group
+// touch eight distinct 16-byte bank groups).
 template <int KV>
-constexpr uint32_t x_slots() { return 68u * KV; }
+constexpr uint32_t x_pad_shift() { return KV == 16 ? 4u : 3u; }
+template <int KV>
+constexpr uint32_t x_row_slots() { return 64u + (64u >> x_pad_shift<KV>()); }
+template <int KV>
+constexpr uint32_t x_slots() { return x_row_slots<KV>() * KV; }
 constexpr uint32_t kXSlots = x_slots<kV>();
 
 // wave layout (lane l, row v: tile position 64v + l) -> chunk layout
 // (lane L, row q: tile position KV L + q), through the wave's LDS buffer
 template <int KV>
 __device__ __forceinline__ void to_chunk_layout(double2 (&amp)[KV], double2* xb, uint32_t lane) {
-  double2* wa = xb + lane + (lane >> 4);
+  constexpr uint32_t PS = x_pad_shift<KV>(), RS = x_row_slots<KV>();
+  double2* wa = xb + lane + (lane >> PS);
 #pragma unroll
-  for (int v = 0; v < KV; ++v) wa[68 * v] = amp[v];
+  for (int v = 0; v < KV; ++v) wa[RS * v] = amp[v];
   __builtin_amdgcn_wave_barrier();
-  const double2* ca = xb + KV * lane + ((KV * lane) >> 4);
+  const double2* ca = xb + (KV + 1) * lane;
 #pragma unroll
   for (int q = 0; q < KV; ++q) amp[q] = ca[q];
   __builtin_amdgcn_wave_barrier();
 }
 template <int KV>
 __device__ __forceinline__ void to_wave_layout(double2 (&amp)[KV], double2* xb, uint32_t lane) {
-  double2* ca = xb + KV * lane + ((KV * lane) >> 4);
+  constexpr uint32_t PS = x_pad_shift<KV>(), RS = x_row_slots<KV>();
+  double2* ca = xb + (KV + 1) * lane;
 #pragma unroll
   for (int q = 0; q < KV; ++q) ca[q] = amp[q];
   __builtin_amdgcn_wave_barrier();
-  const double2* wa = xb + lane + (lane >> 4);
+  const double2* wa = xb + lane + (lane >> PS);
 #pragma unroll
-  for (int v = 0; v < KV; ++v) amp[v] = wa[68 * v];
+  for (int v = 0; v < KV; ++v) amp[v] = wa[RS * v];
   __builtin_amdgcn_wave_barrier();
 }
 
@@ -1321,8 +1328,9 @@ __global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kerne
 hipError_t launch_wave_pair_adjoint(int tile_bits, const PassArgs* d_args, double2* psi, double2* lam,
                                     double* partials, uint64_t ntiles, hipStream_t st) {
   if (tile_bits != 10 && tile_bits != 9) return hipErrorInvalidValue;
-  for (uint64_t t0 = 0; t0 < ntiles; t0 += kMaxLaunchBlocks) {
-    const uint64_t cnt = std::min<uint64_t>(ntiles - t0, kMaxLaunchBlocks);
+  const uint64_t maxg = max_launch_groups();
+  for (uint64_t t0 = 0; t0 < ntiles; t0 += maxg) {
+    const uint64_t cnt = std::min<uint64_t>(ntiles - t0, maxg);
     if (tile_bits == 10)
       hipLaunchKernelGGL(wave_pair_adjoint_kernel<16>, dim3((unsigned)cnt), dim3(128), 0, st, d_args, psi, lam,
                          partials, t0, ntiles);
@@ -1338,8 +1346,9 @@ hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* p
                                uint64_t ntiles, hipStream_t st) {
   constexpr uint64_t per_block = kThreads / 64;
   if (tile_bits != 9) return hipErrorInvalidValue;
-  for (uint64_t t0 = 0; t0 < ntiles; t0 += kMaxLaunchBlocks * per_block) {
-    const uint64_t cnt = std::min<uint64_t>(ntiles - t0, kMaxLaunchBlocks * per_block);
+  const uint64_t maxg = max_launch_groups();
+  for (uint64_t t0 = 0; t0 < ntiles; t0 += maxg * per_block) {
+    const uint64_t cnt = std::min<uint64_t>(ntiles - t0, maxg * per_block);
     hipLaunchKernelGGL(wave_adjoint_kernel, dim3((unsigned)((cnt + per_block - 1) / per_block)), dim3(kThreads), 0,
                        st, d_args, psi, lam, partials, t0, ntiles);
   }
@@ -1354,13 +1363,12 @@ template <int IN, int OUT, int KV>
 static hipError_t launch_wave_mode(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
   constexpr uint64_t per_block = kThreads / 64;
   if constexpr ((IN == IN_DEC && OUT == OUT_ENC) || (IN == IN_RAW && OUT == OUT_RAW && KV == 16)) {
-    constexpr uint64_t kWaveGroups = 8192;
-    const uint64_t groups = std::min<uint64_t>((ntiles + per_block - 1) / per_block, kWaveGroups);
+    const uint64_t groups = std::min<uint64_t>((ntiles + per_block - 1) / per_block, persistent_groups(8192));
     if (groups == 0) return hipSuccess;
     hipLaunchKernelGGL((wave_kernel<IN, OUT, KV>), dim3((unsigned)groups), dim3(kThreads), 0, st, d_args, dy,
                        dy.tile0 + ntiles);
   } else {
-    const uint64_t maxb = kMaxLaunchBlocks;
+    const uint64_t maxb = max_launch_groups();
     PassDyn d = dy;
     for (uint64_t t0 = 0; t0 < ntiles; t0 += maxb * per_block) {
       const uint64_t cnt = std::min<uint64_t>(ntiles - t0, maxb * per_block);
