@@ -36,6 +36,9 @@ hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
 hipError_t launch_scale(double2* v, uint64_t n, double s, hipStream_t st);
 hipError_t launch_commit_null(uint8_t* nf, uint8_t* nz, uint64_t nb, hipStream_t st);
 unsigned reduce_grid(uint64_t n);
+hipError_t noalloc_scratch(void** out);
+hipError_t launch_pauli_expect_noalloc(const double2* a, uint64_t n, uint32_t ng, unsigned char* scr, unsigned grid,
+                                       hipStream_t st);
 hipError_t launch_sv_norm(const double2* a, uint64_t n, double* partials, unsigned grid, hipStream_t st);
 hipError_t launch_sv_inner(const double2* a, const double2* b, uint64_t n, double* partials, unsigned grid,
                            hipStream_t st);
@@ -2198,6 +2201,63 @@ int qsim_sv_expectation_pauli(const qsim_sv* sv, uint64_t n_terms, const uint64_
   s->part.accumulate(2, grid, acc);
   *ore = acc[0].value();
   *oim = acc[1].value();
+  API_END
+}
+
+// The same expectation with every device buffer in the library's static
+// scratch array: term tables, block partials and final sums.  The grouping
+// (flip masks ascending, terms in input order within a group) and the
+// reduction order are those of qsim_sv_expectation_pauli, so the results are
+// equal bit for bit.  No heap or device allocation: the tables are built in
+// fixed-size arrays on the stack.
+int qsim_sv_expectation_pauli_noalloc(const qsim_sv* sv, uint64_t n_terms, const uint64_t* xm, const uint64_t* zm,
+                                      const uint32_t* ny, const double* cre, const double* cim, double* ore,
+                                      double* oim) {
+  API_BEGIN
+  require(sv != nullptr && ore != nullptr && oim != nullptr, "null argument");
+  if (n_terms > kNoAllocMaxTerms)
+    throw std::invalid_argument("qsim_sv_expectation_pauli_noalloc: at most " + std::to_string(kNoAllocMaxTerms) +
+                                " terms (got " + std::to_string(n_terms) + ")");
+  if (n_terms == 0) {
+    *ore = *oim = 0.0;
+    return 0;
+  }
+  require(xm != nullptr && zm != nullptr && ny != nullptr && cre != nullptr && cim != nullptr, "null argument");
+  uint32_t order[kNoAllocMaxTerms];
+  for (uint32_t t = 0; t < (uint32_t)n_terms; ++t) order[t] = t;
+  std::sort(order, order + n_terms, [&](uint32_t a, uint32_t b) { return xm[a] != xm[b] ? xm[a] < xm[b] : a < b; });
+  PauliGroup G[kNoAllocMaxTerms];
+  PauliTermZ T[kNoAllocMaxTerms];
+  uint32_t ng = 0;
+  for (uint32_t i = 0; i < (uint32_t)n_terms; ++i) {
+    const uint32_t t = order[i];
+    if (i == 0 || xm[t] != G[ng - 1].flip) {
+      G[ng] = PauliGroup{};
+      G[ng].flip = xm[t];
+      G[ng].t0 = i;
+      ++ng;
+    }
+    ++G[ng - 1].nt;
+    double pr, pi;
+    ipow(ny[t], pr, pi);
+    T[i].z = zm[t];
+    T[i].cr = 1.0 * (cre[t] * pr - cim[t] * pi);
+    T[i].ci = 1.0 * (cre[t] * pi + cim[t] * pr);
+  }
+  void* scr = nullptr;
+  HIPCHECK(noalloc_scratch(&scr));
+  unsigned char* sb = static_cast<unsigned char*>(scr);
+  const unsigned grid = reduce_grid(sv->dim);
+  if (grid > kNoAllocMaxGrid) throw std::logic_error("reduction grid exceeds the scratch partials");
+  HIPCHECK(hipMemcpyAsync(sb, G, (size_t)ng * sizeof(PauliGroup), hipMemcpyHostToDevice, stream()));
+  HIPCHECK(hipMemcpyAsync(sb + kNoAllocTermsOff, T, (size_t)n_terms * sizeof(PauliTermZ), hipMemcpyHostToDevice,
+                          stream()));
+  HIPCHECK(launch_pauli_expect_noalloc(sv->ptr(), sv->dim, ng, sb, grid, stream()));
+  double fin[4];
+  HIPCHECK(hipMemcpyAsync(fin, sb + kNoAllocFinalOff, sizeof(fin), hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  *ore = fin[0] + fin[1];
+  *oim = fin[2] + fin[3];
   API_END
 }
 

@@ -88,6 +88,17 @@ struct PauliTermZ {
   double cr, ci;
 };
 
+// Device scratch of the allocation-free Pauli expectation: the grouped term
+// tables, four doubles of block partials per block of the reduction and the
+// four final doubles, at fixed offsets in one static device array.
+constexpr uint32_t kNoAllocMaxTerms = 1024;
+constexpr uint32_t kNoAllocMaxGrid = 2048;  // the reductions' largest grid
+constexpr uint64_t kNoAllocTermsOff = (uint64_t)kNoAllocMaxTerms * sizeof(PauliGroup);
+constexpr uint64_t kNoAllocPartialsOff = kNoAllocTermsOff + (uint64_t)kNoAllocMaxTerms * sizeof(PauliTermZ);
+constexpr uint64_t kNoAllocFinalOff = kNoAllocPartialsOff + (uint64_t)kNoAllocMaxGrid * 4 * sizeof(double);
+constexpr uint64_t kNoAllocBytes = kNoAllocFinalOff + 4 * sizeof(double);
+constexpr int kNoAllocMaxDevices = 64;
+
 // 32 bytes: kernels read a term with one scalar load.  An elementwise op's
 // term list is padded to a multiple of kTermGroup with terms that never
 // match (bmask 0, bval 1), so terms are read kTermGroup at a time.

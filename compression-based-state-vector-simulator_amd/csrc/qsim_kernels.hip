@@ -14,6 +14,7 @@
 //         one atomic per tile on one of 64 sub-arena cursors.
 // Chunks whose codes would need more than 52 bits are stored verbatim (w=64).
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <stdint.h>
 #include <math.h>
 #include "qsim_internal.h"
@@ -1089,6 +1090,73 @@ hipError_t launch_pauli_expect(const double2* a, uint64_t n, const PauliGroup* g
                                const PauliTermZ* terms, double* partials, unsigned grid, hipStream_t st) {
   hipLaunchKernelGGL(pauli_sum_kernel<false>, dim3(grid), dim3(kThreads), 0, st, a, (double2*)nullptr, n, groups, ng,
                      terms, partials);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------- allocation-free expectation
+// One static device array per device, held until the process ends: the
+// allocation-free Pauli expectation keeps its term tables, partials and
+// final sums here instead of allocating.  In host code `scratch` names a
+// host-side stand-in; kernels receive the device address from
+// noalloc_scratch(), looked up once per device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on the current device (hipGetSymbolAddress on
+// the first call per device, with that device current; kept per device).
+hipError_t noalloc_scratch(void** out) {
+  static std::mutex mu;
+  static void* ptr[kNoAllocMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    ptr[dev] = p;
+  }
+  *out = ptr[dev];
+  return hipSuccess;
+}
+
+// The block partials (s_re, c_re, s_im, c_im per block) merged in block
+// order by one thread, with the operations of the host's compensated merge
+// (two-sum of the running sum and the partial's sum, then its correction),
+// so the result equals the host reduction of the same partials bit for bit.
+__device__ __forceinline__ void dd_merge_seq(double& s, double& c, double s2, double c2) {
+  const double t = s + s2;
+  const double bp = t - s;
+  c += (s - (t - bp)) + (s2 - bp);
+  s = t;
+  c += c2;
+}
+__global__ void merge_partials_kernel(const double* __restrict__ partials, unsigned grid, double* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double s0 = 0.0, c0 = 0.0, s1 = 0.0, c1 = 0.0;
+  for (unsigned b = 0; b < grid; ++b) {
+    dd_merge_seq(s0, c0, partials[4 * b], partials[4 * b + 1]);
+    dd_merge_seq(s1, c1, partials[4 * b + 2], partials[4 * b + 3]);
+  }
+  out[0] = s0;
+  out[1] = c0;
+  out[2] = s1;
+  out[3] = c1;
+}
+
+// <a|H|a> into scratch (its term tables already uploaded): block partials,
+// then the four final doubles at kNoAllocFinalOff.
+hipError_t launch_pauli_expect_noalloc(const double2* a, uint64_t n, uint32_t ng, unsigned char* scr, unsigned grid,
+                                       hipStream_t st) {
+  if (grid == 0 || grid > kNoAllocMaxGrid) return hipErrorInvalidValue;
+  const PauliGroup* groups = reinterpret_cast<const PauliGroup*>(scr);
+  const PauliTermZ* terms = reinterpret_cast<const PauliTermZ*>(scr + kNoAllocTermsOff);
+  double* partials = reinterpret_cast<double*>(scr + kNoAllocPartialsOff);
+  hipLaunchKernelGGL(pauli_sum_kernel<false>, dim3(grid), dim3(kThreads), 0, st, a, (double2*)nullptr, n, groups, ng,
+                     terms, partials);
+  hipLaunchKernelGGL(merge_partials_kernel, dim3(1), dim3(64), 0, st, partials, grid,
+                     reinterpret_cast<double*>(scr + kNoAllocFinalOff));
   return hipGetLastError();
 }
 

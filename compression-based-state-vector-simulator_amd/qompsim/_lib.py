@@ -89,6 +89,7 @@ _SIGS = {
     "qsim_sv_scale": (ctypes.c_int, [_vp, _dbl]),
     "qsim_sv_inner_product": (ctypes.c_int, [_vp, _vp, _P(_dbl), _P(_dbl)]),
     "qsim_sv_expectation_pauli": (ctypes.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _P(_dbl), _P(_dbl)]),
+    "qsim_sv_expectation_pauli_noalloc": (ctypes.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _P(_dbl), _P(_dbl)]),
     "qsim_sv_apply_pauli_sum": (ctypes.c_int, [_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
     "qsim_cs_expectation_pauli": (ctypes.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _P(_dbl), _P(_dbl)]),
     "qsim_batch_create": (ctypes.c_int, [_P(_vp)]),

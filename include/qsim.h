@@ -97,6 +97,17 @@ int qsim_sv_inner_product(const qsim_sv* bra, const qsim_sv* ket, double* re, do
 int qsim_sv_expectation_pauli(const qsim_sv* sv, uint64_t n_terms, const uint64_t* x_masks,
                               const uint64_t* z_masks, const uint32_t* y_counts, const double* coef_re,
                               const double* coef_im, double* out_re, double* out_im);
+/* qsim_sv_expectation_pauli without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * term tables, block partials and final sums live in one static device array
+ * of the library (104 KiB per device, held until the process ends), so at
+ * most 1024 terms are accepted (status -2 beyond).  Results equal
+ * qsim_sv_expectation_pauli's bit for bit.  Every call on a device shares
+ * that array: calls that overlap on one device race. */
+int qsim_sv_expectation_pauli_noalloc(const qsim_sv* sv, uint64_t n_terms, const uint64_t* x_masks,
+                                      const uint64_t* z_masks, const uint32_t* y_counts, const double* coef_re,
+                                      const double* coef_im, double* out_re, double* out_im);
 /* out = sum_t coef_t P_t |psi> (GeneralQuantumOperator::apply_to_state) */
 int qsim_sv_apply_pauli_sum(const qsim_sv* sv, qsim_sv* out, uint64_t n_terms, const uint64_t* x_masks,
                             const uint64_t* z_masks, const uint32_t* y_counts, const double* coef_re,
