@@ -4,21 +4,24 @@ Workload (default): 32-qubit QFT-with-swaps on a seeded RY*RZ product state,
 608 gates, complex128, block_size 20, inner_size 4, eps_rel 1e-3,
 eps_abs 1e-7, after the QuantumCircuitOptimizer(block_size=2) pre-pass.
 A step resets the compressed store to |0...0> and runs every stage
-(decode -> gates in LDS -> zero check -> encode) with the state resident in
-HBM.  value = gates of the specified circuit / step time.
+(decode -> gates -> zero check -> encode) with the state resident in HBM.
+value = gates of the specified circuit / step time.
 
-Roofline: the dominant kernel is the fused DEC->ENC wave kernel, one launch
-per single-pass stage.  Its algorithmic bytes per launch are
-C_in + C_out + 32 * A (compressed store in and out, A = amplitudes of the
-active groups written and read once at complex128), its time the HIP events
-recorded around the launch on the library stream; achieved = sum of bytes /
-sum of times over those launches, against 8 TB/s.  `traffic` is the measured
-HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-summary of this workload (null for other workloads).
+Roofline (HBM bound, 8 TB/s): every stage s moves C_in(s) + C_out(s) + 32 A(s)
+algorithmic bytes (the compressed store read and written once, A(s) the
+amplitudes of its active groups written and read once at complex128);
+achieved = sum over all stages of those bytes / sum over all stages of the
+stage time (HIP events around each stage's launches on the library stream).
+`traffic` is the HBM traffic of the same launches in one step, measured in
+this run by two rocprofv3 --pmc child runs of one step (FETCH_SIZE, then
+WRITE_SIZE), each scaled by what the counter reads for a 4 GiB
+device-to-device copy in the same child run.  The single-pass DEC->ENC
+launches (the dominant kernel) are reported beside it.
 
-Also reported: the compression ratio, the fidelity against the uncompressed
-GPU run of the same circuit, and the CPU oracle's compressed stage loop timed
-on the host.
+Also reported: compression ratio, fidelity against the uncompressed GPU run,
+the reference-restated CPU path (the oracle: compressed stage loop and the
+uncompressed OpenMP sweep) timed on the host, the HEA-28 adjoint gradient and
+the 512-instance QAOA batch.
 """
 import argparse
 import json
@@ -50,51 +53,185 @@ def dominant_launches(stats):
     return [s for s in stats if int(s["passes"]) == 1 and s["amplitudes"] > 0]
 
 
-def read_traffic(n, B, inner):
-    """HBM bytes per DEC->ENC launch measured with rocprofv3 FETCH_SIZE and
-    WRITE_SIZE (scaled by a plain copy of known size) for this workload, from
-    the committed profile summary; None for other workloads."""
-    path = os.path.join(REPO, "profiles", "wave_kernel_dec_enc_qft32_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    kv = {}
-    for line in open(path):
-        if ":" in line and not line.startswith("#"):
-            k, v = line.split(":", 1)
-            kv[k.strip()] = v.strip()
+ROCPROF = "/opt/rocm/bin/rocprofv3"
+
+
+def _pmc_child(counter, out_dir, args):
+    """One rocprofv3 --pmc child run of `bench.py --traffic-probe`; returns
+    {dispatch id: (kernel name, counter KiB)} in dispatch order."""
+    import csv
+    import subprocess
+
+    cmd = ["timeout", "-s", "KILL", "240", ROCPROF, "--pmc", counter, "--kernel-trace", "-d", out_dir, "-o", "tp",
+           "--output-format", "csv", "--", sys.executable, os.path.abspath(__file__), "--traffic-probe",
+           "--qubits", str(args.qubits), "--inner-size", str(args.inner_size)]
+    if args.block_size:
+        cmd += ["--block-size", str(args.block_size)]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"))
+    if r.returncode != 0:
+        raise RuntimeError(f"rocprofv3 {counter} run failed ({r.returncode}): {r.stderr[-400:]}")
+    path = None
+    for root, _, files in os.walk(out_dir):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                path = os.path.join(root, f)
+    if path is None:
+        raise RuntimeError(f"rocprofv3 {counter}: no counter_collection.csv under {out_dir}")
+    res = {}
+    with open(path) as fh:
+        for row in csv.DictReader(fh):
+            if row["Counter_Name"] != counter:
+                continue
+            did = int(row["Dispatch_Id"])
+            name, v = res.get(did, (row["Kernel_Name"], 0.0))
+            res[did] = (name, v + float(row["Counter_Value"]))
+    return dict(sorted(res.items()))
+
+
+def measure_traffic(args):
+    """HBM bytes of one step's stage-loop launches from two rocprofv3 --pmc
+    child runs (FETCH_SIZE, WRITE_SIZE).  The child (--traffic-probe) runs a
+    warmup step, the measured step and a 4 GiB device-to-device copy; each
+    counter is scaled by bytes moved / counter reading of that copy.  The
+    measured step's launches are those between the last basis fill before the
+    copy and the copy."""
+    import shutil
+    import tempfile
+
+    if not os.path.exists(ROCPROF):
+        raise RuntimeError(f"{ROCPROF} not found")
+    calib = float(1 << 32)
+    out = {"calibration_bytes": calib}
+    tmp = tempfile.mkdtemp(prefix="qsim_traffic_", dir="/tmp")
     try:
-        if (int(kv["qubits"]), int(kv["block_size"]), int(kv["inner_size"])) != (n, B, inner):
-            return None
-        return float(kv["traffic_bytes_per_launch"])
-    except (KeyError, ValueError):
-        return None
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            rows = _pmc_child(counter, os.path.join(tmp, counter), args)
+            ids = list(rows)
+            copies = [i for i in ids if "copyBuffer" in rows[i][0]]
+            if not copies:
+                raise RuntimeError("calibration copy not found in the trace")
+            ic = copies[-1]
+            fills = [i for i in ids if i < ic and "fill_basis_kernel" in rows[i][0]]
+            if not fills:
+                raise RuntimeError("step boundary (basis fill) not found in the trace")
+            step = [i for i in ids if fills[-1] < i < ic]
+            scale = calib / (rows[ic][1] * 1024.0)
+            total = sum(rows[i][1] for i in step) * 1024.0 * scale
+            dom = [i for i in step if "wave_kernel<1, 1" in rows[i][0]]
+            out[counter] = {"scale": scale, "step_bytes": total, "launches": len(step),
+                            "dec_enc_launches": len(dom),
+                            "dec_enc_bytes": sum(rows[i][1] for i in dom) * 1024.0 * scale}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    out["step_bytes"] = out["FETCH_SIZE"]["step_bytes"] + out["WRITE_SIZE"]["step_bytes"]
+    n_dom = max(1, out["FETCH_SIZE"]["dec_enc_launches"])
+    out["dec_enc_bytes_per_launch"] = (out["FETCH_SIZE"]["dec_enc_bytes"] + out["WRITE_SIZE"]["dec_enc_bytes"]) / n_dom
+    return out
 
 
-def cpu_baseline(n_cpu, B_cpu, inner, gates_full, stages_full):
-    """The oracle's compressed stage loop (OpenMP C++) on QFT-n_cpu with the
-    same high-qubit layout (n - B = 12), scaled to 32 qubits by 2^(32-n)."""
+def traffic_probe(args):
+    """Child run under rocprofv3 --pmc: one warmup step, one measured step,
+    then a 4 GiB device-to-device copy (counter calibration)."""
+    from qompsim import CompressedQuantumState, QuantumCircuitOptimizer, QuantumState, _lib
+    from qompsim import workloads as W
+
+    n = args.qubits
+    B = args.block_size if args.block_size else n - 12
+    src, dst = QuantumState(28), QuantumState(28)  # the copy's buffers, filled before the steps
+    circ = W.qft_on_product(n, seed=2)
+    QuantumCircuitOptimizer().optimize(circ, 2)
+    cs = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, verbose=False)
+    cs.partition_and_register_circuit(circ, args.inner_size)
+    for _ in range(2):  # each step starts with the basis fill of its block 0
+        cs.set_zero_state()
+        cs.apply_circuit()
+        _lib.call("qsim_synchronize")
+    del cs
+    dst.load(src)
+    _lib.call("qsim_synchronize")
+    del src, dst
+
+
+def host_info():
+    info = {"cpu_count": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemTotal"):
+                info["ram_GiB"] = round(int(line.split()[1]) / 2**20, 1)
+                break
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline(n_cpu, inner, uncompressed_gates=64):
+    """The reference-restated CPU path (the oracle, C++/OpenMP on the host
+    cores), measured, not extrapolated: the compressed stage loop of the full
+    QFT-n_cpu (the headline circuit at n_cpu qubits, block_size n_cpu - 12 so
+    that the 2^12-block layout of the 32-qubit run is kept, the same codec
+    contract; SZ3 substituted by the contract codec), and the uncompressed
+    sweep (Qulacs' one-sweep-per-gate update) over the circuit's first
+    `uncompressed_gates` gates at the same size.  The GPU runs the same
+    QFT-n_cpu for a same-size ratio."""
     import numpy as np  # noqa: F401
 
     import pyoracle as O
-    from qompsim import QuantumCircuitOptimizer
+    from qompsim import CompressedQuantumState, QuantumCircuitOptimizer, _lib
     from qompsim import workloads as W
 
     c = W.qft_on_product(n_cpu, seed=2)
+    gates_spec = c.get_gate_count()
     QuantumCircuitOptimizer().optimize(c, 2)
-    oc = O.OracleCompressedState(n_cpu, B_cpu, 1e-3, 1e-7, 1)
+    B = n_cpu - 12
+    oc = O.OracleCompressedState(n_cpu, B, 1e-3, 1e-7, 1)
     nst = oc.partition_and_register_circuit(c, inner)
     t0 = time.perf_counter()
     oc.apply_circuit()
-    dt = time.perf_counter() - t0
-    t32 = dt * 2.0 ** (32 - n_cpu) * (stages_full / max(1, nst))
+    t_cs = time.perf_counter() - t0
+    del oc
+    k = min(uncompressed_gates, c.get_gate_count())
+    psi = O.zero_state(n_cpu)
+    t0 = time.perf_counter()
+    psi = O.apply_circuit(psi, c.gate_list[:k])
+    t_sv = time.perf_counter() - t0
+    del psi
+    # the GPU on the same circuit and size
+    cs = CompressedQuantumState(n_cpu, B, eps_rel=1e-3, eps_abs=1e-7, verbose=False)
+    cs.partition_and_register_circuit(c, inner)
+    cs.apply_circuit()
+    _lib.call("qsim_synchronize")
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        cs.set_zero_state()
+        cs.apply_circuit()
+    _lib.call("qsim_synchronize")
+    t_gpu = (time.perf_counter() - t0) / reps
+    del cs
     return {
-        "value": gates_full / t32,
+        "value": gates_spec / t_cs,
         "unit": "gates/s",
         "cores": O.lib.orc_num_threads(),
         "kind": "port",
-        "sample": (f"oracle C++/OpenMP compressed stage loop, full QFT-{n_cpu} (block_size {B_cpu}, inner 4, "
-                   f"{nst} stages) in {dt:.2f} s on the host, scaled by 2^{32 - n_cpu} amplitudes and "
-                   f"{stages_full}/{nst} stages to the 32-qubit run"),
+        "label": "reference-restated, SZ3 substituted",
+        "sample": (f"oracle C++/OpenMP compressed stage loop of the full QFT-{n_cpu} ({gates_spec} gates, block_size "
+                   f"{B}, inner {inner}, {nst} stages, eps_rel 1e-3, eps_abs 1e-7) in {t_cs:.2f} s on the host; measured "
+                   f"at {n_cpu} qubits, not scaled"),
+        "qubits": n_cpu,
+        "compressed_s": t_cs,
+        "uncompressed_gates_per_s": k / t_sv,
+        "uncompressed_sample": f"first {k} gates of the optimized QFT-{n_cpu}, one OpenMP sweep per gate, {t_sv:.2f} s",
+        "gpu_same_circuit_gates_per_s": gates_spec / t_gpu,
+        "gpu_over_cpu_same_circuit": t_cs / t_gpu,
+        "host": host_info(),
     }
 
 
@@ -419,10 +556,18 @@ def main():
     ap.add_argument("--workload", choices=["qft32", "ghz20", "tfim36", "hea28-adjoint", "qaoa-batch"],
                     default="qft32",
                     help="qft32 is the headline bench line; the others print their parity/measurement record")
-    ap.add_argument("--cpu-qubits", type=int, default=25)
+    ap.add_argument("--cpu-qubits", type=int, default=28)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-instance QAOA batch record")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc child runs that measure roofline.traffic")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="(internal) the child run profiled by the traffic measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one 4 GiB device-to-device copy (counter calibration for profiling)")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every block group lives on one "
               "MI355X; the multi-GPU exchange is design-only)", file=sys.stderr)
@@ -440,6 +585,19 @@ def main():
     if args.workload == "hea28-adjoint":
         print(json.dumps({"workload_record": "hea28-adjoint", **adjoint_bench(n=args.qubits if args.qubits != 32 else 28)}))
         return
+
+    # HBM traffic of one step, measured by profiled child runs before this
+    # process touches the GPU
+    traffic, traffic_err = None, None
+    if not args.no_traffic:
+        try:
+            t0 = time.perf_counter()
+            traffic = measure_traffic(args)
+            log(f"traffic: {traffic['step_bytes'] / 1e9:.2f} GB per step over "
+                f"{traffic['FETCH_SIZE']['launches']} launches ({time.perf_counter() - t0:.0f} s)")
+        except Exception as err:  # a profiler failure leaves traffic unmeasured
+            traffic_err = str(err)[-300:]
+            log(f"traffic not measured: {traffic_err}")
 
     import numpy as np
 
@@ -481,19 +639,20 @@ def main():
     ms = dt / args.steps * 1e3
     stats = cs.get_stage_stats()
     gpu_s = sum(s["gpu_seconds"] for s in stats)
+    all_bytes = sum(stage_bytes(s) for s in stats)
+    achieved = all_bytes / gpu_s / 1e9 if gpu_s > 0 else 0.0
     dom = dominant_launches(stats)
     dom_s = sum(s["gpu_seconds"] for s in dom)
     dom_bytes = sum(stage_bytes(s) for s in dom)
-    achieved = dom_bytes / dom_s / 1e9 if dom_s > 0 else 0.0
+    dom_gbps = dom_bytes / dom_s / 1e9 if dom_s > 0 else 0.0
     compressed_only = sum(s["bytes_in"] + s["bytes_out"] for s in dom) / dom_s / 1e9 if dom_s > 0 else 0.0
-    all_bytes = sum(stage_bytes(s) for s in stats)
     hist = cs.get_bytes_history()
     raw = 16.0 * (1 << n)
     max_bytes = max(hist[-(n_stages + 1):])
-    traffic = read_traffic(n, B, inner)
-    log(f"{ms:.1f} ms/step, stage kernels {gpu_s * 1e3:.1f} ms; DEC->ENC launches: {len(dom)}, "
-        f"{dom_s * 1e3:.1f} ms, {achieved:.0f} GB/s algorithmic ({compressed_only:.0f} GB/s compressed bytes only); "
-        f"max compressed {max_bytes / 2**30:.2f} GiB ({raw / max_bytes:.2f}x)")
+    log(f"{ms:.1f} ms/step, stage kernels {gpu_s * 1e3:.1f} ms, {achieved:.0f} GB/s algorithmic over all stages; "
+        f"DEC->ENC launches: {len(dom)}, {dom_s * 1e3:.1f} ms, {dom_gbps:.0f} GB/s algorithmic "
+        f"({compressed_only:.0f} GB/s compressed bytes only); max compressed {max_bytes / 2**30:.2f} GiB "
+        f"({raw / max_bytes:.2f}x)")
 
     result = {
         "metric": "gates/s & HBM GB/s (compressed, 32q) + adjoint grad/s vs plain, fidelity>=0.9999",
@@ -515,17 +674,25 @@ def main():
                    "passes": int(sum(passes))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": traffic},
-        "roofline_kernel": "wave_kernel<IN_DEC, OUT_ENC> (one launch per single-pass stage); bytes per launch "
-                           "= C_in + C_out + 32*A; time = HIP events around the launch on the library stream",
-        "dominant_launches": len(dom),
-        "dominant_ms": dom_s * 1e3,
-        "compressed_bytes_gbps": compressed_only,
-        "stage_loop_gbps": all_bytes / gpu_s / 1e9 if gpu_s > 0 else 0.0,
+                     "traffic": traffic["step_bytes"] if traffic else None},
+        "roofline_basis": "all stages of one step: sum of C_in + C_out + 32*A over the stages / sum of the stage "
+                          "times (HIP events on the library stream); traffic = measured HBM bytes (FETCH_SIZE + "
+                          "WRITE_SIZE, copy-calibrated) of the same step's launches",
+        "roofline_dec_enc": {
+            "kernel": "wave_kernel<IN_DEC, OUT_ENC> (one launch per single-pass stage)",
+            "launches": len(dom), "avg_ms": dom_s * 1e3 / max(1, len(dom)),
+            "achieved": dom_gbps, "frac": dom_gbps / HBM_PEAK_GBPS,
+            "compressed_bytes_gbps": compressed_only,
+            "traffic_per_launch": traffic["dec_enc_bytes_per_launch"] if traffic else None,
+        },
         "stage_kernel_ms": gpu_s * 1e3,
         "compression_ratio": raw / max_bytes,
         "max_sv_GiB": max_bytes / 2**30,
     }
+    if traffic:
+        result["traffic_detail"] = traffic
+    elif traffic_err:
+        result["traffic_error"] = traffic_err
 
     if not args.no_fidelity:
         sv = QuantumState(n)
@@ -575,10 +742,18 @@ def main():
             f"({a['adjoint_over_forward']:.2f}x forward, {a['speedup_vs_central_fd']:.0f}x central FD), "
             f"max |g - g_fd| {a['max_abs_grad_error_vs_fd']:.2e}")
 
+    if not args.no_batch:
+        result["batch"] = qaoa_batch_bench(reps=3)
+        b = result["batch"]
+        log(f"QAOA batch: {b['batch_ms']:.0f} ms, {b['instances_per_s']:.0f} instances/s, "
+            f"{b['gradients_per_s']:.0f} gradients/s")
+
     if not args.no_cpu:
         ncpu = min(args.cpu_qubits, n)
-        result["cpu_baseline"] = cpu_baseline(ncpu, ncpu - (n - B), inner, gates_spec, n_stages)
-        log("cpu baseline:", result["cpu_baseline"]["sample"])
+        result["cpu_baseline"] = cpu_baseline(ncpu, inner)
+        cb = result["cpu_baseline"]
+        log(f"cpu baseline: {cb['sample']}; uncompressed {cb['uncompressed_gates_per_s']:.1f} gates/s "
+            f"({cb['uncompressed_sample']}); GPU on the same circuit {cb['gpu_same_circuit_gates_per_s']:.0f} gates/s")
     if args.copy_calibration:
         # a plain device-to-device copy of 2^28 complex128 (4 GiB) for scaling
         # the FETCH_SIZE / WRITE_SIZE counters

@@ -1016,6 +1016,11 @@ struct qsim_cs {
   uint64_t cur_bytes = 0;
   std::vector<std::array<double, 5>> stats;
   double timers[QSIM_TIMER_COUNT] = {0};
+  // events after each launch of a multi-pass stage and the timer each
+  // interval counts under (decomp, gate or comp)
+  std::vector<hipEvent_t> pev;
+  std::vector<int> pkind;
+  size_t npev = 0;
   Program io;  // zero-op program over one block (load / readout)
   Partials part;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1044,6 +1049,46 @@ static void cs_budget_check(const qsim_cs* cs, uint64_t extra, uint64_t freed, c
                              " bytes exceeded (" + what + " would hold " + std::to_string(after) + ")");
 }
 
+// Timers of get_total_times (compressed_state.hpp:52-79).  Stage kernels
+// count under decomp (DEC->RAW passes), gate (RAW->RAW passes and the fused
+// DEC->ENC launch of a single-pass stage, which decodes, applies the gates and
+// encodes in one kernel) and comp (RAW->ENC passes), from HIP events on the
+// library stream; device allocation, release and host<->device copies of the
+// store are wall-clock times.  fused_swap, allgather and hmalloc stay 0 (no
+// ranks, no host block buffers).
+enum : int { T_DECOMP = 0, T_GATE = 1, T_COMP = 2, T_HMALLOC = 5, T_DMALLOC = 6, T_H2D = 7, T_D2H = 8, T_DFREE = 9 };
+struct WallTimer {
+  double& acc;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit WallTimer(double& a) : acc(a) {}
+  ~WallTimer() { acc += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+static int pass_timer(int in_mode, int out_mode) {
+  return in_mode == IN_DEC && out_mode == OUT_RAW ? T_DECOMP : in_mode == IN_RAW && out_mode == OUT_ENC ? T_COMP : T_GATE;
+}
+// mark the end of a launch of timer kind k (after cs->ev0 was recorded)
+static void cs_mark(qsim_cs* cs, int k) {
+  if (cs->npev == cs->pev.size()) {
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    cs->pev.push_back(e);
+    cs->pkind.push_back(0);
+  }
+  HIPCHECK(hipEventRecord(cs->pev[cs->npev], stream()));
+  cs->pkind[cs->npev++] = k;
+}
+// after the stage's last event: every interval from ev0 through the marks
+static void cs_collect_marks(qsim_cs* cs) {
+  hipEvent_t prev = cs->ev0;
+  for (size_t i = 0; i < cs->npev; ++i) {
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, prev, cs->pev[i]));
+    cs->timers[cs->pkind[i]] += ms * 1e-3;
+    prev = cs->pev[i];
+  }
+  cs->npev = 0;
+}
+
 static uint64_t words_for_cap(uint64_t bytes) {
   uint64_t w = (bytes + 7) / 8;
   w = (w + kSubArenas - 1) / kSubArenas * kSubArenas;
@@ -1054,6 +1099,11 @@ static uint64_t words_for_cap(uint64_t bytes) {
 // bytes past a chunk's payload, so every arena keeps 64 bytes of padding.
 static void cs_alloc_arena(qsim_cs* cs, int which, uint64_t words) {
   cs_budget_check(cs, (words + 8) * 8, cs->arena[which].bytes, "arena growth");
+  {
+    WallTimer t(cs->timers[T_DFREE]);
+    cs->arena[which].release();
+  }
+  WallTimer t(cs->timers[T_DMALLOC]);
   cs->arena[which].alloc((words + 8) * 8);
   cs->cap_words[which] = words;
 }
@@ -1256,8 +1306,12 @@ static void cs_load(qsim_cs* cs, const double* host, uint64_t basis) {
       const uint64_t bpb = cs_blocks_per_batch(cs);
       for (uint64_t b0 = 0; b0 < cs->nb; b0 += bpb) {
         const uint64_t cnt = std::min(bpb, cs->nb - b0);
-        HIPCHECK(hipMemcpyAsync(cs->scratch.p, host + 2 * (b0 << cs->B), (cnt << cs->B) * 16, hipMemcpyHostToDevice,
-                                stream()));
+        {
+          WallTimer t(cs->timers[T_H2D]);
+          HIPCHECK(hipMemcpyAsync(cs->scratch.p, host + 2 * (b0 << cs->B), (cnt << cs->B) * 16, hipMemcpyHostToDevice,
+                                  stream()));
+          HIPCHECK(hipStreamSynchronize(stream()));
+        }
         std::vector<uint32_t> ids(cnt);
         std::iota(ids.begin(), ids.end(), (uint32_t)b0);
         cs_encode_scratch(cs, out, ids);
@@ -1356,6 +1410,7 @@ static void build_stage_program(qsim_cs* cs, CStage& S) {
   const uint64_t need = (1ull << L);
   if (S.prog->passes.size() > 1 && cs->scratch_amps < need) {
     cs_budget_check(cs, need * 16, cs->scratch.bytes, "staging buffer growth");
+    WallTimer t(cs->timers[T_DMALLOC]);
     cs->scratch.alloc(need * 16);
     cs->scratch_amps = need;
   }
@@ -1433,10 +1488,12 @@ static void cs_run_stage(qsim_cs* cs, uint32_t si) {
       cs_reset_out(cs, out, pick_nsub(nact << (L - tb)), attempt > 0);
     }
     HIPCHECK(hipEventRecord(cs->ev0, stream()));
+    cs->npev = 0;
     PassDyn dy = cs_dyn(cs, out);
     if (P.passes.size() == 1) {
       dy.bmap = cs->bmap.as<uint32_t>();
       HIPCHECK(launch_program_pass(P, 0, IN_DEC, OUT_ENC, dy, nact, stream()));
+      cs_mark(cs, T_GATE);
     } else {
       const uint64_t gpb = std::max<uint64_t>(1, cs->scratch_amps >> L);
       dy.raw = cs->scratch.as<double>();
@@ -1447,6 +1504,7 @@ static void cs_run_stage(qsim_cs* cs, uint32_t si) {
           const int in = p == 0 ? IN_DEC : IN_RAW;
           const int om = p + 1 == P.passes.size() ? OUT_ENC : OUT_RAW;
           HIPCHECK(launch_program_pass(P, p, in, om, dy, cnt, stream()));
+          cs_mark(cs, pass_timer(in, om));
         }
       }
     }
@@ -1455,11 +1513,11 @@ static void cs_run_stage(qsim_cs* cs, uint32_t si) {
     if (attempt > 8) throw std::runtime_error("compressed store: arena growth did not converge");
   }
   HIPCHECK(hipEventElapsedTime(&ms, cs->ev0, cs->ev1));
+  cs_collect_marks(cs);
   cs_commit(cs, out);
   const uint64_t bytes_out = cs_current_bytes(cs);
   cs->cur_bytes = bytes_out;
   cs->bytes_hist.push_back(bytes_out);
-  cs->timers[1] += ms * 1e-3;
   cs->stats.push_back({ms * 1e-3, (double)bytes_in, (double)bytes_out, (double)(nact << L), (double)P.passes.size()});
 }
 
@@ -1522,7 +1580,9 @@ static void rs_release_region(qsim_cs* cs, int h, uint32_t r, bool wait = true) 
     R.mapped_bytes -= u.size() * R.unit;
     R.n_unmap += u.size();
     u.clear();
-    R.t_unmap += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    R.t_unmap += dt;
+    cs->timers[T_DFREE] += dt;
   }
   R.half[h].used[r] = 0;
   R.half[h].payload[r] = 0;
@@ -1581,7 +1641,9 @@ static void rs_map(qsim_cs* cs, int h, uint32_t r, uint64_t words) {
     R.peak_mapped_bytes = std::max(R.peak_mapped_bytes, R.mapped_bytes);
     ++R.n_map;
   }
-  R.t_map += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  R.t_map += dt;
+  cs->timers[T_DMALLOC] += dt;
 }
 
 static uint32_t key_value(const qsim_cs* cs, uint64_t block, const std::vector<uint32_t>& key) {
@@ -1723,8 +1785,12 @@ static void cs_load_regions(qsim_cs* cs, const double* host, const double2* dev,
       const uint64_t bpb = cs_blocks_per_batch(cs);
       for (uint64_t b0 = 0; b0 < cs->nb; b0 += bpb) {
         const uint64_t cnt = std::min(bpb, cs->nb - b0);
-        HIPCHECK(hipMemcpyAsync(cs->scratch.p, host + 2 * (b0 << cs->B), (cnt << cs->B) * 16, hipMemcpyHostToDevice,
-                                stream()));
+        {
+          WallTimer t(cs->timers[T_H2D]);
+          HIPCHECK(hipMemcpyAsync(cs->scratch.p, host + 2 * (b0 << cs->B), (cnt << cs->B) * 16, hipMemcpyHostToDevice,
+                                  stream()));
+          HIPCHECK(hipStreamSynchronize(stream()));
+        }
         dy.bmap = cs->bmap.as<uint32_t>() + b0;
         dy.slot_region = cs->slot_region.as<uint32_t>() + b0;
         HIPCHECK(launch_program_pass(cs->io, 0, IN_RAW, OUT_ENC, dy, cnt, stream()));
@@ -1835,6 +1901,7 @@ static void cs_run_stage_regions(qsim_cs* cs, uint32_t si) {
   R.t_map = R.t_unmap = R.t_wait = 0.0;
   const auto tw0 = std::chrono::steady_clock::now();
   HIPCHECK(hipEventRecord(cs->ev0, stream()));
+  cs->npev = 0;
   double ratio = 1.25;
   std::vector<uint64_t> est(Kout), dem(Kout);
   // input regions consumed by the batches so far: unmapped while the next
@@ -1860,6 +1927,7 @@ static void cs_run_stage_regions(qsim_cs* cs, uint32_t si) {
           dy.bmap = cs->bmap.as<uint32_t>() + a0 * G;
           dy.slot_region = cs->slot_region.as<uint32_t>() + a0;
           HIPCHECK(launch_program_pass(P, 0, IN_DEC, OUT_ENC, dy, a1 - a0, stream()));
+          cs_mark(cs, T_GATE);
         } else {
           const uint64_t gpb = std::max<uint64_t>(1, cs->scratch_amps >> L);
           dy.raw = cs->scratch.as<double>();
@@ -1871,6 +1939,7 @@ static void cs_run_stage_regions(qsim_cs* cs, uint32_t si) {
               const int im = p == 0 ? IN_DEC : IN_RAW;
               const int om = p + 1 == P.passes.size() ? OUT_ENC : OUT_RAW;
               HIPCHECK(launch_program_pass(P, p, im, om, dy, cnt, stream()));
+              cs_mark(cs, pass_timer(im, om));
             }
           }
         }
@@ -1888,6 +1957,7 @@ static void cs_run_stage_regions(qsim_cs* cs, uint32_t si) {
   float ms = 0.f;
   HIPCHECK(hipEventSynchronize(cs->ev1));
   HIPCHECK(hipEventElapsedTime(&ms, cs->ev0, cs->ev1));
+  cs_collect_marks(cs);
   cs_commit(cs, out);
   const uint64_t bytes_out = rs_current_bytes(cs);
   if (const char* e = std::getenv("QOMPSIM_REGION_LOG"); e && *e == '1') {
@@ -1901,7 +1971,6 @@ static void cs_run_stage_regions(qsim_cs* cs, uint32_t si) {
   }
   cs->cur_bytes = bytes_out;
   cs->bytes_hist.push_back(bytes_out);
-  cs->timers[1] += ms * 1e-3;
   cs->stats.push_back({ms * 1e-3, (double)bytes_in, (double)bytes_out, (double)(nact << L), (double)P.passes.size()});
 }
 
@@ -2730,6 +2799,7 @@ int qsim_cs_destroy(qsim_cs* cs) {
   if (cs) {
     if (cs->ev0) (void)hipEventDestroy(cs->ev0);
     if (cs->ev1) (void)hipEventDestroy(cs->ev1);
+    for (auto e : cs->pev) (void)hipEventDestroy(e);
   }
   delete cs;
   API_END
@@ -2976,6 +3046,8 @@ int qsim_cs_read_vector(qsim_cs* cs, double* host, int normalise) {
     std::vector<uint32_t> ids(cnt);
     std::iota(ids.begin(), ids.end(), (uint32_t)b0);
     cs_decode_to_scratch(cs, ids);
+    HIPCHECK(hipStreamSynchronize(stream()));
+    WallTimer t(cs->timers[T_D2H]);
     HIPCHECK(hipMemcpyAsync(host + 2 * (b0 << cs->B), cs->scratch.p, (cnt << cs->B) * 16, hipMemcpyDeviceToHost,
                             stream()));
     HIPCHECK(hipStreamSynchronize(stream()));
@@ -2997,6 +3069,8 @@ int qsim_cs_read_block(qsim_cs* cs, uint64_t block, double* host) {
   API_BEGIN
   if (block >= cs->nb) throw std::out_of_range("block index out of range");
   cs_decode_to_scratch(cs, {(uint32_t)block});
+  HIPCHECK(hipStreamSynchronize(stream()));
+  WallTimer t(cs->timers[T_D2H]);
   HIPCHECK(hipMemcpyAsync(host, cs->scratch.p, (1ull << cs->B) * 16, hipMemcpyDeviceToHost, stream()));
   HIPCHECK(hipStreamSynchronize(stream()));
   API_END
@@ -3118,15 +3192,32 @@ int qsim_cs_block_bytes(qsim_cs* cs, uint64_t* out) {
 }
 
 // ---- compressed-store checkpoint files (no reference counterpart)
+//
+// A file holds a 128-byte header (store kind, shape, codec settings, the
+// caller's next stage, the compressed size), the store's payload layout (the
+// double-buffered store: its sub-arena table and allocation cursors; the
+// region store: the current half's region key, span and per-region extents),
+// the null flags, the descriptor table and the payload words.  Loading reads
+// and validates everything but the payload before it touches the store,
+// checks that the file holds exactly the payload the layout announces, and
+// only then replaces the store; a failure while the payload is copied resets
+// the store to |0...0>.
+extern "C++" {
 namespace qsim {
 struct CkptHeader {
-  char magic[8];  // "QSIMCKP1"
-  uint32_t version, n, B, nsub;
+  char magic[8];  // "QSIMCKP2"
+  uint32_t version, store, n, B;
+  uint64_t nb, cpb;
   double eps_rel, eps_abs, zero_thr;
   int32_t guard, zero_mode;
-  uint64_t nb, cpb, cap_words, next_stage, cur_bytes;
+  uint64_t next_stage, cur_bytes;
+  uint64_t aux0;  // double store: sub-arena count; region store: the current half
+  uint64_t aux1;  // double store: arena capacity in words; region store: region count
+  uint64_t payload_words;
+  uint64_t key_bits;  // region store: qubits of the region key
+  uint64_t reserved;
 };
-static_assert(sizeof(CkptHeader) == 96, "checkpoint header layout");
+static_assert(sizeof(CkptHeader) == 128, "checkpoint header layout");
 
 // device <-> file through one host staging buffer of at most 256 MiB
 static void dev_to_file(std::FILE* f, const void* dev, uint64_t bytes) {
@@ -3147,81 +3238,234 @@ static void file_to_dev(std::FILE* f, void* dev, uint64_t bytes) {
     HIPCHECK(hipMemcpy((char*)dev + o, h.data(), c, hipMemcpyHostToDevice));
   }
 }
+template <class T>
+static void host_to_file(std::FILE* f, const T* p, uint64_t count) {
+  if (count && std::fwrite(p, sizeof(T), count, f) != count) throw std::runtime_error("checkpoint: write failed");
+}
+template <class T>
+static void file_to_host(std::FILE* f, T* p, uint64_t count) {
+  if (count && std::fread(p, sizeof(T), count, f) != count) throw std::runtime_error("checkpoint: file is truncated");
+}
 struct FileCloser {
   std::FILE* f;
   ~FileCloser() {
     if (f) std::fclose(f);
   }
 };
+static uint64_t file_size(std::FILE* f) {
+  const long pos = std::ftell(f);
+  std::fseek(f, 0, SEEK_END);
+  const long end = std::ftell(f);
+  std::fseek(f, pos, SEEK_SET);
+  if (pos < 0 || end < 0) throw std::runtime_error("checkpoint: cannot size the file");
+  return (uint64_t)end;
+}
+
+// Every chunk of a non-null block must decode inside the payload extents:
+// width at most 64 and words [off, off + w) within some extent (the arena,
+// or one region's mapped words; every extent keeps 8 words of readable
+// padding past its end).
+static void ckpt_check_descriptors(const qsim_cs* cs, const std::vector<uint8_t>& nullf,
+                                   const std::vector<uint64_t>& desc,
+                                   const std::vector<std::pair<uint64_t, uint64_t>>& extents) {
+  for (uint64_t b = 0; b < cs->nb; ++b) {
+    if (nullf[b] > 1) throw std::invalid_argument("checkpoint: corrupt null flag");
+    if (nullf[b]) continue;
+    for (uint64_t c = 0; c < cs->cpb; ++c) {
+      const uint64_t d = desc[b * cs->cpb + c];
+      const uint64_t off = desc_off(d), w = desc_w(d);
+      if (w > 64) throw std::invalid_argument("checkpoint: corrupt descriptor (code width)");
+      bool ok = false;
+      for (const auto& e : extents)
+        if (off >= e.first && off + w <= e.second) {
+          ok = true;
+          break;
+        }
+      if (!ok) throw std::invalid_argument("checkpoint: corrupt descriptor (payload outside the store)");
+    }
+  }
+}
 }  // namespace qsim
+}  // extern "C++"
 
 int qsim_cs_save_checkpoint(qsim_cs* cs, const char* path, uint64_t next_stage) {
   API_BEGIN
-  if (cs->regions) throw std::invalid_argument("checkpoint files need the double-buffered store");
   require(cs != nullptr && path != nullptr, "null argument");
   HIPCHECK(hipStreamSynchronize(stream()));
   const int c = cs->cur;
   CkptHeader h{};
-  std::memcpy(h.magic, "QSIMCKP1", 8);
-  h.version = 1;
+  std::memcpy(h.magic, "QSIMCKP2", 8);
+  h.version = 2;
+  h.store = cs->regions ? QSIM_STORE_REGIONS : QSIM_STORE_DOUBLE;
   h.n = cs->n;
   h.B = cs->B;
-  h.nsub = cs->nsub[c];
+  h.nb = cs->nb;
+  h.cpb = cs->cpb;
   h.eps_rel = cs->cfg.eps_rel;
   h.eps_abs = cs->cfg.eps_abs;
   h.zero_thr = cs->cfg.zero_threshold;
   h.guard = cs->cfg.guard_bits;
   h.zero_mode = cs->cfg.zero_mode;
-  h.nb = cs->nb;
-  h.cpb = cs->cpb;
-  h.cap_words = cs->cap_words[c];
   h.next_stage = next_stage;
   h.cur_bytes = cs->cur_bytes;
-  std::vector<uint64_t> sb = cs->h_subbase[c];
-  sb.resize((size_t)h.nsub + 1, 0);
   FileCloser fc{std::fopen(path, "wb")};
   if (!fc.f) throw std::runtime_error(std::string("checkpoint: cannot open ") + path);
-  if (std::fwrite(&h, sizeof(h), 1, fc.f) != 1) throw std::runtime_error("checkpoint: write failed");
-  dev_to_file(fc.f, cs->null_flags.p, cs->nb);
-  dev_to_file(fc.f, cs->desc[c].p, cs->nb * cs->cpb * 8);
-  if (std::fwrite(sb.data(), 8, sb.size(), fc.f) != sb.size()) throw std::runtime_error("checkpoint: write failed");
-  dev_to_file(fc.f, cs->cursors[c].p, (uint64_t)h.nsub * 16 * 8);
-  dev_to_file(fc.f, cs->arena[c].p, (h.cap_words + 8) * 8);
+  if (!cs->regions) {
+    h.aux0 = cs->nsub[c];
+    h.aux1 = cs->cap_words[c];
+    h.payload_words = cs->cap_words[c] + 8;
+    std::vector<uint64_t> sb = cs->h_subbase[c];
+    sb.resize((size_t)h.aux0 + 1, 0);
+    host_to_file(fc.f, &h, 1);
+    host_to_file(fc.f, sb.data(), sb.size());
+    dev_to_file(fc.f, cs->cursors[c].p, h.aux0 * 16 * 8);
+    dev_to_file(fc.f, cs->null_flags.p, cs->nb);
+    dev_to_file(fc.f, cs->desc[c].p, cs->nb * cs->cpb * 8);
+    dev_to_file(fc.f, cs->arena[c].p, h.payload_words * 8);
+  } else {
+    const auto& H = cs->rs.half[c];
+    const uint32_t K = (uint32_t)H.used.size();
+    h.aux0 = (uint64_t)c;
+    h.aux1 = K;
+    h.key_bits = H.key.size();
+    for (auto u : H.used) h.payload_words += u;
+    uint32_t key[kRegionKeyBits] = {};
+    for (size_t i = 0; i < H.key.size(); ++i) key[i] = H.key[i];
+    const uint64_t span = H.span;
+    host_to_file(fc.f, &h, 1);
+    host_to_file(fc.f, key, kRegionKeyBits);
+    host_to_file(fc.f, &span, 1);
+    host_to_file(fc.f, H.used.data(), K);
+    host_to_file(fc.f, H.payload.data(), K);
+    dev_to_file(fc.f, cs->null_flags.p, cs->nb);
+    dev_to_file(fc.f, cs->desc[c].p, cs->nb * cs->cpb * 8);
+    for (uint32_t r = 0; r < K; ++r) dev_to_file(fc.f, rs_region_ptr(cs, c, r), H.used[r] * 8);
+  }
   if (std::fflush(fc.f) != 0) throw std::runtime_error("checkpoint: write failed");
   API_END
 }
 
 int qsim_cs_load_checkpoint(qsim_cs* cs, const char* path, uint64_t* next_stage) {
   API_BEGIN
-  if (cs->regions) throw std::invalid_argument("checkpoint files need the double-buffered store");
   require(cs != nullptr && path != nullptr, "null argument");
   FileCloser fc{std::fopen(path, "rb")};
   if (!fc.f) throw std::runtime_error(std::string("checkpoint: cannot open ") + path);
+  const uint64_t fsize = file_size(fc.f);
   CkptHeader h{};
-  if (std::fread(&h, sizeof(h), 1, fc.f) != 1) throw std::runtime_error("checkpoint: file is truncated");
-  if (std::memcmp(h.magic, "QSIMCKP1", 8) != 0 || h.version != 1)
+  if (fsize < sizeof(h)) throw std::invalid_argument("checkpoint: not a compressed-store checkpoint (magic)");
+  file_to_host(fc.f, &h, 1);
+  if (std::memcmp(h.magic, "QSIMCKP2", 8) != 0 || h.version != 2)
     throw std::invalid_argument("checkpoint: not a compressed-store checkpoint (magic/version)");
+  const uint32_t kind = cs->regions ? QSIM_STORE_REGIONS : QSIM_STORE_DOUBLE;
+  if (h.store != kind) throw std::invalid_argument("checkpoint: written by the other payload store kind");
   if (h.n != cs->n || h.B != cs->B || h.nb != cs->nb || h.cpb != cs->cpb)
     throw std::invalid_argument("checkpoint: qubit count or block size differs from this store");
   if (h.eps_rel != cs->cfg.eps_rel || h.eps_abs != cs->cfg.eps_abs || h.guard != cs->cfg.guard_bits ||
       h.zero_mode != cs->cfg.zero_mode || h.zero_thr != cs->cfg.zero_threshold)
     throw std::invalid_argument("checkpoint: codec settings differ from this store");
-  if (h.nsub == 0 || h.nsub > (uint32_t)kMaxSubArenas || (h.nsub & (h.nsub - 1)))
-    throw std::invalid_argument("checkpoint: bad sub-arena count");
+  // ---- layout, null flags and descriptors into host memory, validated
+  std::vector<uint64_t> sb, cur;
+  uint32_t key[kRegionKeyBits] = {};
+  uint64_t span = 0;
+  std::vector<uint64_t> used, payload;
+  std::vector<std::pair<uint64_t, uint64_t>> extents;
+  uint64_t layout_bytes = 0;
+  if (kind == QSIM_STORE_DOUBLE) {
+    const uint64_t nsub = h.aux0, cap = h.aux1;
+    if (nsub == 0 || nsub > (uint64_t)kMaxSubArenas || (nsub & (nsub - 1)))
+      throw std::invalid_argument("checkpoint: bad sub-arena count");
+    if (cap == 0 || cap >= (1ull << 40) || h.payload_words != cap + 8)
+      throw std::invalid_argument("checkpoint: bad arena capacity");
+    layout_bytes = (nsub + 1) * 8 + nsub * 16 * 8;
+    if (fsize < sizeof(h) + layout_bytes) throw std::runtime_error("checkpoint: file is truncated");
+    sb.resize(nsub + 1);
+    cur.resize(nsub * 16);
+    file_to_host(fc.f, sb.data(), sb.size());
+    file_to_host(fc.f, cur.data(), cur.size());
+    if (sb[0] != 0 || sb[nsub] > cap) throw std::invalid_argument("checkpoint: sub-arena table exceeds the arena");
+    for (uint64_t i = 0; i < nsub; ++i) {
+      if (sb[i + 1] < sb[i]) throw std::invalid_argument("checkpoint: sub-arena table is not ascending");
+      if (cur[i * 16] > sb[i + 1] - sb[i]) throw std::invalid_argument("checkpoint: allocation cursor past its sub-arena");
+    }
+    extents.emplace_back(0, cap);
+  } else {
+    const uint64_t hsel = h.aux0, K = h.aux1;
+    const uint32_t kb = (uint32_t)h.key_bits;
+    if (hsel > 1 || kb > std::min<uint32_t>(kRegionKeyBits, cs->n - cs->B) || K != (1ull << kb))
+      throw std::invalid_argument("checkpoint: bad region layout");
+    layout_bytes = kRegionKeyBits * 4 + 8 + 2 * K * 8;
+    if (fsize < sizeof(h) + layout_bytes) throw std::runtime_error("checkpoint: file is truncated");
+    file_to_host(fc.f, key, kRegionKeyBits);
+    file_to_host(fc.f, &span, 1);
+    used.resize(K);
+    payload.resize(K);
+    file_to_host(fc.f, used.data(), K);
+    file_to_host(fc.f, payload.data(), K);
+    for (uint32_t i = 0; i < kb; ++i)
+      if (key[i] < cs->B || key[i] >= cs->n || (i > 0 && key[i] <= key[i - 1]))
+        throw std::invalid_argument("checkpoint: bad region key");
+    const uint64_t rr = (cs->dim * 16) >> kb;
+    if (span != (rr + rr / 8 + 2 * cs->rs.unit - 1) / cs->rs.unit * cs->rs.unit)
+      throw std::invalid_argument("checkpoint: region span differs from this store");
+    uint64_t tot = 0;
+    for (uint64_t r = 0; r < K; ++r) {
+      if ((used[r] + 8) * 8 > span || payload[r] > used[r])
+        throw std::invalid_argument("checkpoint: region extent exceeds its span");
+      tot += used[r];
+      const uint64_t w0 = (cs->rs.unit + hsel * cs->rs.half_bytes + r * span) / 8;
+      extents.emplace_back(w0, w0 + used[r]);
+    }
+    if (tot != h.payload_words) throw std::invalid_argument("checkpoint: region extents do not add up");
+    extents.emplace_back(0, 0);  // chunks without payload point at offset 0
+  }
+  if (fsize != sizeof(h) + layout_bytes + cs->nb + cs->nb * cs->cpb * 8 + h.payload_words * 8)
+    throw std::runtime_error("checkpoint: file size does not match its layout (truncated or corrupt)");
+  std::vector<uint8_t> nullf(cs->nb);
+  std::vector<uint64_t> desc(cs->nb * cs->cpb);
+  file_to_host(fc.f, nullf.data(), nullf.size());
+  file_to_host(fc.f, desc.data(), desc.size());
+  ckpt_check_descriptors(cs, nullf, desc, extents);
+  // ---- replace the store
   HIPCHECK(hipStreamSynchronize(stream()));
-  const int c = cs->cur;
-  file_to_dev(fc.f, cs->null_flags.p, cs->nb);
-  file_to_dev(fc.f, cs->desc[c].p, cs->nb * cs->cpb * 8);
-  std::vector<uint64_t> sb((size_t)h.nsub + 1);
-  if (std::fread(sb.data(), 8, sb.size(), fc.f) != sb.size()) throw std::runtime_error("checkpoint: file is truncated");
-  if (sb.back() > h.cap_words) throw std::invalid_argument("checkpoint: sub-arena table exceeds the arena");
-  cs->nsub[c] = h.nsub;
-  cs->h_subbase[c] = sb;
-  cs_upload_subbase(cs, c);
-  file_to_dev(fc.f, cs->cursors[c].p, (uint64_t)h.nsub * 16 * 8);
-  if (cs->cap_words[c] < h.cap_words) cs_alloc_arena(cs, c, h.cap_words);
-  file_to_dev(fc.f, cs->arena[c].p, (h.cap_words + 8) * 8);
-  HIPCHECK(hipMemcpy(cs->h_null.data(), cs->null_flags.p, cs->nb, hipMemcpyDeviceToHost));
+  if (kind == QSIM_STORE_DOUBLE) {
+    const int c = cs->cur;
+    if (cs->cap_words[c] < h.aux1)
+      cs_budget_check(cs, (h.aux1 + 8) * 8, cs->arena[c].bytes, "checkpoint arena");
+  }
+  try {
+    if (kind == QSIM_STORE_DOUBLE) {
+      const int c = cs->cur;
+      if (cs->cap_words[c] < h.aux1) cs_alloc_arena(cs, c, h.aux1);
+      cs->nsub[c] = (uint32_t)h.aux0;
+      cs->h_subbase[c] = sb;
+      cs_upload_subbase(cs, c);
+      HIPCHECK(hipMemcpy(cs->cursors[c].p, cur.data(), cur.size() * 8, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(cs->null_flags.p, nullf.data(), cs->nb, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(cs->desc[c].p, desc.data(), desc.size() * 8, hipMemcpyHostToDevice));
+      file_to_dev(fc.f, cs->arena[c].p, h.payload_words * 8);
+    } else {
+      const int hs = (int)h.aux0;
+      rs_release_half(cs, 0);
+      rs_release_half(cs, 1);
+      std::vector<uint32_t> kv(key, key + h.key_bits);
+      rs_set_layout(cs, hs, kv);
+      auto& H = cs->rs.half[hs];
+      for (uint32_t r = 0; r < used.size(); ++r) {
+        if (used[r]) rs_map(cs, hs, r, used[r]);
+        file_to_dev(fc.f, rs_region_ptr(cs, hs, r), used[r] * 8);
+        H.used[r] = used[r];
+        H.payload[r] = payload[r];
+      }
+      cs->cur = hs;
+      HIPCHECK(hipMemcpy(cs->null_flags.p, nullf.data(), cs->nb, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(cs->desc[hs].p, desc.data(), desc.size() * 8, hipMemcpyHostToDevice));
+    }
+  } catch (const std::exception& e) {
+    std::string msg = std::string("checkpoint: load failed (") + e.what() + "); the store was reset to |0...0>";
+    cs_load(cs, nullptr, 0);
+    throw std::runtime_error(msg);
+  }
+  cs->h_null = nullf;
   cs->cur_bytes = h.cur_bytes;
   if (next_stage) *next_stage = h.next_stage;
   API_END

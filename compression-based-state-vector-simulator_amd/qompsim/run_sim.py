@@ -83,7 +83,7 @@ def run_sim(simulator, circuit_name, nqubits, outdir=None, append=False, block_s
         circuit.update_quantum_state(qs)
         call("qsim_synchronize")
         t2 = time.time()
-        r.times = {"init": t1 - t0, "overall": t2 - t1, "fused_swap": 0.0}
+        r.times = {"init": t1 - t0, "overall": t2 - t1, "fused_swap": qs.get_fused_swap_time()}
         r.sv_MiB = 2 ** (nqubits + 4) / 2 ** 20
         _write(r, out, append)
         results["qulacs"] = r

@@ -78,6 +78,12 @@ class QuantumState:
     def device_ptr(self):
         return lib.qsim_sv_device_ptr(self._h)
 
+    def get_fused_swap_time(self):
+        """Seconds spent in FusedSWAP exchanges between ranks
+        (python/cppsim_wrapper.cpp:412-416).  One GPU holds the whole state,
+        so there is no exchange: always 0.0."""
+        return 0.0
+
     def __repr__(self):
         return f"QuantumState(qubit_count={self._n})"
 
@@ -102,6 +108,17 @@ def inner_product(state_bra, state_ket):
     re, im = ctypes.c_double(), ctypes.c_double()
     call("qsim_sv_inner_product", state_bra._h, state_ket._h, ctypes.byref(re), ctypes.byref(im))
     return complex(re.value, im.value)
+
+
+def _format_size(size):
+    """format_size of compressed_state.hpp:360-372."""
+    if size > 1 << 30:
+        return f"{size / (1 << 30):.3f} GiB"
+    if size > 1 << 20:
+        return f"{size / (1 << 20):.3f} MiB"
+    if size > 1 << 10:
+        return f"{size / (1 << 10):.3f} KiB"
+    return f"{size} bytes"
 
 
 def _env_float(name, default):
@@ -164,6 +181,7 @@ class CompressedQuantumState:
         self._B = int(block_size)
         self._cfg = cfg
         self._gates = []
+        self._verbose = bool(verbose)
         if verbose:
             print("GPU (MI355X) block codec")
             print(f"  relative error bound: {cfg.eps_rel}")
@@ -234,7 +252,18 @@ class CompressedQuantumState:
         self._gates = gates
 
     def apply_circuit(self):
+        """The stage loop (compressed_state.hpp:840-937).  With verbose=True
+        the total compressed size is printed before the first stage and
+        after every stage, as append_total_SV_bytes prints it (:374-437,
+        931-933), once the loop has finished."""
         call("qsim_cs_apply_circuit", self._h)
+        if self._verbose:
+            hist = self.get_bytes_history()
+            ns = self.get_num_stages()
+            hist = hist[-(ns + 1):]
+            print(f"Total SV bytes: {_format_size(hist[0])}")
+            for i, b in enumerate(hist[1:]):
+                print(f"[Stage {i}/{ns - 1}] Total SV bytes: {_format_size(b)}")
 
     def run_stages(self, begin, end):
         call("qsim_cs_run_stages", self._h, int(begin), int(end))

@@ -163,8 +163,7 @@ typedef struct qsim_cs_config {
  *   consumes its input region by region and routes its output into the
  *   regions the next stage will consume; each input region's units are
  *   unmapped and reused as soon as it has been consumed (peak about one
- *   compressed state plus one region).  Checkpoint files are not supported
- *   in this mode. */
+ *   compressed state plus one region). */
 #define QSIM_STORE_DOUBLE 0
 #define QSIM_STORE_REGIONS 1
 
@@ -206,12 +205,17 @@ int qsim_cs_read_block(qsim_cs* cs, uint64_t block, double* host_amplitudes);
 int qsim_cs_set_basis(qsim_cs* cs, uint64_t index);
 int qsim_cs_load(qsim_cs* cs, const double* host_amplitudes);
 /* Compressed-store checkpoint files (no reference counterpart; the
- * checkpoint format of a long run, so that it can resume): the null flags,
- * the chunk descriptors, the arena's sub-arena table and cursors and the
- * payload arena of the current store, behind a header with the store's shape
- * and codec settings.  next_stage is stored for the caller (the first stage
- * still to run).  Loading requires a store of the same qubit count, block
- * size and codec settings (status -2 otherwise). */
+ * checkpoint format of a long run, so that it can resume), for both payload
+ * stores: a header with the store kind, shape and codec settings, the payload
+ * layout (double store: sub-arena table and allocation cursors; region
+ * store: the region key, span and per-region extents), the null flags, the
+ * chunk descriptors and the payload words.  next_stage is stored for the
+ * caller (the first stage still to run).  Loading requires a store of the
+ * same kind, qubit count, block size and codec settings (status -2
+ * otherwise), validates every descriptor against the payload extents (-2 on
+ * a corrupt file) and the file size against the layout (-1 when truncated)
+ * before it changes the store; an I/O failure while the payload is copied
+ * resets the store to |0...0> (-1). */
 int qsim_cs_save_checkpoint(qsim_cs* cs, const char* path, uint64_t next_stage);
 int qsim_cs_load_checkpoint(qsim_cs* cs, const char* path, uint64_t* next_stage);
 int qsim_cs_null_bitmap(const qsim_cs* cs, uint8_t* out);
@@ -235,6 +239,13 @@ int qsim_cs_store_bytes(const qsim_cs* cs, uint64_t* out3);
  * {gpu_seconds, bytes_in, bytes_out, decompressed_amplitudes, n_passes}. */
 int qsim_cs_stage_stats(const qsim_cs* cs, double* out5, uint64_t cap, uint64_t* n);
 #define QSIM_TIMER_COUNT 10 /* decomp gate comp fused_swap allgather hmalloc dmalloc h2d d2h dfree */
+/* get_total_times (python/cppsim_wrapper.cpp:503-519), seconds since the
+ * store was created: decomp, gate and comp from HIP events around the stage
+ * launches (DEC->RAW passes, RAW->RAW passes plus the fused DEC->ENC launch
+ * of a single-pass stage, RAW->ENC passes); dmalloc and dfree the wall time
+ * of device allocation, release, region mapping and unmapping; h2d and d2h
+ * the wall time of amplitude copies in load and readout.  fused_swap,
+ * allgather and hmalloc are 0 (one GPU, no host block buffers). */
 int qsim_cs_times(const qsim_cs* cs, double* out);
 /* sum_t coef_t <psi|P_t|psi> over the compressed store (no reference
  * counterpart: the reference reads expectation values from the decompressed

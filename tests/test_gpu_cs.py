@@ -342,6 +342,62 @@ def test_checkpoint_file_rejects_other_store(tmp_path):
         a.load_checkpoint(str(bad))
 
 
+def _ckpt_layout(path):
+    """(header fields, offset of the null flags) of a checkpoint file."""
+    import struct
+
+    raw = open(path, "rb").read()
+    magic, ver, store, n, B, nb, cpb = struct.unpack_from("<8s4I2Q", raw, 0)
+    aux0, aux1 = struct.unpack_from("<2Q", raw, 88)
+    if store == 0:
+        layout = (aux0 + 1) * 8 + aux0 * 16 * 8
+    else:
+        layout = 4 * 4 + 8 + 2 * aux1 * 8
+    return dict(n=n, B=B, nb=nb, cpb=cpb, aux0=aux0, aux1=aux1), 128 + layout
+
+
+@pytest.mark.parametrize("store", ["double", "regions"])
+def test_checkpoint_load_rejects_truncated_and_corrupt_files(tmp_path, store):
+    """A truncated file and a file whose descriptor points past the payload
+    are refused before the store changes: the state afterwards is the one
+    before the call."""
+    n, B = 14, 8
+    c = W.qft_on_product(n, seed=5)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    a = CompressedQuantumState(n, B, store=store, verbose=False)
+    a.partition_and_register_circuit(c, 4)
+    a.apply_circuit()
+    path = tmp_path / "a.qckp"
+    a.save_checkpoint(str(path), next_stage=3)
+    good = path.read_bytes()
+    b = CompressedQuantumState(n, B, store=store, verbose=False)
+    b.set_computational_basis(77)
+    before = b.get_raw_vector()
+    (tmp_path / "t.qckp").write_bytes(good[:-9])
+    with pytest.raises(RuntimeError, match="size"):
+        b.load_checkpoint(str(tmp_path / "t.qckp"))
+    assert np.array_equal(b.get_raw_vector(), before)
+    hdr, null_off = _ckpt_layout(str(path))
+    nullf = np.frombuffer(good, dtype=np.uint8, count=hdr["nb"], offset=null_off)
+    blk = int(np.flatnonzero(nullf == 0)[0])
+    doff = null_off + hdr["nb"] + blk * hdr["cpb"] * 8
+    bad = bytearray(good)
+    d = int.from_bytes(bad[doff:doff + 8], "little")
+    d = (d & ~((1 << 40) - 1)) | ((1 << 40) - 1)  # payload offset far past the store
+    bad[doff:doff + 8] = d.to_bytes(8, "little")
+    (tmp_path / "c.qckp").write_bytes(bytes(bad))
+    with pytest.raises(ValueError, match="corrupt descriptor"):
+        b.load_checkpoint(str(tmp_path / "c.qckp"))
+    bad = bytearray(good)
+    bad[doff + 5] = 70  # code width 70
+    (tmp_path / "w.qckp").write_bytes(bytes(bad))
+    with pytest.raises(ValueError, match="corrupt descriptor"):
+        b.load_checkpoint(str(tmp_path / "w.qckp"))
+    assert np.array_equal(b.get_raw_vector(), before)
+    assert b.load_checkpoint(str(path)) == 3
+    assert np.array_equal(b.get_raw_vector(), a.get_raw_vector())
+
+
 def test_compressed_tile_shapes_agree(monkeypatch):
     """Single-pass stages on 512-amplitude (default) and 1024-amplitude tiles:
     same stage results up to the rounding of the phase factors, same bitmap."""

@@ -110,10 +110,39 @@ def test_regions_basis_and_reset():
     assert st["arena"] > 0 and st["arena_peak"] >= st["arena"]
 
 
-def test_regions_reject_checkpoint_files(tmp_path):
-    _, b = _pair(12, 6)
-    with pytest.raises(ValueError, match="double-buffered"):
-        b.save_checkpoint(str(tmp_path / "x.ckpt"), 0)
+def test_regions_checkpoint_resume_is_exact(tmp_path):
+    """Region store checkpoint files: Trotter steps run, saved after step 2,
+    restored into a fresh region store and continued, end bit-identical to
+    an uninterrupted run; a region store refuses a double-store file."""
+    n, B = 18, 8
+    full = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, store="regions", verbose=False)
+    steps = [_tfim_step(n, s == 0) for s in range(4)]
+    for c in steps:
+        full.partition_and_register_circuit(c, 4)
+        full.apply_circuit()
+    a = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, store="regions", verbose=False)
+    for c in steps[:2]:
+        a.partition_and_register_circuit(c, 4)
+        a.apply_circuit()
+    path = str(tmp_path / "r.qckp")
+    a.save_checkpoint(path, next_stage=2)
+    b = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, store="regions", verbose=False)
+    assert b.load_checkpoint(path) == 2
+    assert np.array_equal(b.get_raw_vector().view(np.uint64), a.get_raw_vector().view(np.uint64))
+    assert np.array_equal(b.get_null_bitmap(), a.get_null_bitmap())
+    for c in steps[2:]:
+        b.partition_and_register_circuit(c, 4)
+        b.apply_circuit()
+    _same_state(b, full)
+    d = CompressedQuantumState(n, B, eps_rel=1e-3, eps_abs=1e-7, store="double", verbose=False)
+    with pytest.raises(ValueError, match="store kind"):
+        d.load_checkpoint(path)
+
+
+def _same_state(a, b):
+    assert np.array_equal(a.get_raw_vector().view(np.uint64), b.get_raw_vector().view(np.uint64))
+    assert np.array_equal(a.get_null_bitmap(), b.get_null_bitmap())
+    assert np.array_equal(a.get_block_bytes(), b.get_block_bytes())
 
 
 def test_regions_peak_below_two_states():

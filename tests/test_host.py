@@ -125,3 +125,25 @@ def test_commuting_partition_plan_is_valid_and_shorter():
     ref = O.apply_circuit(v, c2.gate_list)
     got = O.apply_circuit(v, [c2.gate_list[i] for s in st2 for i in s])
     assert np.max(np.abs(got - ref)) < 1e-12
+
+
+def test_qulacs_import_surface():
+    """The reference harness's imports (script/run_sim.py:4-6) resolve to this
+    package's classes."""
+    import qulacs
+    from qulacs import CompressedQuantumState, QuantumState
+    from qulacs.circuit import QuantumCircuitOptimizer
+    from qulacs.gate import CNOT, H
+    from qulacs.state import inner_product
+
+    import qompsim
+
+    assert QuantumState is qompsim.QuantumState
+    assert CompressedQuantumState is qompsim.CompressedQuantumState
+    assert QuantumCircuitOptimizer is qompsim.QuantumCircuitOptimizer
+    assert inner_product is qompsim.inner_product
+    assert qulacs.check_build_for_mpi() is False
+    c = qulacs.QuantumCircuit(3)
+    c.add_gate(H(0))
+    c.add_gate(CNOT(0, 1))
+    assert [g.get_name() for g in c.gate_list] == ["H", "CNOT"]
