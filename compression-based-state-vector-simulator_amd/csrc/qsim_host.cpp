@@ -58,6 +58,8 @@ uint32_t block_scan_tiles(uint64_t namps);
 hipError_t launch_block_sample(const double2* a, uint64_t namps, double* cdf_local, double* tile_sum, double* tile_incl,
                                const double* u, uint32_t nshots, uint64_t base, unsigned long long* out,
                                hipStream_t st);
+hipError_t launch_phase_table(const PassArgs* d_args, const Op* d_op, uint64_t ntau, uint64_t entries, double* ptab,
+                              hipStream_t st);
 hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
                                uint64_t ntiles, hipStream_t st);
 hipError_t launch_wave_pair_adjoint(int tile_bits, const PassArgs* d_args, double2* psi, double2* lam,
@@ -448,10 +450,68 @@ struct Program {
   std::vector<PhaseTerm> terms;
   std::vector<DiagTable> diags;
   std::vector<double> coefs;
-  DBuf d_args, d_ops, d_terms, d_diags, d_coefs;
+  // phase tables of the wave passes' elementwise ops, built on the device
+  // from the uploaded terms: (pass, op, tiles per group) per table
+  struct PtabJob {
+    size_t pass, op;
+    uint64_t ntau;
+  };
+  std::vector<PtabJob> ptab_jobs;
+  uint64_t ptab_len = 0;  // complex entries
+  DBuf d_args, d_ops, d_terms, d_diags, d_coefs, d_ptab;
   bool uploaded = false;
 
+  // Elementwise ops of wave passes whose phase terms separate into the
+  // factor tables (a slot term carries a lane or a base condition, not both;
+  // a term on several slots carries neither) take the table path, while the
+  // tables of the program fit kPhaseTableMaxEntries.  QOMPSIM_PHASE_TABLES=0
+  // keeps every op on the per-term loop.
+  void plan_phase_tables() {
+    ptab_jobs.clear();
+    ptab_len = 0;
+    for (auto& op : ops) op.ptab_off = kNoPhaseTable;
+    const char* e = std::getenv("QOMPSIM_PHASE_TABLES");
+    if (e && e[0] == '0') return;
+    for (size_t pi = 0; pi < passes.size(); ++pi) {
+      const PassArgs& a = passes[pi];
+      if (a.engine != ENGINE_WAVE || a.t < 7 || a.t > 10 || a.c != 6 || L < a.t) continue;
+      if (L - a.t > kPhaseTableMaxTauLog) continue;
+      const uint64_t ntau = 1ull << (L - a.t);
+      for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
+        Op& op = ops[o];
+        if (op.kind != OP_ELEM || op.nterms < kPhaseTableMinTerms) continue;
+        uint32_t slots = 0, flags = 0;
+        bool ok = true;
+        for (uint32_t i = 0; i < op.nterms; ++i) {
+          const PhaseTerm& T = terms[op.term_off + i];
+          if (T.bval & ~T.bmask) continue;  // padding: never matches
+          const uint32_t lm = T.tmask & 63u, rm = (T.tmask >> 6) & 15u, rv = (T.tval >> 6) & 15u;
+          if (rm == 0u) {
+            flags |= kPtabT0;
+          } else if (rm & (rm - 1u)) {
+            ok = ok && lm == 0u && T.bmask == 0ull;
+            flags |= kPtabRows;
+          } else {
+            ok = ok && !(lm != 0u && T.bmask != 0ull);
+            slots |= rm;
+            if (rv == 0u) flags |= kPtabT0;
+          }
+        }
+        const uint64_t len = kPtabF + 68ull * ntau;
+        if (!ok || ptab_len + len > kPhaseTableMaxEntries) continue;
+        op.ptab_off = ptab_len;
+        op.ptab_t0 = (uint32_t)(kPtabF + 4ull * ntau);
+        op.ptab_slots = slots;
+        op.ptab_flags = flags;
+        ptab_jobs.push_back({pi, o, ntau});
+        ptab_len += (len + 63) & ~63ull;
+      }
+    }
+  }
+
   void upload_all() {
+    plan_phase_tables();
+    if (ptab_len) d_ptab.ensure(ptab_len * 2 * sizeof(double));
     upload(d_ops, ops.data(), ops.size() * sizeof(Op));
     // kTermGroup never-matching terms past the end: the kernels read the
     // next group of an op's terms ahead of the current one
@@ -469,8 +529,14 @@ struct Program {
       p.terms = d_terms.as<PhaseTerm>();
       p.diags = d_diags.as<DiagTable>();
       p.coefs = d_coefs.as<double>();
+      p.ptab = ptab_len ? d_ptab.as<double>() : nullptr;
     }
     upload(d_args, passes.data(), passes.size() * sizeof(PassArgs));
+    if (std::getenv("QOMPSIM_PLAN_LOG"))
+      fprintf(stderr, "[plan] phase tables: ops=%zu entries=%llu\n", ptab_jobs.size(), (unsigned long long)ptab_len);
+    for (const auto& j : ptab_jobs)
+      HIPCHECK(launch_phase_table(dargs(j.pass), d_ops.as<Op>() + j.op, j.ntau, kPtabF + 68ull * j.ntau,
+                                  d_ptab.as<double>(), stream()));
     HIPCHECK(hipStreamSynchronize(stream()));
     uploaded = true;
   }
@@ -483,6 +549,8 @@ struct Program {
     terms.clear();
     diags.clear();
     coefs.clear();
+    ptab_jobs.clear();
+    ptab_len = 0;
     uploaded = false;
   }
 };

@@ -43,6 +43,8 @@ constexpr uint32_t kWaveTileSmall = 9;  // 6 lane bits + 3 register slots (8 row
 constexpr uint32_t kGradReduceChunks = 64;
 constexpr uint64_t kMaxLaunchBlocks = 1ull << 23;
 
+constexpr uint64_t kNoPhaseTable = ~0ull;
+
 // One op of a pass.  Dense: 2^nsel matrices of 2^k x 2^k (row-major, matrix
 // index bit i <-> tile bit tpos[i]); the matrix is selected by the selector
 // bits (each a tile bit or a base bit).  Elementwise: a run of diagonal gates,
@@ -69,7 +71,35 @@ struct Op {
   // OP_DENSE matrix structure (kDenseReal / kDenseRealDiagImagOff, else 0);
   // the one-target paths then skip the products with zero parts
   uint32_t real, pad_;
+  // OP_ELEM on the wave engine: the op's phase terms, when ptab_off is not
+  // kNoPhaseTable, are applied from factor tables at that complex offset of
+  // the pass's phase-table array (the terms stay for the other engines)
+  uint64_t ptab_off = kNoPhaseTable;
+  uint32_t ptab_t0;        // offset of T0 from ptab_off (complex)
+  uint32_t ptab_slots;     // bit j: slot j has a factor
+  uint32_t ptab_flags;     // kPtabT0 | kPtabRows: those tables hold a term
+  uint32_t pad2_;
 };
+
+// Phase tables of an elementwise op on a wave tile: lanes on tile bits 0..5,
+// RB register slots on tile bits 6 .., tau the tile's index in its group
+// (ntau = 2^(L - t) tiles).  Element (lane l, row v) of tile tau gets
+//   T0[tau][l] * prod_{j : bit j of v} (C[j][l] * F[tau][j]) * R[v],
+// each entry cis(sum of the angles of its terms, in term order), from
+//   C[4][64]   single-slot terms without a base condition (+angle when they
+//              require slot bit 1, -angle when they require 0),
+//   R[16]      terms on two or more slots (no lane or base condition),
+//   F[ntau][4] single-slot terms with a base condition (no lane condition),
+//   T0[ntau][64] terms without a slot condition and, for single-slot terms
+//              requiring slot bit 0, their +angle,
+// at complex offsets 0, 256, 272 and 272 + 4 ntau.  Ops whose terms do not
+// separate so (a slot term with both a lane and a base condition) keep the
+// per-term loop.
+constexpr uint32_t kPtabC = 0, kPtabR = 256, kPtabF = 272;
+constexpr uint32_t kPtabT0 = 1, kPtabRows = 2;
+constexpr uint32_t kPhaseTableMinTerms = 4;     // fewer terms: the per-term loop
+constexpr uint32_t kPhaseTableMaxTauLog = 16;   // T0 of at most 2^16 x 64 entries (64 MiB)
+constexpr uint64_t kPhaseTableMaxEntries = 1ull << 26;  // 1 GiB of tables per program
 
 // Dense matrix structures with cheaper one-target paths: every entry real
 // (RY, H, X), or one target with real diagonal and imaginary off-diagonal
@@ -127,6 +157,7 @@ struct PassArgs {
   const PhaseTerm* terms;
   const DiagTable* diags;
   const double* coefs;     // interleaved complex
+  const double* ptab;      // phase tables of the program's elementwise ops (interleaved complex)
   // compressed side
   uint32_t B;              // block bits
   uint32_t khi;            // L - B: group-local high bits (blocks per group = 2^khi)

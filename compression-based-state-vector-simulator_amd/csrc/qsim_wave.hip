@@ -376,6 +376,8 @@ __device__ __forceinline__ uint32_t lane_quantise(double2 (&amp)[KV], const Code
 struct WaveCtx {
   int lane;
   uint64_t base;
+  uint64_t tau;          // the tile's index in its group (phase tables)
+  const double* ptab;    // the pass's phase tables
 };
 
 // selector of the op for register index v: bits from lane, slots or base
@@ -694,7 +696,51 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], c
                                           const QC PhaseTerm* __restrict__ terms, const QC DiagTable* __restrict__ diags,
                                           const QC double* __restrict__ C) {
   const uint32_t lane = (uint32_t)cx.lane;
-  if (op.nterms) {
+  if (op.ptab_off != kNoPhaseTable) {
+    // phase tables: a per-lane factor of the tile, per slot a per-lane
+    // factor times a per-tile one, and per row a uniform factor
+    const QG double* TG = (const QG double*)(cx.ptab) + 2 * op.ptab_off;
+    const QC double* TU = (const QC double*)(cx.ptab) + 2 * op.ptab_off;
+    auto ldg = [&](uint64_t i) { return make_double2(TG[2 * i], TG[2 * i + 1]); };
+    const uint32_t sl = op.ptab_slots;
+    double2 g0 = make_double2(1.0, 0.0), g1 = g0, g2 = g0, g3 = g0, t0 = g0;
+    if (op.ptab_flags & kPtabT0) t0 = ldg(op.ptab_t0 + (cx.tau << 6) + lane);
+    if (sl & 1u) g0 = ldg(kPtabC + lane);
+    if (sl & 2u) g1 = ldg(kPtabC + 64 + lane);
+    if (sl & 4u) g2 = ldg(kPtabC + 128 + lane);
+    if (KV > 8 && (sl & 8u)) g3 = ldg(kPtabC + 192 + lane);
+    const uint64_t F = kPtabF + (cx.tau << 2);
+    if (op.ptab_flags & kPtabT0) {
+#pragma unroll
+      for (int v = 0; v < KV; ++v) amp[v] = cmul(amp[v], t0);
+    }
+    if (sl & 1u) {
+      const double2 f = cmul(g0, ldc(TU, F + 0));
+#pragma unroll
+      for (int v = 1; v < KV; v += 2) amp[v] = cmul(amp[v], f);
+    }
+    if (sl & 2u) {
+      const double2 f = cmul(g1, ldc(TU, F + 1));
+#pragma unroll
+      for (int v = 0; v < KV; ++v)
+        if (v & 2) amp[v] = cmul(amp[v], f);
+    }
+    if (sl & 4u) {
+      const double2 f = cmul(g2, ldc(TU, F + 2));
+#pragma unroll
+      for (int v = 0; v < KV; ++v)
+        if (v & 4) amp[v] = cmul(amp[v], f);
+    }
+    if (KV > 8 && (sl & 8u)) {
+      const double2 f = cmul(g3, ldc(TU, F + 3));
+#pragma unroll
+      for (int v = 8; v < KV; ++v) amp[v] = cmul(amp[v], f);
+    }
+    if (op.ptab_flags & kPtabRows) {
+#pragma unroll
+      for (int v = 0; v < KV; ++v) amp[v] = cmul(amp[v], ldc(TU, kPtabR + v));
+    }
+  } else if (op.nterms) {
     double aL = 0.0, aF0 = 0.0, aF1 = 0.0, aF2 = 0.0, aF3 = 0.0;
     uint32_t used = 0;
     bool generic = false;
@@ -885,7 +931,7 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   auto roff = [&](int v) -> uint64_t {
     return ((v & 1) ? e0 : 0ull) | ((v & 2) ? e1 : 0ull) | ((v & 4) ? e2 : 0ull) | ((v & 8) ? e3 : 0ull);
   };
-  const WaveCtx cx{lane, base};
+  const WaveCtx cx{lane, base, tau, a.ptab};
   double2 amp[KV];
   double2* xb = reinterpret_cast<double2*>(wbuf[wave]);
   const uint32_t g = (uint32_t)lane & (LPC - 1u);  // part of the chunk (chunk layout)
@@ -1138,7 +1184,7 @@ __global__ __launch_bounds__(kThreads, 2) void wave_adjoint_kernel(const PassArg
   const uint64_t base = tm.base(wt);
   const uint64_t e0 = 1ull << a.E[0], e1 = 1ull << a.E[1], e2 = 1ull << a.E[2];
   auto roff = [&](int v) -> uint64_t { return ((v & 1) ? e0 : 0ull) | ((v & 2) ? e1 : 0ull) | ((v & 4) ? e2 : 0ull); };
-  const WaveCtx cx{lane, base};
+  const WaveCtx cx{lane, base, wt, a.ptab};
   double2 amp[kV];
   {
     const QG double* sp = (const QG double*)(psi) + 2 * (base + lane);
@@ -1250,7 +1296,7 @@ __global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kerne
   auto roff = [&](int v) -> uint64_t {
     return ((v & 1) ? e0 : 0ull) | ((v & 2) ? e1 : 0ull) | ((v & 4) ? e2 : 0ull) | ((v & 8) ? e3 : 0ull);
   };
-  const WaveCtx cx{lane, base};
+  const WaveCtx cx{lane, base, wt, a.ptab};
   QG double* sv = (QG double*)(is_lam ? lam : psi) + 2 * (base + lane);
   double2 amp[KV];
 #pragma unroll
@@ -1321,6 +1367,73 @@ __global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kerne
     sv[2 * roff(v)] = amp[v].x;
     sv[2 * roff(v) + 1] = amp[v].y;
   }
+}
+
+// Phase tables of one elementwise op (layout beside kPtabC in
+// qsim_internal.h): one entry per thread, the angles of the op's matching
+// terms summed in term order and turned into cis().  A term on one slot j
+// that requires bit j = 0 adds its angle to T0 and subtracts it from slot
+// j's factor (angle * [s_j = 0] = angle - angle * s_j).
+__global__ __launch_bounds__(kThreads) void phase_table_kernel(const PassArgs* __restrict__ pa,
+                                                               const Op* __restrict__ opp, uint64_t ntau,
+                                                               double2* __restrict__ ptab) {
+  const PassArgs& a = *pa;
+  const Op& op = *opp;
+  const uint64_t e = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t total = op.ptab_t0 + (ntau << 6);
+  if (e >= total) return;
+  const TileMapDev tm{a.t, 6u, a.t - 6u, a.E};
+  enum { K_C, K_R, K_F, K_T0 } kind;
+  uint32_t lane = 0, j = 0, v = 0;
+  uint64_t base = 0;
+  if (e < kPtabR) {
+    kind = K_C;
+    j = (uint32_t)(e >> 6);
+    lane = (uint32_t)(e & 63u);
+  } else if (e < kPtabF) {
+    kind = K_R;
+    v = (uint32_t)(e - kPtabR);
+  } else if (e < op.ptab_t0) {
+    kind = K_F;
+    j = (uint32_t)((e - kPtabF) & 3u);
+    base = tm.base((e - kPtabF) >> 2);
+  } else {
+    kind = K_T0;
+    lane = (uint32_t)((e - op.ptab_t0) & 63u);
+    base = tm.base((e - op.ptab_t0) >> 6);
+  }
+  double s = 0.0;
+  for (uint32_t i = 0; i < op.nterms; ++i) {
+    const PhaseTerm& T = a.terms[op.term_off + i];
+    const uint32_t lm = T.tmask & 63u, lv = T.tval & 63u;
+    const uint32_t rm = (T.tmask >> 6) & 15u, rv = (T.tval >> 6) & 15u;
+    const bool lane_ok = (lane & lm) == lv, base_ok = (base & T.bmask) == T.bval;
+    if (rm == 0u) {
+      if (kind == K_T0 && lane_ok && base_ok) s += T.angle;
+    } else if (rm & (rm - 1u)) {
+      if (kind == K_R && (v & rm) == rv) s += T.angle;
+    } else {
+      const uint32_t js = __ffs(rm) - 1;
+      if (kind == K_T0) {
+        if (rv == 0u && lane_ok && base_ok) s += T.angle;
+      } else if (js == j) {
+        const double sa = rv ? T.angle : -T.angle;
+        if (kind == K_C && T.bmask == 0ull && lane_ok) s += sa;
+        if (kind == K_F && T.bmask != 0ull && base_ok) s += sa;
+      }
+    }
+  }
+  ptab[op.ptab_off + e] = cis_reduced(s);
+}
+
+hipError_t launch_phase_table(const PassArgs* d_args, const Op* d_op, uint64_t ntau, uint64_t entries, double* ptab,
+                              hipStream_t st) {
+  const uint64_t groups = (entries + kThreads - 1) / kThreads;
+  if (groups == 0) return hipSuccess;
+  if (groups > kMaxLaunchBlocks) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(phase_table_kernel, dim3((unsigned)groups), dim3(kThreads), 0, st, d_args, d_op, ntau,
+                     reinterpret_cast<double2*>(ptab));
+  return hipGetLastError();
 }
 
 // ntiles counts tiles of 2^tile_bits amplitudes (10: 16 rows per wave, 9: 8
