@@ -34,6 +34,13 @@ inline uint64_t persistent_groups(uint64_t dflt) {
   return (v > 0 && (uint64_t)v < kMaxLaunchBlocks) ? (uint64_t)v : dflt;
 }
 
+// Waves of a persistent wave-kernel launch over ntiles tiles (four waves per
+// work-group, at most persistent_groups(8192) work-groups).
+inline uint64_t persistent_waves(uint64_t ntiles) {
+  const uint64_t groups = (ntiles + 3) / 4 < persistent_groups(8192) ? (ntiles + 3) / 4 : persistent_groups(8192);
+  return groups * 4;
+}
+
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ uint32_t ins0(uint32_t x, uint32_t pos) {
   uint32_t lo = x & ((1u << pos) - 1u);

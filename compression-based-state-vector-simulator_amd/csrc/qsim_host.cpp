@@ -58,6 +58,8 @@ uint32_t block_scan_tiles(uint64_t namps);
 hipError_t launch_block_sample(const double2* a, uint64_t namps, double* cdf_local, double* tile_sum, double* tile_incl,
                                const double* u, uint32_t nshots, uint64_t base, unsigned long long* out,
                                hipStream_t st);
+// waves of the persistent DEC->ENC grid over ntiles tiles
+uint64_t wave_persistent_waves(uint64_t ntiles);
 hipError_t launch_phase_table(const PassArgs* d_args, const Op* d_op, uint64_t ntau, uint64_t entries, double* ptab,
                               hipStream_t st);
 hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
@@ -1548,16 +1550,19 @@ static void cs_run_stage(qsim_cs* cs, uint32_t si) {
   upload(cs->bmap, bm.data(), bm.size() * sizeof(uint32_t));
   const int out = 1 - cs->cur;
   float ms = 0.f;
+  // A single wave pass (the persistent DEC->ENC kernel) on the double store
+  // gives each wave of its grid a sub-arena of its own.
+  const uint64_t last_tiles = nact << (L - P.passes.back().t);
+  const bool own = P.passes.size() == 1 && P.passes[0].engine == ENGINE_WAVE && !cs->regions &&
+                   wave_persistent_waves(last_tiles) <= (uint64_t)kMaxSubArenas;
+  const uint32_t nsub = own ? (uint32_t)wave_persistent_waves(last_tiles) : pick_nsub(last_tiles);
   for (int attempt = 0;; ++attempt) {
     HIPCHECK(launch_fill_u8(cs->nonzero.as<uint8_t>(), cs->nb, 0, stream()));
-    {
-      const PassArgs& last = P.passes.back();
-      const uint32_t tb = last.t;
-      cs_reset_out(cs, out, pick_nsub(nact << (L - tb)), attempt > 0);
-    }
+    cs_reset_out(cs, out, nsub, attempt > 0);
     HIPCHECK(hipEventRecord(cs->ev0, stream()));
     cs->npev = 0;
     PassDyn dy = cs_dyn(cs, out);
+    dy.wave_subs = own ? 1u : 0u;
     if (P.passes.size() == 1) {
       dy.bmap = cs->bmap.as<uint32_t>();
       HIPCHECK(launch_program_pass(P, 0, IN_DEC, OUT_ENC, dy, nact, stream()));
@@ -3440,8 +3445,7 @@ int qsim_cs_load_checkpoint(qsim_cs* cs, const char* path, uint64_t* next_stage)
   uint64_t layout_bytes = 0;
   if (kind == QSIM_STORE_DOUBLE) {
     const uint64_t nsub = h.aux0, cap = h.aux1;
-    if (nsub == 0 || nsub > (uint64_t)kMaxSubArenas || (nsub & (nsub - 1)))
-      throw std::invalid_argument("checkpoint: bad sub-arena count");
+    if (nsub == 0 || nsub > (uint64_t)kMaxSubArenas) throw std::invalid_argument("checkpoint: bad sub-arena count");
     if (cap == 0 || cap >= (1ull << 40) || h.payload_words != cap + 8)
       throw std::invalid_argument("checkpoint: bad arena capacity");
     layout_bytes = (nsub + 1) * 8 + nsub * 16 * 8;

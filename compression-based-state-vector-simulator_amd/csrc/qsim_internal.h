@@ -191,7 +191,8 @@ struct PassDyn {
                                 //   bounds at sub_base[2s], sub_base[2s + 1]); else null
   unsigned long long* cursors;  // [sub_mask + 1] (stride 16 words)
   uint32_t sub_mask;        // number of sub-arenas - 1 (power of two)
-  uint32_t pad_;
+  uint32_t wave_subs;       // persistent DEC->ENC: wave w of the grid owns sub-arena w
+                            //   (as many sub-arenas as waves; cursors[16 w] = its demand)
   uint32_t* overflow;
   uint8_t* nonzero;         // [nb]: set when a block null at start holds data
   uint64_t tile0;           // first tile of this launch

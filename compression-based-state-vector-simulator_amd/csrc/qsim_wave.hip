@@ -920,6 +920,13 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
                                        16, 0, 0);
     }
   };
+  // Wave-owned sub-arenas (persistent DEC->ENC on the double store, when
+  // dy.wave_subs is set): wave w of the grid packs the payloads of all its
+  // tiles back to back into sub-arena w, so the allocation is a running sum
+  // kept in a scalar register instead of a device-scope atomic per tile.
+  const bool own = kPersistent && OUT == OUT_ENC && dy.wave_subs != 0u;
+  const uint32_t gwave = (uint32_t)blockIdx.x * (kThreads / 64) + (uint32_t)wave;
+  uint64_t wcur = 0;
   auto tile_body = [&](const uint64_t wt, auto&& pre) {
   const uint32_t L = a.L;
   const uint64_t slot = wt >> (L - TB);
@@ -1059,7 +1066,12 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
     const uint32_t sub = dy.slot_region ? 2u * (dy.slot_region[slot] * (dy.sub_mask + 1u) + region_piece(wt, dy.sub_mask))
                                         : (uint32_t)(wt & dy.sub_mask);
     unsigned long long aoff = 0;
-    if (lane == 0 && tot > 0) aoff = atomicAdd(&dy.cursors[sub * 16], (unsigned long long)tot);
+    if (own) {
+      aoff = wcur;
+      wcur += tot;
+    } else if (lane == 0 && tot > 0) {
+      aoff = atomicAdd(&dy.cursors[sub * 16], (unsigned long long)tot);
+    }
     uint64_t* st = wbuf[wave];
     for (uint32_t q = lane; q < tot; q += 64) st[q] = 0ull;
     __builtin_amdgcn_wave_barrier();
@@ -1081,7 +1093,8 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
     __builtin_amdgcn_wave_barrier();
     if (tot > 0) {
       const uint64_t off = readlane64(aoff, 0);
-      const uint64_t sb0 = dy.sub_base[sub], sb1 = dy.sub_base[sub + 1];
+      const uint32_t asub = own ? gwave : sub;
+      const uint64_t sb0 = dy.sub_base[asub], sb1 = dy.sub_base[asub + 1];
       if (off + tot > sb1 - sb0) {
         if (lane == 0) atomicOr(dy.overflow, 1u);
       } else {
@@ -1105,6 +1118,8 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   } else if constexpr (kPersistent) {
     const uint64_t stride = (uint64_t)gridDim.x * (kThreads / 64);
     for (uint64_t wt = wt0; wt < ntiles; wt += stride) tile_body(wt, [] {});
+    // the sub-arena's demand (all tiles counted, also past an overflow)
+    if (own && lane == 0) dy.cursors[(uint64_t)gwave * 16] = wcur;
   } else {
     if (wt0 < ntiles) tile_body(wt0, [] {});
   }
@@ -1468,6 +1483,8 @@ hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* p
   return hipGetLastError();
 }
 
+uint64_t wave_persistent_waves(uint64_t ntiles) { return persistent_waves(ntiles); }
+
 // DEC->ENC, and RAW->RAW on 1024-amplitude tiles: one launch, a persistent
 // grid of at most kWaveGroups work-groups (each wave walks its tiles with a
 // stride of the grid).  Other forms: one tile per wave, launches split at
@@ -1476,7 +1493,8 @@ template <int IN, int OUT, int KV>
 static hipError_t launch_wave_mode(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
   constexpr uint64_t per_block = kThreads / 64;
   if constexpr ((IN == IN_DEC && OUT == OUT_ENC) || (IN == IN_RAW && OUT == OUT_RAW && KV == 16)) {
-    const uint64_t groups = std::min<uint64_t>((ntiles + per_block - 1) / per_block, persistent_groups(8192));
+    static_assert(per_block == 4, "persistent_waves assumes four waves per work-group");
+    const uint64_t groups = persistent_waves(ntiles) / per_block;
     if (groups == 0) return hipSuccess;
     hipLaunchKernelGGL((wave_kernel<IN, OUT, KV>), dim3((unsigned)groups), dim3(kThreads), 0, st, d_args, dy,
                        dy.tile0 + ntiles);
