@@ -456,7 +456,7 @@ struct Program {
   // from the uploaded terms: (pass, op, tiles per group) per table
   struct PtabJob {
     size_t pass, op;
-    uint64_t ntau;
+    uint64_t ntau, entries;
   };
   std::vector<PtabJob> ptab_jobs;
   uint64_t ptab_len = 0;  // complex entries
@@ -483,29 +483,34 @@ struct Program {
         Op& op = ops[o];
         if (op.kind != OP_ELEM || op.nterms < kPhaseTableMinTerms) continue;
         uint32_t slots = 0, flags = 0;
-        bool ok = true;
+        bool ok = true, t0_lane = false;
         for (uint32_t i = 0; i < op.nterms; ++i) {
           const PhaseTerm& T = terms[op.term_off + i];
           if (T.bval & ~T.bmask) continue;  // padding: never matches
           const uint32_t lm = T.tmask & 63u, rm = (T.tmask >> 6) & 15u, rv = (T.tval >> 6) & 15u;
           if (rm == 0u) {
             flags |= kPtabT0;
+            if (lm) t0_lane = true;
           } else if (rm & (rm - 1u)) {
             ok = ok && lm == 0u && T.bmask == 0ull;
             flags |= kPtabRows;
           } else {
             ok = ok && !(lm != 0u && T.bmask != 0ull);
             slots |= rm;
-            if (rv == 0u) flags |= kPtabT0;
+            if (rv == 0u) {
+              flags |= kPtabT0;
+              if (lm) t0_lane = true;
+            }
           }
         }
-        const uint64_t len = kPtabF + 68ull * ntau;
+        if (!t0_lane) flags |= kPtabT0Uniform;
+        const uint64_t len = kPtabF + (t0_lane ? 68ull : 5ull) * ntau;
         if (!ok || ptab_len + len > kPhaseTableMaxEntries) continue;
         op.ptab_off = ptab_len;
         op.ptab_t0 = (uint32_t)(kPtabF + 4ull * ntau);
         op.ptab_slots = slots;
         op.ptab_flags = flags;
-        ptab_jobs.push_back({pi, o, ntau});
+        ptab_jobs.push_back({pi, o, ntau, len});
         ptab_len += (len + 63) & ~63ull;
       }
     }
@@ -537,8 +542,8 @@ struct Program {
     if (std::getenv("QOMPSIM_PLAN_LOG"))
       fprintf(stderr, "[plan] phase tables: ops=%zu entries=%llu\n", ptab_jobs.size(), (unsigned long long)ptab_len);
     for (const auto& j : ptab_jobs)
-      HIPCHECK(launch_phase_table(dargs(j.pass), d_ops.as<Op>() + j.op, j.ntau, kPtabF + 68ull * j.ntau,
-                                  d_ptab.as<double>(), stream()));
+      HIPCHECK(launch_phase_table(dargs(j.pass), d_ops.as<Op>() + j.op, j.ntau, j.entries, d_ptab.as<double>(),
+                                  stream()));
     HIPCHECK(hipStreamSynchronize(stream()));
     uploaded = true;
   }

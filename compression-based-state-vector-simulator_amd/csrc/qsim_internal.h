@@ -97,6 +97,9 @@ struct Op {
 // per-term loop.
 constexpr uint32_t kPtabC = 0, kPtabR = 256, kPtabF = 272;
 constexpr uint32_t kPtabT0 = 1, kPtabRows = 2;
+// T0 without a lane condition (every term of T0 conditions on base bits
+// only): one entry per tile, T0[ntau], read with a scalar load
+constexpr uint32_t kPtabT0Uniform = 4;
 constexpr uint32_t kPhaseTableMinTerms = 4;     // fewer terms: the per-term loop
 constexpr uint32_t kPhaseTableMaxTauLog = 16;   // T0 of at most 2^16 x 64 entries (64 MiB)
 constexpr uint64_t kPhaseTableMaxEntries = 1ull << 26;  // 1 GiB of tables per program

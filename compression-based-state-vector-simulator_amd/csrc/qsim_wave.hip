@@ -704,7 +704,8 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], c
     auto ldg = [&](uint64_t i) { return make_double2(TG[2 * i], TG[2 * i + 1]); };
     const uint32_t sl = op.ptab_slots;
     double2 g0 = make_double2(1.0, 0.0), g1 = g0, g2 = g0, g3 = g0, t0 = g0;
-    if (op.ptab_flags & kPtabT0) t0 = ldg(op.ptab_t0 + (cx.tau << 6) + lane);
+    if (op.ptab_flags & kPtabT0)
+      t0 = (op.ptab_flags & kPtabT0Uniform) ? ldc(TU, op.ptab_t0 + cx.tau) : ldg(op.ptab_t0 + (cx.tau << 6) + lane);
     if (sl & 1u) g0 = ldg(kPtabC + lane);
     if (sl & 2u) g1 = ldg(kPtabC + 64 + lane);
     if (sl & 4u) g2 = ldg(kPtabC + 128 + lane);
@@ -1395,7 +1396,8 @@ __global__ __launch_bounds__(kThreads) void phase_table_kernel(const PassArgs* _
   const PassArgs& a = *pa;
   const Op& op = *opp;
   const uint64_t e = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  const uint64_t total = op.ptab_t0 + (ntau << 6);
+  const uint32_t ls = (op.ptab_flags & kPtabT0Uniform) ? 0u : 6u;  // log2 T0 entries per tile
+  const uint64_t total = op.ptab_t0 + (ntau << ls);
   if (e >= total) return;
   const TileMapDev tm{a.t, 6u, a.t - 6u, a.E};
   enum { K_C, K_R, K_F, K_T0 } kind;
@@ -1414,8 +1416,8 @@ __global__ __launch_bounds__(kThreads) void phase_table_kernel(const PassArgs* _
     base = tm.base((e - kPtabF) >> 2);
   } else {
     kind = K_T0;
-    lane = (uint32_t)((e - op.ptab_t0) & 63u);
-    base = tm.base((e - op.ptab_t0) >> 6);
+    lane = (uint32_t)((e - op.ptab_t0) & ((1u << ls) - 1u));
+    base = tm.base((e - op.ptab_t0) >> ls);
   }
   double s = 0.0;
   for (uint32_t i = 0; i < op.nterms; ++i) {

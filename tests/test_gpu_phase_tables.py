@@ -66,6 +66,32 @@ def test_tables_match_term_loop_and_oracle(n, rng, monkeypatch, capfd):
     assert np.max(np.abs(v1 - O.apply_circuit(psi, c.gate_list))) < TOL
 
 
+def test_tables_tile_uniform_factor(rng, monkeypatch, capfd):
+    """Diagonal runs conditioned on base qubits only (controlled phases and
+    RZ among the high qubits, between H layers on the low ones): the
+    per-tile factor is one entry per tile, read as a uniform value."""
+    n = 15
+    monkeypatch.setenv("QOMPSIM_PLAN_LOG", "1")
+    c = QuantumCircuit(n)
+    for layer in range(3):
+        for q in range(4):
+            c.add_H_gate(q)
+        for a in range(11, n):
+            c.add_RZ_gate(a, rng.uniform(0, 6.3))
+            for b in range(a + 1, n):
+                c.add_gate(W.cphase(a, b, rng.uniform(0, 6.3)))
+    psi = rand_state(n, rng)
+    s1, s2 = QuantumState(n), QuantumState(n)
+    s1.load(psi)
+    s2.load(psi)
+    c.update_quantum_state(s1)
+    assert _table_ops(capfd.readouterr().err) > 0
+    monkeypatch.setenv("QOMPSIM_PHASE_TABLES", "0")
+    c.update_quantum_state(s2)
+    assert np.max(np.abs(s1.get_vector() - s2.get_vector())) < TOL
+    assert np.max(np.abs(s1.get_vector() - O.apply_circuit(psi, c.gate_list))) < TOL
+
+
 @pytest.mark.parametrize("tb", ["9", "10"])
 def test_tables_compressed_mixed_circuit(tb, rng, monkeypatch, capfd):
     """The mixed diagonal circuit through the compressed stage loop on 512-
