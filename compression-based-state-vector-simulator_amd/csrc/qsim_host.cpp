@@ -483,31 +483,44 @@ struct Program {
         Op& op = ops[o];
         if (op.kind != OP_ELEM || op.nterms < kPhaseTableMinTerms) continue;
         uint32_t slots = 0, flags = 0;
-        bool ok = true, t0_lane = false;
+        bool ok = true;
+        // terms on one lane bit and base bits: the lane table M while it
+        // fits, else the lane-bit angles G
+        const bool use_g = L - a.t > kPhaseTableMaxTauLogM;
+        if (use_g) flags |= kPtabG;
+        auto slot_free = [&](uint32_t lm, uint32_t lv, uint64_t bm) {
+          if (use_g && lm && bm && !(lm & (lm - 1u))) {
+            slots |= lm << 8;
+            if (!lv) flags |= kPtabU;
+          } else {
+            flags |= (lm && bm) ? kPtabM : (bm ? kPtabU : kPtabQ);
+          }
+        };
         for (uint32_t i = 0; i < op.nterms; ++i) {
           const PhaseTerm& T = terms[op.term_off + i];
           if (T.bval & ~T.bmask) continue;  // padding: never matches
-          const uint32_t lm = T.tmask & 63u, rm = (T.tmask >> 6) & 15u, rv = (T.tval >> 6) & 15u;
+          const uint32_t lm = T.tmask & 63u, lv = T.tval & 63u, rm = (T.tmask >> 6) & 15u, rv = (T.tval >> 6) & 15u;
           if (rm == 0u) {
-            flags |= kPtabT0;
-            if (lm) t0_lane = true;
+            slot_free(lm, lv, T.bmask);
           } else if (rm & (rm - 1u)) {
             ok = ok && lm == 0u && T.bmask == 0ull;
             flags |= kPtabRows;
           } else {
             ok = ok && !(lm != 0u && T.bmask != 0ull);
             slots |= rm;
-            if (rv == 0u) {
-              flags |= kPtabT0;
-              if (lm) t0_lane = true;
-            }
+            if (rv == 0u) slot_free(lm, lv, T.bmask);
           }
         }
-        if (!t0_lane) flags |= kPtabT0Uniform;
-        const uint64_t len = kPtabF + (t0_lane ? 68ull : 5ull) * ntau;
+        if ((flags & kPtabM) && L - a.t > kPhaseTableMaxTauLogM) ok = false;
+        // a lane table M takes every slot-free part (one read per tile)
+        if (flags & kPtabM) flags &= ~(kPtabQ | kPtabU);
+        const uint64_t q = kPtabF + 4ull * ntau, m = q + 64 + 9ull * ntau;
+        const uint64_t len = m + ((flags & kPtabM) ? 64ull * ntau : 0ull);
         if (!ok || ptab_len + len > kPhaseTableMaxEntries) continue;
         op.ptab_off = ptab_len;
-        op.ptab_t0 = (uint32_t)(kPtabF + 4ull * ntau);
+        op.ptab_q = (uint32_t)q;
+        op.ptab_g = (uint32_t)(q + 64 + ntau);
+        op.ptab_m = (uint32_t)m;
         op.ptab_slots = slots;
         op.ptab_flags = flags;
         ptab_jobs.push_back({pi, o, ntau, len});

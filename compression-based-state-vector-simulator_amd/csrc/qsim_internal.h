@@ -75,33 +75,42 @@ struct Op {
   // kNoPhaseTable, are applied from factor tables at that complex offset of
   // the pass's phase-table array (the terms stay for the other engines)
   uint64_t ptab_off = kNoPhaseTable;
-  uint32_t ptab_t0;        // offset of T0 from ptab_off (complex)
-  uint32_t ptab_slots;     // bit j: slot j has a factor
-  uint32_t ptab_flags;     // kPtabT0 | kPtabRows: those tables hold a term
-  uint32_t pad2_;
+  uint32_t ptab_q;         // offsets from ptab_off (complex) of Q (U follows Q), G and M
+  uint32_t ptab_g, ptab_m, pad2_;
+  uint32_t ptab_slots;     // bit j: slot j has a factor; bit 8 + k: lane bit k has a G factor
+  uint32_t ptab_flags;     // kPtabQ | kPtabU | kPtabM | kPtabRows: those tables hold a term
 };
 
 // Phase tables of an elementwise op on a wave tile: lanes on tile bits 0..5,
 // RB register slots on tile bits 6 .., tau the tile's index in its group
 // (ntau = 2^(L - t) tiles).  Element (lane l, row v) of tile tau gets
-//   T0[tau][l] * prod_{j : bit j of v} (C[j][l] * F[tau][j]) * R[v],
-// each entry cis(sum of the angles of its terms, in term order), from
-//   C[4][64]   single-slot terms without a base condition (+angle when they
-//              require slot bit 1, -angle when they require 0),
-//   R[16]      terms on two or more slots (no lane or base condition),
-//   F[ntau][4] single-slot terms with a base condition (no lane condition),
-//   T0[ntau][64] terms without a slot condition and, for single-slot terms
-//              requiring slot bit 0, their +angle,
-// at complex offsets 0, 256, 272 and 272 + 4 ntau.  Ops whose terms do not
-// separate so (a slot term with both a lane and a base condition) keep the
-// per-term loop.
+//   Q[l] U[tau] M[tau][l] * prod_{j : bit j of v} (C[j][l] F[tau][j]) * R[v],
+// each entry cis(sum of the angles of its terms, in term order).  A term on
+// one slot j that requires s_j = 0 contributes +angle to the slot-free part
+// and -angle to slot j's factor (angle [s_j = 0] = angle - angle s_j).
+//   C[4][64]    single-slot terms without a base condition,
+//   R[16]       terms on two or more slots (no lane or base condition),
+//   F[ntau][4]  single-slot terms with a base condition (no lane condition),
+//   Q[64]       slot-free parts without a base condition,
+//   U[ntau]     slot-free parts with a base condition and no lane condition,
+//   G[ntau][8]  (kPtabG, groups of more than 2^16 tiles) slot-free parts
+//               with a base condition and a condition on one lane bit k,
+//               kept as angles (real part): lane l gets cis of
+//               the sum of G[tau][k] over its set bits k (a term requiring
+//               bit k = 0 adds +angle to U, -angle to G),
+//   M[ntau][64] the other slot-free parts with lane and base conditions
+//               (only when ntau <= 2^16; M then takes every slot-free part
+//               and Q, U are unused),
+// at complex offsets 0, 256, 272, ptab_q = 272 + 4 ntau, ptab_q + 64,
+// ptab_g = ptab_q + 64 + ntau and ptab_m.  Ops whose terms do not separate so (a slot
+// term with both a lane and a base condition) keep the per-term loop.
 constexpr uint32_t kPtabC = 0, kPtabR = 256, kPtabF = 272;
-constexpr uint32_t kPtabT0 = 1, kPtabRows = 2;
-// T0 without a lane condition (every term of T0 conditions on base bits
-// only): one entry per tile, T0[ntau], read with a scalar load
-constexpr uint32_t kPtabT0Uniform = 4;
+constexpr uint32_t kPtabQ = 1, kPtabRows = 2, kPtabU = 4, kPtabM = 8;
+// G replaces M for single-lane-bit terms when M would exceed its limit
+constexpr uint32_t kPtabG = 16;
 constexpr uint32_t kPhaseTableMinTerms = 4;     // fewer terms: the per-term loop
-constexpr uint32_t kPhaseTableMaxTauLog = 16;   // T0 of at most 2^16 x 64 entries (64 MiB)
+constexpr uint32_t kPhaseTableMaxTauLog = 20;   // F and U of at most 2^20 tiles (80 MiB)
+constexpr uint32_t kPhaseTableMaxTauLogM = 16;  // M of at most 2^16 x 64 entries (64 MiB)
 constexpr uint64_t kPhaseTableMaxEntries = 1ull << 26;  // 1 GiB of tables per program
 
 // Dense matrix structures with cheaper one-target paths: every entry real

@@ -702,42 +702,59 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], c
     const QG double* TG = (const QG double*)(cx.ptab) + 2 * op.ptab_off;
     const QC double* TU = (const QC double*)(cx.ptab) + 2 * op.ptab_off;
     auto ldg = [&](uint64_t i) { return make_double2(TG[2 * i], TG[2 * i + 1]); };
-    const uint32_t sl = op.ptab_slots;
-    double2 g0 = make_double2(1.0, 0.0), g1 = g0, g2 = g0, g3 = g0, t0 = g0;
-    if (op.ptab_flags & kPtabT0)
-      t0 = (op.ptab_flags & kPtabT0Uniform) ? ldc(TU, op.ptab_t0 + cx.tau) : ldg(op.ptab_t0 + (cx.tau << 6) + lane);
-    if (sl & 1u) g0 = ldg(kPtabC + lane);
-    if (sl & 2u) g1 = ldg(kPtabC + 64 + lane);
-    if (sl & 4u) g2 = ldg(kPtabC + 128 + lane);
-    if (KV > 8 && (sl & 8u)) g3 = ldg(kPtabC + 192 + lane);
-    const uint64_t F = kPtabF + (cx.tau << 2);
-    if (op.ptab_flags & kPtabT0) {
+    const uint32_t sl = op.ptab_slots, fl = op.ptab_flags, gb = (sl >> 8) & 63u;
+    // the slot-free factor: Q[lane] M[tau][lane] U[tau] prod_k G[tau][k]^(lane bit k)
+    if ((fl & (kPtabQ | kPtabU | kPtabM)) || gb) {
+      double2 t0 = make_double2(1.0, 0.0);
+      if (fl & kPtabQ) t0 = ldg(op.ptab_q + lane);
+      if (fl & kPtabM) {
+        const double2 m = ldg(op.ptab_m + (cx.tau << 6) + lane);
+        t0 = (fl & kPtabQ) ? cmul(t0, m) : m;
+      }
+      if (fl & kPtabU) t0 = cmul(t0, ldc(TU, op.ptab_q + 64 + cx.tau));
+      if (gb) {
+        // G holds angles: the lane's factor is cis of the sum over its set bits
+        const QC double* G = TU + 2 * (op.ptab_g + (cx.tau << 3));
+        double ga = 0.0;
+#pragma unroll
+        for (uint32_t k = 0; k < 6; ++k)
+          if (gb & (1u << k)) ga += ((lane >> k) & 1u) ? G[2 * k] : 0.0;
+        t0 = cmul(t0, cis_reduced(ga));
+      }
 #pragma unroll
       for (int v = 0; v < KV; ++v) amp[v] = cmul(amp[v], t0);
     }
+    // slot factors, one at a time (each table read right before its use
+    // keeps the registers of one factor live)
+    const uint64_t F = kPtabF + (cx.tau << 2);
     if (sl & 1u) {
-      const double2 f = cmul(g0, ldc(TU, F + 0));
+      __builtin_amdgcn_sched_barrier(0);
+      const double2 f = cmul(ldg(kPtabC + lane), ldc(TU, F + 0));
 #pragma unroll
       for (int v = 1; v < KV; v += 2) amp[v] = cmul(amp[v], f);
     }
     if (sl & 2u) {
-      const double2 f = cmul(g1, ldc(TU, F + 1));
+      __builtin_amdgcn_sched_barrier(0);
+      const double2 f = cmul(ldg(kPtabC + 64 + lane), ldc(TU, F + 1));
 #pragma unroll
       for (int v = 0; v < KV; ++v)
         if (v & 2) amp[v] = cmul(amp[v], f);
     }
     if (sl & 4u) {
-      const double2 f = cmul(g2, ldc(TU, F + 2));
+      __builtin_amdgcn_sched_barrier(0);
+      const double2 f = cmul(ldg(kPtabC + 128 + lane), ldc(TU, F + 2));
 #pragma unroll
       for (int v = 0; v < KV; ++v)
         if (v & 4) amp[v] = cmul(amp[v], f);
     }
     if (KV > 8 && (sl & 8u)) {
-      const double2 f = cmul(g3, ldc(TU, F + 3));
+      __builtin_amdgcn_sched_barrier(0);
+      const double2 f = cmul(ldg(kPtabC + 192 + lane), ldc(TU, F + 3));
 #pragma unroll
       for (int v = 8; v < KV; ++v) amp[v] = cmul(amp[v], f);
     }
-    if (op.ptab_flags & kPtabRows) {
+    __builtin_amdgcn_sched_barrier(0);
+    if (fl & kPtabRows) {
 #pragma unroll
       for (int v = 0; v < KV; ++v) amp[v] = cmul(amp[v], ldc(TU, kPtabR + v));
     }
@@ -1387,22 +1404,19 @@ __global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kerne
 
 // Phase tables of one elementwise op (layout beside kPtabC in
 // qsim_internal.h): one entry per thread, the angles of the op's matching
-// terms summed in term order and turned into cis().  A term on one slot j
-// that requires bit j = 0 adds its angle to T0 and subtracts it from slot
-// j's factor (angle * [s_j = 0] = angle - angle * s_j).
+// terms summed in term order and turned into cis().
 __global__ __launch_bounds__(kThreads) void phase_table_kernel(const PassArgs* __restrict__ pa,
                                                                const Op* __restrict__ opp, uint64_t ntau,
-                                                               double2* __restrict__ ptab) {
+                                                               uint64_t total, double2* __restrict__ ptab) {
   const PassArgs& a = *pa;
   const Op& op = *opp;
   const uint64_t e = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  const uint32_t ls = (op.ptab_flags & kPtabT0Uniform) ? 0u : 6u;  // log2 T0 entries per tile
-  const uint64_t total = op.ptab_t0 + (ntau << ls);
   if (e >= total) return;
   const TileMapDev tm{a.t, 6u, a.t - 6u, a.E};
-  enum { K_C, K_R, K_F, K_T0 } kind;
+  enum { K_C, K_R, K_F, K_Q, K_U, K_G, K_M } kind;
   uint32_t lane = 0, j = 0, v = 0;
   uint64_t base = 0;
+  const uint64_t qo = op.ptab_q, uo = qo + 64, go = op.ptab_g, mo = op.ptab_m;
   if (e < kPtabR) {
     kind = K_C;
     j = (uint32_t)(e >> 6);
@@ -1410,37 +1424,58 @@ __global__ __launch_bounds__(kThreads) void phase_table_kernel(const PassArgs* _
   } else if (e < kPtabF) {
     kind = K_R;
     v = (uint32_t)(e - kPtabR);
-  } else if (e < op.ptab_t0) {
+  } else if (e < qo) {
     kind = K_F;
     j = (uint32_t)((e - kPtabF) & 3u);
     base = tm.base((e - kPtabF) >> 2);
+  } else if (e < uo) {
+    kind = K_Q;
+    lane = (uint32_t)(e - qo);
+  } else if (e < go) {
+    kind = K_U;
+    base = tm.base(e - uo);
+  } else if (e < mo) {
+    kind = K_G;
+    j = (uint32_t)((e - go) & 7u);  // the lane bit
+    base = tm.base((e - go) >> 3);
   } else {
-    kind = K_T0;
-    lane = (uint32_t)((e - op.ptab_t0) & ((1u << ls) - 1u));
-    base = tm.base((e - op.ptab_t0) >> ls);
+    kind = K_M;
+    lane = (uint32_t)((e - mo) & 63u);
+    base = tm.base((e - mo) >> 6);
   }
   double s = 0.0;
   for (uint32_t i = 0; i < op.nterms; ++i) {
     const PhaseTerm& T = a.terms[op.term_off + i];
+    if (T.bval & ~T.bmask) continue;  // padding: never matches
     const uint32_t lm = T.tmask & 63u, lv = T.tval & 63u;
     const uint32_t rm = (T.tmask >> 6) & 15u, rv = (T.tval >> 6) & 15u;
     const bool lane_ok = (lane & lm) == lv, base_ok = (base & T.bmask) == T.bval;
-    if (rm == 0u) {
-      if (kind == K_T0 && lane_ok && base_ok) s += T.angle;
-    } else if (rm & (rm - 1u)) {
+    // the slot-free part of a term (rm == 0, or a single-slot term's s_j = 0
+    // share) goes to Q, U or M by its conditions
+    const bool slot_free = rm == 0u || (!(rm & (rm - 1u)) && rv == 0u);
+    if (slot_free) {
+      if ((op.ptab_flags & kPtabG) && lm && T.bmask && !(lm & (lm - 1u))) {
+        // one lane bit k and a base condition: G[tau][k] (+angle for bit
+        // k = 1; for bit k = 0 +angle to U and -angle to G)
+        if (kind == K_G && (uint32_t)(__ffs(lm) - 1) == j && base_ok) s += lv ? T.angle : -T.angle;
+        if (kind == K_U && !lv && base_ok) s += T.angle;
+      } else {
+        const int fk = (op.ptab_flags & kPtabM) ? K_M : (lm && T.bmask) ? K_M : (T.bmask ? K_U : K_Q);
+        if (kind == fk && lane_ok && base_ok) s += T.angle;
+      }
+    }
+    if (rm & (rm - 1u)) {
       if (kind == K_R && (v & rm) == rv) s += T.angle;
-    } else {
-      const uint32_t js = __ffs(rm) - 1;
-      if (kind == K_T0) {
-        if (rv == 0u && lane_ok && base_ok) s += T.angle;
-      } else if (js == j) {
+    } else if (rm) {
+      if ((uint32_t)(__ffs(rm) - 1) == j) {
         const double sa = rv ? T.angle : -T.angle;
         if (kind == K_C && T.bmask == 0ull && lane_ok) s += sa;
         if (kind == K_F && T.bmask != 0ull && base_ok) s += sa;
       }
     }
   }
-  ptab[op.ptab_off + e] = cis_reduced(s);
+  // G entries stay angles (the kernel adds a lane's angles, then takes cis)
+  ptab[op.ptab_off + e] = kind == K_G ? make_double2(s, 0.0) : cis_reduced(s);
 }
 
 hipError_t launch_phase_table(const PassArgs* d_args, const Op* d_op, uint64_t ntau, uint64_t entries, double* ptab,
@@ -1448,7 +1483,7 @@ hipError_t launch_phase_table(const PassArgs* d_args, const Op* d_op, uint64_t n
   const uint64_t groups = (entries + kThreads - 1) / kThreads;
   if (groups == 0) return hipSuccess;
   if (groups > kMaxLaunchBlocks) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(phase_table_kernel, dim3((unsigned)groups), dim3(kThreads), 0, st, d_args, d_op, ntau,
+  hipLaunchKernelGGL(phase_table_kernel, dim3((unsigned)groups), dim3(kThreads), 0, st, d_args, d_op, ntau, entries,
                      reinterpret_cast<double2*>(ptab));
   return hipGetLastError();
 }

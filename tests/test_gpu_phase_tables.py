@@ -135,12 +135,20 @@ def test_tables_random_circuits(n, rng, monkeypatch):
     assert np.max(np.abs(s.get_vector() - O.apply_circuit(psi, c.gate_list))) < TOL
 
 
-def test_tables_large_group_qft22(rng, monkeypatch, capfd):
-    """QFT-22 on 1024-amplitude tiles: 4096 tiles per group, so the per-tile
-    table holds 2^18 entries; tables against the per-term loop."""
-    n = 22
+@pytest.mark.parametrize("n", [22, 27])
+def test_tables_large_group_qft(n, rng, monkeypatch, capfd):
+    """QFT-n with an HEA layer on 1024-amplitude tiles: 2^12 tiles per group
+    (a per-tile lane table of 2^18 entries) and 2^17 tiles (past the lane
+    table's limit: terms on one lane bit and base bits take the per-tile
+    lane-bit angles, the rest the per-term loop); tables against the
+    per-term loop."""
     monkeypatch.setenv("QOMPSIM_PLAN_LOG", "1")
     c = W.qft_on_product(n, seed=5)
+    for q in range(n):
+        c.add_RY_gate(q, rng.uniform(0, 6.3))
+        c.add_RZ_gate(q, rng.uniform(0, 6.3))
+    for q in range(n - 1):
+        c.add_CZ_gate(q, q + 1)
     s1, s2 = QuantumState(n), QuantumState(n)
     c.update_quantum_state(s1)
     assert _table_ops(capfd.readouterr().err) > 0
