@@ -512,6 +512,7 @@ def qaoa_batch_bench(count=512, p=4, reps=3):
         assert np.array_equal(e, e0) and np.array_equal(g, g0)
     t = min(times)
     amps = sum(2.0 ** n for n, _, _, _ in insts)
+    traffic, n_persistent = b.traffic()
     err = 0.0
     try:
         import pyoracle as O
@@ -532,6 +533,13 @@ def qaoa_batch_bench(count=512, p=4, reps=3):
         "instances_per_s": count / t,
         "gradients_per_s": count * 2 * p / t,
         "state_amplitudes_per_s": amps / t,
+        "persistent_kernel_instances": n_persistent,
+        "roofline": {"bound": "hbm", "achieved": traffic / t / 1e9, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": traffic / t / 1e9 / HBM_PEAK_GBPS, "traffic": None,
+                     "algorithmic_bytes_per_batch": traffic,
+                     "basis": "per instance of n qubits 16*2^n for |0>, 32*2^n per forward pass and for "
+                              "lambda = H psi, 64*2^n per reverse pass (qsim_batch_traffic), over the "
+                              "graph replay's wall time"},
         "first_call_ms": t_first * 1e3,
         "tune_s": t_tune,
         "plan_add_s": t_add,
@@ -746,7 +754,8 @@ def main():
         result["batch"] = qaoa_batch_bench(reps=3)
         b = result["batch"]
         log(f"QAOA batch: {b['batch_ms']:.0f} ms, {b['instances_per_s']:.0f} instances/s, "
-            f"{b['gradients_per_s']:.0f} gradients/s")
+            f"{b['gradients_per_s']:.0f} gradients/s, {b['roofline']['achieved']:.0f} GB/s algorithmic "
+            f"({b['persistent_kernel_instances']} instances in the persistent kernel)")
 
     if not args.no_cpu:
         ncpu = min(args.cpu_qubits, n)

@@ -58,6 +58,8 @@ uint32_t block_scan_tiles(uint64_t namps);
 hipError_t launch_block_sample(const double2* a, uint64_t namps, double* cdf_local, double* tile_sum, double* tile_incl,
                                const double* u, uint32_t nshots, uint64_t base, unsigned long long* out,
                                hipStream_t st);
+hipError_t launch_batch_persistent(const BatchJob* jobs, uint32_t njobs, unsigned* next, unsigned groups,
+                                   hipStream_t st);
 // waves of the persistent DEC->ENC grid over ntiles tiles
 uint64_t wave_persistent_waves(uint64_t ntiles);
 hipError_t launch_phase_table(const PassArgs* d_args, const Op* d_op, uint64_t ntau, uint64_t entries, double* ptab,
@@ -2532,9 +2534,12 @@ static std::vector<LGate> cluster_gradient_points(const std::vector<LGate>& lg) 
   return out;
 }
 
-static void plan_reverse(Program& P, const std::vector<LGate>& lg0, uint32_t n, uint32_t tb, bool wave) {
+// wave_tile: 0 for the default wave form (adjoint_wave_tile), else the tile
+// bits of the wave passes
+static void plan_reverse(Program& P, const std::vector<LGate>& lg0, uint32_t n, uint32_t tb, bool wave,
+                         uint32_t wave_tile = 0) {
   const std::vector<LGate> lg = cluster_gradient_points(lg0);
-  const uint32_t wt = adjoint_wave_tile();
+  const uint32_t wt = wave_tile ? wave_tile : adjoint_wave_tile();
   if (wave && n >= wt) plan_program_engine(P, lg, n, wt, 6, false, false, true, wt);
   else plan_program_engine(P, lg, n, tb, std::min<uint32_t>(5, tb), false, false, false);
 }
@@ -3626,9 +3631,18 @@ struct BInst {
   Program fwd, rev;
   std::vector<uint32_t> map_off, map_cnt;  // per reverse pass: offset/count into dmap
   std::vector<int32_t> hmap;
-  DBuf dmap;
+  DBuf dmap, dmap_off, dmap_cnt;
   bool planned = false;
+  bool persistent = false;  // run by the persistent batch kernel
 };
+
+// Small instances (10..16 qubits, wave passes only) run in the persistent
+// batch kernel, one work-group each; QOMPSIM_BATCH_PERSISTENT=0 sends every
+// instance through its own launches.
+static bool batch_persistent_enabled() {
+  const char* e = std::getenv("QOMPSIM_BATCH_PERSISTENT");
+  return !(e && e[0] == '0');
+}
 
 struct TuneChoice {
   int fwd_engine = -1;  // -1: default choice, ENGINE_WAVE or ENGINE_LDS
@@ -3658,10 +3672,18 @@ static void plan_instance(BInst& I, const TuneChoice& tc) {
   I.map_off.clear();
   I.map_cnt.clear();
   I.hmap.clear();
+  // candidates for the persistent kernel plan their reverse sweep on the
+  // single-wave 512-amplitude tiles it runs
+  const bool want_p = batch_persistent_enabled() && I.n >= kBatchMinQubits && I.n <= kBatchMaxQubits &&
+                      tc.fwd_engine != ENGINE_LDS && tc.adj_engine != ENGINE_LDS &&
+                      (tc.adj_tb == 0 || tc.adj_tb == kWaveTileSmall) && !I.fwd.passes.empty();
+  // (the autotuner leaves these sizes at the default choice: the persistent
+  // kernel's gain is throughput over many instances, which a one-instance
+  // timing does not see)
   if (!rg.empty()) {
     const uint32_t tb = tc.adj_tb ? std::min(tc.adj_tb, I.n) : adjoint_tile_bits(I.n);
     const bool wave = tc.adj_engine == ENGINE_LDS ? false : adjoint_wave_ok(rg, I.n);
-    plan_reverse(I.rev, rg, I.n, tb, wave);
+    plan_reverse(I.rev, rg, I.n, tb, wave, want_p ? kWaveTileSmall : 0u);
     for (auto& a : I.rev.passes) {
       I.map_off.push_back((uint32_t)I.hmap.size());
       uint32_t cnt = 0;
@@ -3674,7 +3696,14 @@ static void plan_instance(BInst& I, const TuneChoice& tc) {
     }
     I.rev.upload_all();
     upload(I.dmap, I.hmap.data(), std::max<size_t>(1, I.hmap.size()) * sizeof(int32_t));
+    upload(I.dmap_off, I.map_off.data(), std::max<size_t>(1, I.map_off.size()) * sizeof(uint32_t));
+    upload(I.dmap_cnt, I.map_cnt.data(), std::max<size_t>(1, I.map_cnt.size()) * sizeof(uint32_t));
   }
+  bool ok = want_p && !rg.empty();
+  for (auto& a : I.fwd.passes) ok = ok && a.engine == ENGINE_WAVE && a.t == kWaveTile;
+  for (auto& a : I.rev.passes) ok = ok && a.engine == ENGINE_WAVE && a.t == kWaveTileSmall;
+  for (auto c : I.map_cnt) ok = ok && c <= kBatchMaxPoints;
+  I.persistent = ok;
   upload(I.dG, I.G.data(), std::max<size_t>(1, I.G.size()) * sizeof(PauliGroup));
   upload(I.dT, I.T.data(), std::max<size_t>(1, I.T.size()) * sizeof(PauliTermZ));
   HIPCHECK(hipStreamSynchronize(stream()));
@@ -3699,6 +3728,10 @@ struct qsim_batch {
   std::vector<std::unique_ptr<Lane>> lanes;
   hipEvent_t start = nullptr;
   qsim::DBuf acc;
+  // instances of the persistent kernel: their job table, psi and lambda of
+  // each (side by side), and the kernel's instance counter
+  std::vector<size_t> pjobs;
+  qsim::DBuf jobs, pstates, counter;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool ready = false;  // planned and buffers sized
@@ -3751,14 +3784,21 @@ static void batch_prepare(qsim_batch* b) {
     b->lanes.push_back(std::move(L));
   }
   if (!b->start) HIPCHECK(hipEventCreateWithFlags(&b->start, hipEventDisableTiming));
-  // largest instances first, each to the lane with the least work so far
+  // largest instances first, each to the lane with the least work so far;
+  // the persistent kernel's instances go to its job table (it runs on lane
+  // 0, which then takes other instances only when it is the only lane)
   std::vector<size_t> idx(b->inst.size());
   std::iota(idx.begin(), idx.end(), 0);
   std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return b->inst[x]->n > b->inst[y]->n; });
+  b->pjobs.clear();
+  for (size_t i : idx)
+    if (b->inst[i]->persistent) b->pjobs.push_back(i);
+  const size_t lane0 = (!b->pjobs.empty() && nl > 1) ? 1 : 0;
   std::vector<double> work(nl, 0.0);
   for (auto& L : b->lanes) L->order.clear();
   for (size_t i : idx) {
-    const size_t l = std::min_element(work.begin(), work.begin() + nl) - work.begin();
+    if (b->inst[i]->persistent) continue;
+    const size_t l = std::min_element(work.begin() + lane0, work.begin() + nl) - work.begin();
     b->lanes[l]->order.push_back(i);
     work[l] += std::ldexp(1.0, (int)b->inst[i]->n) + 65536.0;  // amplitudes plus a launch-bound floor
   }
@@ -3772,6 +3812,38 @@ static void batch_prepare(qsim_batch* b) {
     L.stage.ensure(std::max<uint64_t>(1, maxpts) * kGradReduceChunks * 16);
   }
   b->acc.ensure(b->inst.size() * acc_stride(b) * 16);
+  if (!b->pjobs.empty()) {
+    uint64_t words = 0;
+    for (size_t i : b->pjobs) words += 2ull << b->inst[i]->n;  // psi and lambda, complex entries
+    b->pstates.ensure(words * 16);
+    std::vector<BatchJob> jt;
+    double2* st = b->pstates.as<double2>();
+    for (size_t i : b->pjobs) {
+      BInst& I = *b->inst[i];
+      BatchJob J{};
+      J.n = I.n;
+      J.nfwd = (uint32_t)I.fwd.passes.size();
+      J.nrev = (uint32_t)I.rev.passes.size();
+      J.ng = (uint32_t)I.G.size();
+      J.fwd = I.fwd.dargs(0);
+      J.rev = I.rev.dargs(0);
+      J.groups = I.dG.as<PauliGroup>();
+      J.terms = I.dT.as<PauliTermZ>();
+      J.map_off = I.dmap_off.as<uint32_t>();
+      J.map_cnt = I.dmap_cnt.as<uint32_t>();
+      J.pmap = I.dmap.as<int32_t>();
+      J.psi = st;
+      J.lam = st + (1ull << I.n);
+      st += 2ull << I.n;
+      J.acc = b->acc.as<double>() + i * acc_stride(b) * 2;
+      jt.push_back(J);
+    }
+    upload(b->jobs, jt.data(), jt.size() * sizeof(BatchJob));
+    b->counter.ensure(64);
+    if (std::getenv("QOMPSIM_PLAN_LOG"))
+      fprintf(stderr, "[batch] persistent kernel instances: %zu of %zu\n", b->pjobs.size(), b->inst.size());
+    HIPCHECK(hipStreamSynchronize(stream()));
+  }
   b->ready = true;
 }
 
@@ -3810,6 +3882,16 @@ static void batch_enqueue_all(qsim_batch* b) {
   HIPCHECK(launch_fill_u64(b->acc.as<unsigned long long>(), b->inst.size() * acc_stride(b) * 2, 0ull, stream()));
   HIPCHECK(hipEventRecord(b->start, stream()));
   for (size_t l = 1; l < b->lanes.size(); ++l) HIPCHECK(hipStreamWaitEvent(b->lanes[l]->st, b->start, 0));
+  if (!b->pjobs.empty()) {
+    // two work-groups per CU at most (256 VGPRs a wave), no more than jobs
+    int dev = 0, cus = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const unsigned groups = (unsigned)std::min<uint64_t>(b->pjobs.size(), 2ull * (uint64_t)std::max(cus, 1));
+    HIPCHECK(launch_fill_u8(b->counter.as<uint8_t>(), sizeof(unsigned), 0, stream()));
+    HIPCHECK(launch_batch_persistent(b->jobs.as<BatchJob>(), (uint32_t)b->pjobs.size(), b->counter.as<unsigned>(),
+                                     groups, stream()));
+  }
   for (auto& L : b->lanes)
     for (size_t i : L->order) batch_enqueue(b, i, *L);
   for (size_t l = 1; l < b->lanes.size(); ++l) {
@@ -3896,6 +3978,14 @@ int qsim_batch_tune(qsim_batch* b, int reps) {
   for (auto& kv : first) {
     const uint32_t n = kv.first;
     BInst& I = *b->inst[kv.second];
+    if (batch_persistent_enabled() && n >= kBatchMinQubits && n <= kBatchMaxQubits) {
+      TuneChoice p{};
+      p.fwd_engine = ENGINE_WAVE;
+      p.adj_engine = ENGINE_WAVE;
+      p.adj_tb = kWaveTileSmall;
+      b->tune[n] = p;
+      continue;
+    }
     std::vector<TuneChoice> fwd_c, adj_c;
     {
       TuneChoice w;
@@ -4016,6 +4106,23 @@ int qsim_batch_run(qsim_batch* b, int use_graph, double* energies, double* grads
     HIPCHECK(hipGraphLaunch(b->exec, stream()));
   }
   batch_read(b, energies, grads);
+  API_END
+}
+
+int qsim_batch_traffic(qsim_batch* b, uint64_t* bytes, uint64_t* persistent) {
+  API_BEGIN
+  require(b != nullptr && bytes != nullptr, "null argument");
+  if (!b->ready && !b->inst.empty()) {
+    batch_prepare(b);
+    b->drop_graph();
+  }
+  uint64_t t = 0;
+  for (auto& I : b->inst) {
+    const uint64_t dim = 1ull << I->n;
+    t += 16 * dim + 32 * dim * (I->fwd.passes.size() + 1) + 64 * dim * I->rev.passes.size();
+  }
+  *bytes = t;
+  if (persistent) *persistent = b->pjobs.size();
   API_END
 }
 

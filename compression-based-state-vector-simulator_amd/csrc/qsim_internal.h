@@ -210,6 +210,28 @@ struct PassDyn {
   uint64_t tile0;           // first tile of this launch
 };
 
+// One instance of the persistent batch kernel: a small circuit run start to
+// end by one work-group (forward passes on 1024-amplitude wave tiles,
+// lambda = H psi with <psi|lambda>, then the reverse passes on
+// 512-amplitude single-wave tiles with their gradient points), its work-group
+// taking the next instance from a counter when done.
+struct BatchJob {
+  uint32_t n, nfwd, nrev, ng;
+  const PassArgs* fwd;
+  const PassArgs* rev;
+  const PauliGroup* groups;
+  const PauliTermZ* terms;
+  const uint32_t* map_off;  // per reverse pass: first of its gradient points in pmap
+  const uint32_t* map_cnt;  // per reverse pass: number of gradient points
+  const int32_t* pmap;      // gradient point -> parameter
+  double2* psi;
+  double2* lam;
+  double* acc;              // [0, 1] <psi|lambda> (compensated), [2 + 2p, 3 + 2p] gradient sums
+};
+constexpr uint32_t kBatchMaxPoints = 256;  // gradient points of one reverse pass
+constexpr uint32_t kBatchMaxQubits = 16;   // largest instance of the persistent kernel
+constexpr uint32_t kBatchMinQubits = 10;   // one 1024-amplitude forward tile
+
 __host__ __device__ inline uint64_t desc_pack(uint64_t off_words, uint32_t w, int e) {
   return (off_words & ((1ull << 40) - 1)) | ((uint64_t)(w & 0xff) << 40) |
          ((uint64_t)(uint16_t)(int16_t)e << 48);

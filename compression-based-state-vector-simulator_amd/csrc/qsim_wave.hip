@@ -1198,21 +1198,17 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
 // Re sum conj(lambda) (i P psi) over the tile from the Pauli partners (row
 // swaps, lane shuffles, or the wave's LDS buffer for mixed flips) and writes it
 // to partials[(gidx * ntiles + tile) * 2].
-__global__ __launch_bounds__(kThreads, 2) void wave_adjoint_kernel(const PassArgs* __restrict__ pa,
-                                                                   double2* __restrict__ psi,
-                                                                   double2* __restrict__ lam,
-                                                                   double* __restrict__ partials, uint64_t tile0,
-                                                                   uint64_t ntiles) {
+// One 512-amplitude tile wt of a single-wave adjoint pass: psi rows 0..7,
+// lambda rows 8..15; sink(gidx, tot) receives every gradient point's tile
+// sum (the same value in every lane).
+template <class Sink>
+__device__ __forceinline__ void adjoint_tile(const QC PassArgs& a, uint64_t wt, double2* __restrict__ psi,
+                                             double2* __restrict__ lam, double2* xb, int lane, Sink&& sink) {
   constexpr int R = kV / 2;  // rows per state
-  __shared__ __align__(16) double2 xbuf[kThreads / 64][4 * 128 + 32];
-  const QC PassArgs& a = *(const QC PassArgs*)(pa);
   const QC Op* ops = (const QC Op*)(a.ops);
   const QC PhaseTerm* terms = (const QC PhaseTerm*)(a.terms);
   const QC DiagTable* diags = (const QC DiagTable*)(a.diags);
   const QC double* coefs = (const QC double*)(a.coefs);
-  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const uint64_t wt = tile0 + (uint64_t)blockIdx.x * (kThreads / 64) + wave;
-  if (wt >= ntiles) return;
   const TileMapConst tm{9u, 6u, 3u, a.E};
   const uint64_t base = tm.base(wt);
   const uint64_t e0 = 1ull << a.E[0], e1 = 1ull << a.E[1], e2 = 1ull << a.E[2];
@@ -1228,7 +1224,6 @@ __global__ __launch_bounds__(kThreads, 2) void wave_adjoint_kernel(const PassArg
       amp[R + v] = make_double2(sl[2 * roff(v)], sl[2 * roff(v) + 1]);
     }
   }
-  double2* xb = xbuf[wave];
   for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
     const QC Op& op = ops[o];
     if (op.kind == OP_DENSE) {
@@ -1277,11 +1272,7 @@ __global__ __launch_bounds__(kThreads, 2) void wave_adjoint_kernel(const PassArg
         __builtin_amdgcn_wave_barrier();
       }
       // tile sum: the lanes' compensated sums added in a DPP tree
-      const double tot = __shfl(wave_sum_f64(s + c), 63, 64);
-      if (lane == 0) {
-        partials[((uint64_t)op.gidx * ntiles + wt) * 2] = tot;
-        partials[((uint64_t)op.gidx * ntiles + wt) * 2 + 1] = 0.0;
-      }
+      sink(op.gidx, __shfl(wave_sum_f64(s + c), 63, 64));
     }
   }
   {
@@ -1295,6 +1286,157 @@ __global__ __launch_bounds__(kThreads, 2) void wave_adjoint_kernel(const PassArg
       dl[2 * roff(v) + 1] = amp[R + v].y;
     }
   }
+}
+
+__global__ __launch_bounds__(kThreads, 2) void wave_adjoint_kernel(const PassArgs* __restrict__ pa,
+                                                                   double2* __restrict__ psi,
+                                                                   double2* __restrict__ lam,
+                                                                   double* __restrict__ partials, uint64_t tile0,
+                                                                   uint64_t ntiles) {
+  __shared__ __align__(16) double2 xbuf[kThreads / 64][4 * 128 + 32];
+  const QC PassArgs& a = *(const QC PassArgs*)(pa);
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint64_t wt = tile0 + (uint64_t)blockIdx.x * (kThreads / 64) + wave;
+  if (wt >= ntiles) return;
+  adjoint_tile(a, wt, psi, lam, xbuf[wave], lane, [&](uint32_t g, double tot) {
+    if (lane == 0) {
+      partials[((uint64_t)g * ntiles + wt) * 2] = tot;
+      partials[((uint64_t)g * ntiles + wt) * 2 + 1] = 0.0;
+    }
+  });
+}
+
+// One 1024-amplitude tile wt of a RAW->RAW wave pass (rows loaded and stored
+// directly), for the persistent batch kernel.
+__device__ __forceinline__ void raw_tile16(const QC PassArgs& a, uint64_t wt, double2* __restrict__ raw, int lane) {
+  const QC Op* ops = (const QC Op*)(a.ops);
+  const QC PhaseTerm* terms = (const QC PhaseTerm*)(a.terms);
+  const QC DiagTable* diags = (const QC DiagTable*)(a.diags);
+  const QC double* coefs = (const QC double*)(a.coefs);
+  const TileMapConst tm{10u, 6u, 4u, a.E};
+  const uint64_t base = tm.base(wt);
+  const uint64_t e0 = 1ull << a.E[0], e1 = 1ull << a.E[1], e2 = 1ull << a.E[2], e3 = 1ull << a.E[3];
+  auto roff = [&](int v) -> uint64_t {
+    return ((v & 1) ? e0 : 0ull) | ((v & 2) ? e1 : 0ull) | ((v & 4) ? e2 : 0ull) | ((v & 8) ? e3 : 0ull);
+  };
+  const WaveCtx cx{lane, base, wt, a.ptab};
+  double2 amp[16];
+  QG double* sp = (QG double*)(raw) + 2 * (base + lane);
+#pragma unroll
+  for (int v = 0; v < 16; ++v) amp[v] = make_double2(sp[2 * roff(v)], sp[2 * roff(v) + 1]);
+  for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
+    const QC Op& op = ops[o];
+    if (op.kind == OP_DENSE) wave_dense(op, amp, cx, coefs);
+    else wave_elem(op, amp, cx, terms, diags, coefs);
+  }
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    sp[2 * roff(v)] = amp[v].x;
+    sp[2 * roff(v) + 1] = amp[v].y;
+  }
+}
+
+// Persistent batch kernel (the batched adjoint of qsim_batch_run, per
+// instance ParametricQuantumCircuit::backprop semantics,
+// src/vqcsim/parametric_circuit.cpp:268-308): each work-group takes whole
+// instances from *next until none is left and runs one start to end, the
+// passes of an instance separated by work-group barriers (its tiles are all
+// in this work-group, so no other synchronisation is needed).  Every sum of
+// an instance is formed in a fixed order inside its work-group: results do
+// not depend on which work-group ran it.
+__global__ __launch_bounds__(kThreads, 2) void batch_persistent_kernel(const BatchJob* __restrict__ jobs,
+                                                                       uint32_t njobs, unsigned* __restrict__ next) {
+  __shared__ __align__(16) double2 xbuf[kThreads / 64][4 * 128 + 32];
+  __shared__ double pts[kBatchMaxPoints][kThreads / 64];
+  __shared__ double red[kThreads / 64];
+  __shared__ uint32_t sj;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  constexpr uint32_t W = kThreads / 64;
+  for (uint32_t q = threadIdx.x; q < kBatchMaxPoints * W; q += kThreads) pts[q / W][q % W] = 0.0;
+  for (;;) {
+    if (threadIdx.x == 0) sj = atomicAdd(next, 1u);
+    __syncthreads();
+    const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)sj);
+    __syncthreads();
+    if (j >= njobs) return;  // uniform over the work-group
+    const QC BatchJob& J = *(const QC BatchJob*)(jobs + j);
+    const uint64_t dim = 1ull << J.n;
+    double2* psi = J.psi;
+    double2* lam = J.lam;
+    for (uint64_t x = threadIdx.x; x < dim; x += kThreads) psi[x] = make_double2(x == 0 ? 1.0 : 0.0, 0.0);
+    __syncthreads();
+    for (uint32_t p = 0; p < J.nfwd; ++p) {
+      const QC PassArgs& a = *(const QC PassArgs*)(J.fwd + p);
+      for (uint64_t wt = (uint64_t)wave; wt < (dim >> 10); wt += W) raw_tile16(a, wt, psi, lane);
+      __syncthreads();
+    }
+    // lambda = H psi (Observable::apply_to_state) and <psi|lambda>
+    {
+      const QC PauliGroup* groups = (const QC PauliGroup*)(J.groups);
+      const QC PauliTermZ* terms = (const QC PauliTermZ*)(J.terms);
+      double s = 0.0, c = 0.0;
+      for (uint64_t x = threadIdx.x; x < dim; x += kThreads) {
+        double re = 0.0, im = 0.0;
+        for (uint32_t g = 0; g < J.ng; ++g) {
+          const QC PauliGroup& G = groups[g];
+          const uint64_t y = x ^ G.flip;
+          double fr = 0.0, fi = 0.0;
+          for (uint32_t t = G.t0; t < G.t0 + G.nt; ++t) {
+            const bool neg = __popcll(y & terms[t].z) & 1;
+            fr += neg ? -terms[t].cr : terms[t].cr;
+            fi += neg ? -terms[t].ci : terms[t].ci;
+          }
+          const double2 v = psi[y];
+          re += fr * v.x - fi * v.y;
+          im += fr * v.y + fi * v.x;
+        }
+        lam[x] = make_double2(re, im);
+        const double2 ax = psi[x];
+        dd_acc(s, c, ax.x * re + ax.y * im);
+      }
+      const double tot = __shfl(wave_sum_f64(s + c), 63, 64);
+      if (lane == 0) red[wave] = tot;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double e = 0.0, ec = 0.0;
+        for (uint32_t w = 0; w < W; ++w) dd_acc(e, ec, red[w]);
+        J.acc[0] = e;
+        J.acc[1] = ec;
+      }
+      __syncthreads();
+    }
+    for (uint32_t p = 0; p < J.nrev; ++p) {
+      const QC PassArgs& a = *(const QC PassArgs*)(J.rev + p);
+      for (uint64_t wt = (uint64_t)wave; wt < (dim >> 9); wt += W)
+        adjoint_tile(a, wt, psi, lam, xbuf[wave], lane, [&](uint32_t g, double tot) {
+          if (lane == 0) pts[g][wave] += tot;
+        });
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const uint32_t off = J.map_off[p], cnt = J.map_cnt[p];
+        for (uint32_t g = 0; g < cnt; ++g) {
+          double v = 0.0;
+          for (uint32_t w = 0; w < W; ++w) {
+            v += pts[g][w];
+            pts[g][w] = 0.0;
+          }
+          double* acc = J.acc + 2 + 2 * (uint64_t)J.pmap[off + g];
+          double hi = acc[0], lo = acc[1];
+          dd_acc(hi, lo, v);
+          acc[0] = hi;
+          acc[1] = lo;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+hipError_t launch_batch_persistent(const BatchJob* jobs, uint32_t njobs, unsigned* next, unsigned groups,
+                                   hipStream_t st) {
+  if (njobs == 0 || groups == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_persistent_kernel, dim3(groups), dim3(kThreads), 0, st, jobs, njobs, next);
+  return hipGetLastError();
 }
 
 // Paired form on the forward engine's 1024-amplitude tiles: a workgroup of

@@ -100,6 +100,7 @@ _SIGS = {
     "qsim_batch_run": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "qsim_batch_size": (_u64, [_vp]),
     "qsim_batch_max_params": (_u64, [_vp]),
+    "qsim_batch_traffic": (ctypes.c_int, [_vp, _P(_u64), _P(_u64)]),
     "qsim_sv_backprop": (ctypes.c_int, [_vp, _vp, _u64, _vp, _u64, _u64, _vp, _vp, _vp, _vp, _vp, _u64, _P(_dbl), _vp]),
     "qsim_adj_create": (ctypes.c_int, [_P(_vp)]),
     "qsim_adj_destroy": (ctypes.c_int, [_vp]),

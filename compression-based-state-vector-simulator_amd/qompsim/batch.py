@@ -48,6 +48,12 @@ class BatchedAdjoint:
         return {"forward_engine": names.get(e.value, str(e.value)), "forward_tile_bits": f.value,
                 "adjoint_engine": names.get(ae.value, str(ae.value)), "adjoint_tile_bits": a.value}
 
+    def traffic(self):
+        """(algorithmic HBM bytes of one run, instances in the persistent kernel)"""
+        t, k = ctypes.c_uint64(), ctypes.c_uint64()
+        call("qsim_batch_traffic", self._h, ctypes.byref(t), ctypes.byref(k))
+        return t.value, k.value
+
     def run(self, use_graph=True):
         """(energies[n_instances], grads[n_instances, max_params])"""
         n = lib.qsim_batch_size(self._h)

@@ -287,7 +287,11 @@ int qsim_synchronize(void);
  * Pauli-sum observable: energy and adjoint gradient of every instance (the
  * semantics of qsim_sv_backprop from |0...0>).  qsim_batch_run plans every
  * instance once and, with use_graph, captures the whole batch into one
- * hipGraph on the first call and replays it afterwards.  qsim_batch_tune
+ * hipGraph on the first call and replays it afterwards.  Instances of 10 to
+ * 16 qubits run in one persistent kernel launch (a work-group per instance,
+ * each taking the next from a counter; QOMPSIM_BATCH_PERSISTENT=0 gives
+ * them launches of their own), larger ones in launches of their own on
+ * forked streams.  qsim_batch_tune
  * times the forward engine / tile choices and the adjoint tile sizes per qubit
  * count and keeps the fastest.  No counterpart in the reference, which runs
  * one circuit per call (the batch entry points the boundary adds for this
@@ -307,6 +311,13 @@ int qsim_batch_config(const qsim_batch* b, uint32_t n_qubits, int32_t* fwd_engin
 int qsim_batch_run(qsim_batch* b, int use_graph, double* energies, double* grads);
 uint64_t qsim_batch_size(const qsim_batch* b);
 uint64_t qsim_batch_max_params(const qsim_batch* b);
+/* Algorithmic HBM bytes of one qsim_batch_run (plans the instances first if
+ * needed): per instance of n qubits 16 * 2^n for |0...0>, 32 * 2^n per
+ * forward pass and for lambda = H psi, 64 * 2^n per reverse pass (psi and
+ * lambda).  *persistent receives the number of instances the persistent
+ * kernel runs.  No counterpart in the reference: the measurement of the
+ * batched path's roofline. */
+int qsim_batch_traffic(qsim_batch* b, uint64_t* bytes, uint64_t* persistent);
 
 #ifdef __cplusplus
 }

@@ -68,3 +68,40 @@ def test_batch_autotuner_keeps_results(rng):
     _check(e, g, ref)
     e2, g2 = b.run(use_graph=True)
     assert np.array_equal(e, e2) and np.array_equal(g, g2)
+
+
+def test_batch_persistent_kernel_matches_launches_and_oracle(rng, monkeypatch, capfd):
+    """Instances of 10..16 qubits run in the persistent batch kernel (one
+    work-group per instance, taken from a counter), an 18-qubit instance
+    through its own launches beside it: energies and gradients against the
+    oracle's backprop (1e-10) and the per-instance launches
+    (QOMPSIM_BATCH_PERSISTENT=0, 1e-12); graph replays equal the eager run."""
+    monkeypatch.setenv("QOMPSIM_PLAN_LOG", "1")
+    insts = _instances(rng, sizes=(10, 12, 16, 11, 14, 13, 18, 12, 10), p=3)
+    ref = _reference(insts)
+    b = _build(insts)
+    e, g = b.run(use_graph=False)
+    assert "[batch] persistent kernel instances: 8 of 9" in capfd.readouterr().err
+    _check(e, g, ref)
+    for _ in range(3):
+        s = QuantumState(10)
+        s.get_vector()
+        e2, g2 = b.run(use_graph=True)
+        assert np.array_equal(e, e2) and np.array_equal(g, g2)
+    monkeypatch.setenv("QOMPSIM_BATCH_PERSISTENT", "0")
+    b0 = _build(insts)
+    e0, g0 = b0.run(use_graph=False)
+    assert "persistent kernel" not in capfd.readouterr().err
+    assert np.max(np.abs(e - e0)) < 1e-12 and np.max(np.abs(g - g0)) < 1e-12
+
+
+def test_batch_persistent_many_instances(rng):
+    """96 instances of 10..13 qubits, more than one per work-group of the
+    persistent grid on a small device share: every result against the
+    oracle."""
+    sizes = [10 + (i % 4) for i in range(96)]
+    insts = _instances(rng, sizes=sizes, p=2)
+    ref = _reference(insts)
+    b = _build(insts)
+    e, g = b.run(use_graph=True)
+    _check(e, g, ref)
