@@ -58,6 +58,8 @@ uint32_t block_scan_tiles(uint64_t namps);
 hipError_t launch_block_sample(const double2* a, uint64_t namps, double* cdf_local, double* tile_sum, double* tile_incl,
                                const double* u, uint32_t nshots, uint64_t base, unsigned long long* out,
                                hipStream_t st);
+hipError_t launch_gather_desc(const uint64_t* desc, const uint32_t* bmap, uint64_t nblk, uint64_t cpb, uint64_t* out,
+                              hipStream_t st);
 hipError_t launch_batch_persistent(const BatchJob* jobs, uint32_t njobs, unsigned* next, unsigned groups,
                                    hipStream_t st);
 // waves of the persistent DEC->ENC grid over ntiles tiles
@@ -1094,7 +1096,10 @@ struct qsim_cs {
   qsim_cs_config cfg{};
   std::vector<HGate> gates;
   std::vector<CStage> stages;
-  DBuf desc[2], arena[2], cursors[2], cstats, subbase[2];
+  // descriptor tables: the double store's input and output; the region store
+  // keeps one (desc[0], encoded in place) and reads a stage batch's input
+  // descriptors from a copy in desc_stage
+  DBuf desc[2], desc_stage, arena[2], cursors[2], cstats, subbase[2];
   std::vector<uint64_t> h_subbase[2];
   uint64_t cap_words[2] = {0, 0};
   uint32_t nsub[2] = {kSubArenas, kSubArenas};
@@ -1120,10 +1125,13 @@ struct qsim_cs {
 };
 
 namespace qsim {
+// The descriptor table of store half `which` (the region store has one)
+static DBuf& cs_desc(qsim_cs* cs, int which) { return cs->regions ? cs->desc[0] : cs->desc[which]; }
+
 // Device bytes the store holds now: descriptor tables, staging buffer and
 // arenas (region store: mapped units, the offset-0 unit included).
 static uint64_t cs_held_bytes(const qsim_cs* cs) {
-  uint64_t b = cs->desc[0].bytes + cs->desc[1].bytes + cs->scratch.bytes;
+  uint64_t b = cs->desc[0].bytes + cs->desc[1].bytes + cs->desc_stage.bytes + cs->scratch.bytes;
   if (cs->regions) b += cs->rs.mapped_bytes + (cs->rs.zero_mapped ? cs->rs.unit : 0);
   else b += cs->arena[0].bytes + cs->arena[1].bytes;
   return b;
@@ -1283,8 +1291,8 @@ static bool cs_check_overflow(qsim_cs* cs, int out) {
 static PassDyn cs_dyn(qsim_cs* cs, int out) {
   PassDyn dy{};
   dy.null_in = cs->null_flags.as<uint8_t>();
-  dy.desc_in = cs->desc[cs->cur].as<uint64_t>();
-  dy.desc_out = cs->desc[out].as<uint64_t>();
+  dy.desc_in = cs_desc(cs, cs->cur).as<uint64_t>();
+  dy.desc_out = cs_desc(cs, out).as<uint64_t>();
   if (cs->regions) {
     dy.arena_in = reinterpret_cast<const uint64_t*>(cs->rs.va);
     dy.arena_out = reinterpret_cast<uint64_t*>(cs->rs.va);
@@ -1936,7 +1944,7 @@ static void cs_run_stage_regions(qsim_cs* cs, uint32_t si) {
   // per-block payload words of the input (descriptor widths)
   std::vector<unsigned long long> bb(cs->nb);
   cs->bytes_tmp.ensure(cs->nb * 8);
-  HIPCHECK(launch_block_bytes(cs->desc[in].as<uint64_t>(), cs->null_flags.as<uint8_t>(), cs->nb, cs->cpb,
+  HIPCHECK(launch_block_bytes(cs_desc(cs, in).as<uint64_t>(), cs->null_flags.as<uint8_t>(), cs->nb, cs->cpb,
                               cs->bytes_tmp.as<unsigned long long>(), stream()));
   HIPCHECK(hipMemcpyAsync(bb.data(), cs->bytes_tmp.p, cs->nb * 8, hipMemcpyDeviceToHost, stream()));
   HIPCHECK(hipStreamSynchronize(stream()));
@@ -2009,13 +2017,24 @@ static void cs_run_stage_regions(qsim_cs* cs, uint32_t si) {
     if (a1 == a0) release_pending();
     if (a1 > a0) {
       uint64_t ein = 0;
+      // QOMPSIM_REGION_TIGHT=1 starts every batch from empty pieces, so that
+      // each encode retries at the measured demand (for tests)
+      const char* tight = std::getenv("QOMPSIM_REGION_TIGHT");
       for (uint32_t r = 0; r < Kout; ++r) {
         const uint64_t e = est_in[(size_t)j * Kout + r];
         ein += e;
         est[r] = std::min<uint64_t>((uint64_t)((double)e * ratio) + 64, raw_out[(size_t)j * Kout + r]);
+        if (tight && tight[0] == '1') est[r] = 0;
       }
+      // the batch's input descriptors, copied out before its output is
+      // encoded over them (a retry reads the copy again)
+      cs->desc_stage.ensure((a1 - a0) * G * cs->cpb * 8);
+      HIPCHECK(launch_gather_desc(cs_desc(cs, in).as<uint64_t>(), cs->bmap.as<uint32_t>() + a0 * G, (a1 - a0) * G,
+                                  cs->cpb, cs->desc_stage.as<uint64_t>(), stream()));
       rs_encode(cs, out, est, [&] {
         PassDyn dy = cs_dyn(cs, out);
+        dy.desc_staged = 1;
+        dy.desc_in = cs->desc_stage.as<uint64_t>();
         if (P.passes.size() == 1) {
           dy.bmap = cs->bmap.as<uint32_t>() + a0 * G;
           dy.slot_region = cs->slot_region.as<uint32_t>() + a0;
@@ -2027,6 +2046,7 @@ static void cs_run_stage_regions(qsim_cs* cs, uint32_t si) {
           for (uint64_t b0 = a0; b0 < a1; b0 += gpb) {
             const uint64_t cnt = std::min(gpb, a1 - b0);
             dy.bmap = cs->bmap.as<uint32_t>() + b0 * G;
+            dy.desc_in = cs->desc_stage.as<uint64_t>() + (b0 - a0) * G * cs->cpb;
             dy.slot_region = cs->slot_region.as<uint32_t>() + b0;
             for (size_t p = 0; p < P.passes.size(); ++p) {
               const int im = p == 0 ? IN_DEC : IN_RAW;
@@ -2852,7 +2872,7 @@ int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** 
   sbytes = std::min<uint64_t>(sbytes, raw);
   sbytes = std::max<uint64_t>(sbytes, (1ull << B) * 16);
   // the fixed part of the budget: two descriptor tables and the staging buffer
-  const uint64_t fixed = 2 * cs->nb * cs->cpb * 8 + sbytes;
+  const uint64_t fixed = (cs->regions ? 1 : 2) * cs->nb * cs->cpb * 8 + sbytes;
   if (c.hbm_budget && fixed >= c.hbm_budget)
     throw std::invalid_argument("hbm_budget " + std::to_string(c.hbm_budget) +
                                 " is below the descriptor tables and staging buffer (" + std::to_string(fixed) +
@@ -2861,7 +2881,7 @@ int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** 
   cs->scratch_amps = sbytes / 16;
   if (cs->regions) rs_init(cs.get());
   for (int w = 0; w < 2; ++w) {
-    cs->desc[w].alloc(cs->nb * cs->cpb * 8);
+    if (w == 0 || !cs->regions) cs->desc[w].alloc(cs->nb * cs->cpb * 8);
     cs->cursors[w].alloc((size_t)kMaxSubArenas * 16 * 8);
   }
   if (!cs->regions) {
@@ -3280,7 +3300,7 @@ int qsim_cs_null_bitmap(const qsim_cs* cs, uint8_t* out) {
 int qsim_cs_block_bytes(qsim_cs* cs, uint64_t* out) {
   API_BEGIN
   cs->bytes_tmp.ensure(cs->nb * 8);
-  HIPCHECK(launch_block_bytes(cs->desc[cs->cur].as<uint64_t>(), cs->null_flags.as<uint8_t>(), cs->nb, cs->cpb,
+  HIPCHECK(launch_block_bytes(cs_desc(cs, cs->cur).as<uint64_t>(), cs->null_flags.as<uint8_t>(), cs->nb, cs->cpb,
                               cs->bytes_tmp.as<unsigned long long>(), stream()));
   HIPCHECK(hipMemcpyAsync(out, cs->bytes_tmp.p, cs->nb * 8, hipMemcpyDeviceToHost, stream()));
   HIPCHECK(hipStreamSynchronize(stream()));
@@ -3434,7 +3454,7 @@ int qsim_cs_save_checkpoint(qsim_cs* cs, const char* path, uint64_t next_stage) 
     host_to_file(fc.f, H.used.data(), K);
     host_to_file(fc.f, H.payload.data(), K);
     dev_to_file(fc.f, cs->null_flags.p, cs->nb);
-    dev_to_file(fc.f, cs->desc[c].p, cs->nb * cs->cpb * 8);
+    dev_to_file(fc.f, cs_desc(cs, c).p, cs->nb * cs->cpb * 8);
     for (uint32_t r = 0; r < K; ++r) dev_to_file(fc.f, rs_region_ptr(cs, c, r), H.used[r] * 8);
   }
   if (std::fflush(fc.f) != 0) throw std::runtime_error("checkpoint: write failed");
@@ -3553,7 +3573,7 @@ int qsim_cs_load_checkpoint(qsim_cs* cs, const char* path, uint64_t* next_stage)
       }
       cs->cur = hs;
       HIPCHECK(hipMemcpy(cs->null_flags.p, nullf.data(), cs->nb, hipMemcpyHostToDevice));
-      HIPCHECK(hipMemcpy(cs->desc[hs].p, desc.data(), desc.size() * 8, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(cs_desc(cs, hs).p, desc.data(), desc.size() * 8, hipMemcpyHostToDevice));
     }
   } catch (const std::exception& e) {
     std::string msg = std::string("checkpoint: load failed (") + e.what() + "); the store was reset to |0...0>";
@@ -3589,7 +3609,7 @@ int qsim_cs_store_bytes(const qsim_cs* cs, uint64_t* out3) {
   } else {
     out3[0] = out3[1] = cs->arena[0].bytes + cs->arena[1].bytes;
   }
-  out3[2] = cs->desc[0].bytes + cs->desc[1].bytes;
+  out3[2] = cs->desc[0].bytes + cs->desc[1].bytes + cs->desc_stage.bytes;
   API_END
 }
 

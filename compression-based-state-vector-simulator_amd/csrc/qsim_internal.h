@@ -205,6 +205,9 @@ struct PassDyn {
   uint32_t sub_mask;        // number of sub-arenas - 1 (power of two)
   uint32_t wave_subs;       // persistent DEC->ENC: wave w of the grid owns sub-arena w
                             //   (as many sub-arenas as waves; cursors[16 w] = its demand)
+  uint32_t desc_staged;     // desc_in holds the launch's blocks in bmap order: the block at
+                            //   bmap position i (from this launch's bmap) at desc_in[i cpb ..]
+  uint32_t pad_;
   uint32_t* overflow;
   uint8_t* nonzero;         // [nb]: set when a block null at start holds data
   uint64_t tile0;           // first tile of this launch

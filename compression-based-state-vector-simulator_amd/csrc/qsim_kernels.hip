@@ -216,12 +216,13 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(const PassArgs* __restri
   if constexpr (IN == IN_DEC || OUT == OUT_ENC) {
     for (uint32_t j = threadIdx.x; j < NCH; j += kThreads) {
       const uint64_t loc = base | tm.off(j << kChunkAmpsLog2);
-      const uint32_t bid = dy.bmap[(slot << a.khi) + (loc >> a.B)];
+      const uint64_t bpos = (slot << a.khi) + (loc >> a.B);
+      const uint32_t bid = dy.bmap[bpos];
       const uint32_t ci = (uint32_t)((loc & bmask) >> kChunkAmpsLog2);
       cs->bid[j] = bid;
       cs->ci[j] = ci;
       if constexpr (IN == IN_DEC) {
-        cs->desc[j] = dy.null_in[bid] ? 0ull : dy.desc_in[(uint64_t)bid * a.cpb + ci];
+        cs->desc[j] = dy.null_in[bid] ? 0ull : dy.desc_in[(dy.desc_staged ? bpos : (uint64_t)bid) * a.cpb + ci];
       }
     }
     __syncthreads();
@@ -529,6 +530,17 @@ __global__ __launch_bounds__(256) void cursor_stats_kernel(const unsigned long l
     out[0] = ss[0];
     out[1] = sm[0];
   }
+}
+
+// Descriptors of a launch's blocks in bmap order (the region store encodes
+// its output descriptors in place, so a stage reads its input from this
+// copy): out[i cpb + c] = desc[bmap[i] cpb + c].
+__global__ __launch_bounds__(256) void gather_desc_kernel(const uint64_t* __restrict__ desc,
+                                                          const uint32_t* __restrict__ bmap, uint64_t nblk,
+                                                          uint64_t cpb, uint64_t* __restrict__ out) {
+  const uint64_t n = nblk * cpb;
+  for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (uint64_t)gridDim.x * 256)
+    out[x] = desc[(uint64_t)bmap[x / cpb] * cpb + x % cpb];
 }
 
 // ------------------------------------------------ adjoint reverse sweep
@@ -969,6 +981,14 @@ hipError_t launch_cursor_stats(const unsigned long long* cur, uint32_t n, unsign
 }
 
 unsigned reduce_grid(uint64_t n) { return grid_for(n, 2048u); }
+
+hipError_t launch_gather_desc(const uint64_t* desc, const uint32_t* bmap, uint64_t nblk, uint64_t cpb, uint64_t* out,
+                              hipStream_t st) {
+  if (nblk == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_desc_kernel, dim3(grid_for(nblk * cpb, 65536u)), dim3(256), 0, st, desc, bmap, nblk, cpb,
+                     out);
+  return hipGetLastError();
+}
 
 hipError_t launch_sv_norm(const double2* a, uint64_t n, double* partials, unsigned grid, hipStream_t st) {
   hipLaunchKernelGGL((sv_reduce_kernel<0>), dim3(grid), dim3(kThreads), 0, st, a, a, n, partials);

@@ -968,13 +968,14 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   uint32_t mbid = 0, mci = 0, mnull = 1;
   if constexpr (IN == IN_DEC || OUT == OUT_ENC) {
     const uint64_t loc = base | tm.off(32u * ((uint32_t)lane / LPC));
-    mbid = bmap[(slot << a.khi) + (loc >> a.B)];
+    const uint64_t bpos = (slot << a.khi) + (loc >> a.B);
+    mbid = bmap[bpos];
     mci = (uint32_t)((loc & ((1ull << a.B) - 1ull)) >> kChunkAmpsLog2);
     mnull = null_in[mbid];
     if constexpr (IN == IN_DEC) {
       // the descriptor table spans every block, so the descriptor load need
       // not wait for the null flag (a null block's stale entry is discarded)
-      const uint64_t d = desc_in[(uint64_t)mbid * a.cpb + mci];
+      const uint64_t d = desc_in[(dy.desc_staged ? bpos : (uint64_t)mbid) * a.cpb + mci];
       mdesc = mnull ? 0ull : d;
     }
     if constexpr (OUT == OUT_ENC) {

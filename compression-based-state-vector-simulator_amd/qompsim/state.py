@@ -347,7 +347,9 @@ class CompressedQuantumState:
 
     def get_store_bytes(self):
         """Device bytes of the payload store: {"arena": held now, "arena_peak":
-        most held at once, "descriptors": the two descriptor tables}."""
+        most held at once, "descriptors": the descriptor tables (double store:
+        input and output; region store: one table, encoded in place, and the
+        copy of a stage batch's input descriptors)}."""
         out = np.zeros(3, dtype=np.uint64)
         call("qsim_cs_store_bytes", self._h, ptr(out))
         return {"arena": int(out[0]), "arena_peak": int(out[1]), "descriptors": int(out[2])}

@@ -233,7 +233,9 @@ int qsim_cs_bytes_history(const qsim_cs* cs, uint64_t* out, uint64_t cap, uint64
 int qsim_cs_max_sv_bytes(const qsim_cs* cs, uint64_t* out);
 /* Device memory of the payload store (no reference counterpart): out3[0] the
  * arena bytes held now (region store: mapped units), out3[1] the most held
- * at once, out3[2] the descriptor tables. */
+ * at once, out3[2] the descriptor tables (double store: input and output;
+ * region store: one table, encoded in place, plus the copy of a stage
+ * batch's input descriptors). */
 int qsim_cs_store_bytes(const qsim_cs* cs, uint64_t* out3);
 /* Per-stage measurement of the last run: for each executed stage
  * {gpu_seconds, bytes_in, bytes_out, decompressed_amplitudes, n_passes}. */

@@ -188,3 +188,28 @@ def test_regions_few_block_qubits(n, B):
     b.set_computational_basis(5)
     a.set_computational_basis(5)
     _same(a, b)
+
+
+@pytest.mark.parametrize("engine", ["wave", "lds"])
+def test_regions_retries_read_the_batch_copy(engine, monkeypatch):
+    """Every batch's first encode overflows (QOMPSIM_REGION_TIGHT=1): the
+    retry reads the batch's input descriptors from their copy (the region
+    store's single descriptor table is encoded in place), and the result is
+    bit-identical to the double-buffered store, for single-pass stages and
+    for stages of several passes through the staging buffer."""
+    monkeypatch.setenv("QOMPSIM_REGION_TIGHT", "1")
+    if engine == "lds":
+        monkeypatch.setenv("QOMPSIM_ENGINE", "lds")
+    n, B = 15, 7
+    c = W.qft_on_product(n, seed=12)
+    QuantumCircuitOptimizer().optimize(c, 2)
+    kw = {"tile_bits": 6} if engine == "lds" else {}
+    a, b = _pair(n, B, **kw)
+    for cs in (a, b):
+        cs.partition_and_register_circuit(c, 4)
+        cs.apply_circuit()
+    _same(a, b)
+    if engine == "lds":
+        assert max(b.get_stage_passes()) > 1
+    # one descriptor table and a batch's copy, against the double store's two
+    assert b.get_store_bytes()["descriptors"] <= a.get_store_bytes()["descriptors"]
