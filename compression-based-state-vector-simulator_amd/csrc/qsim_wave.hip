@@ -250,6 +250,34 @@ __device__ __forceinline__ void stage_or(uint64_t* stg, uint32_t bit, uint64_t c
   atomicOr(reinterpret_cast<unsigned long long*>(&stg[qw + 1]), (unsigned long long)((c >> 1) >> (63u - sh)));
 }
 
+// Pack this lane's 2 KV codes of w <= 32 bits (chunk layout: doubles d0 ..
+// d0 + 2 KV - 1 of its chunk, code of double d at bit d w) into the staging
+// row, M consecutive codes per OR (M w <= 64).  An OR that would add nothing
+// (an all-zero piece, or a piece that ends inside its first word) goes to
+// the lane's own spare word `dummy` instead: lanes of one chunk then never
+// OR into one word when their codes are zero (w = 0 chunks, whose lanes all
+// start at the same bit), and narrow codes take 2 KV / M ORs instead of KV.
+template <int KV, int M>
+__device__ __forceinline__ void pack_pieces(uint64_t* sc, uint64_t* dummy, const double2 (&amp)[KV], uint32_t d0,
+                                            uint32_t w) {
+  static_assert((2 * KV) % M == 0, "pieces tile the lane's codes");
+#pragma unroll
+  for (int p = 0; p < 2 * KV / M; ++p) {
+    uint64_t F = 0;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      const int d = p * M + i;
+      const uint32_t c = (uint32_t)__double_as_longlong((d & 1) ? amp[d >> 1].y : amp[d >> 1].x);
+      F |= i == 0 ? (uint64_t)c : (uint64_t)c << ((uint32_t)i * w);
+    }
+    const uint32_t bit = (d0 + (uint32_t)(p * M)) * w, qw = bit >> 6, sh = bit & 63u;
+    uint64_t* a0 = F != 0ull ? sc + qw : dummy;
+    uint64_t* a1 = F != 0ull && sh + (uint32_t)M * w > 64u ? sc + qw + 1 : dummy;
+    atomicOr(reinterpret_cast<unsigned long long*>(a0), (unsigned long long)(F << sh));
+    atomicOr(reinterpret_cast<unsigned long long*>(a1), (unsigned long long)(sh ? F >> (64u - sh) : 0ull));
+  }
+}
+
 // Quantise the part of a chunk held by this lane (chunk layout: a chunk
 // spans 32/KV lanes, two for KV = 16 and four for KV = 8) and replace amp by
 // the codes (zig-zag integers of rint(x * 2^-e), or the raw bit patterns when
@@ -898,8 +926,9 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   constexpr uint32_t XS = x_slots<KV>();
   // per wave: the layout transposes (XS complex128) and, reused after the
   // encoder's transpose, the packed payload of the tile (at most 128 KV
-  // words) plus slack for the zero-valued spill OR past the last field
-  static_assert(2 * XS >= 128 * KV + 8, "staging must fit the transpose buffer");
+  // words) plus slack for the zero-valued spill OR past the last field and
+  // one spare word per lane
+  static_assert(2 * XS >= 128 * KV + 8 + 64, "staging and the lanes' spare words must fit the transpose buffer");
   __shared__ __align__(16) uint64_t wbuf[kThreads / 64][2 * XS];
   const QC PassArgs& a = *(const QC PassArgs*)(pa);
   const QC Op* ops = (const QC Op*)(a.ops);
@@ -1096,11 +1125,14 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
     __builtin_amdgcn_wave_barrier();
     uint64_t* sc = st + loff;
     if (ballot64(w > 32u) == 0) {
-      // two codes of at most 32 bits form one field of at most 64 bits
-#pragma unroll
-      for (int q = 0; q < KV; ++q)
-        stage_or(sc, (d0 + 2u * q) * w,
-                 (uint64_t)__double_as_longlong(amp[q].x) | ((uint64_t)__double_as_longlong(amp[q].y) << w));
+      // codes of at most 32 bits: pieces of as many codes as the tile's
+      // widest chunk allows in 64 bits; each lane's spare word (past the
+      // staging area) takes the ORs that would add nothing
+      uint64_t* dummy = st + 128 * KV + 8 + lane;
+      if (ballot64(w > 16u)) pack_pieces<KV, 2>(sc, dummy, amp, d0, w);
+      else if (ballot64(w > 8u)) pack_pieces<KV, 4>(sc, dummy, amp, d0, w);
+      else if (ballot64(w > 4u)) pack_pieces<KV, 8>(sc, dummy, amp, d0, w);
+      else pack_pieces<KV, 16>(sc, dummy, amp, d0, w);
     } else {
 #pragma unroll
       for (int q = 0; q < KV; ++q) {
