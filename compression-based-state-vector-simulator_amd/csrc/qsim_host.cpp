@@ -355,6 +355,9 @@ struct LGate {
   uint64_t gx = 0, gz = 0;
   uint32_t nY = 0;
   bool diag = false;
+  // a trailing SWAP folded into the encoder's tile permutation: it holds its
+  // qubits in the tile while the pass is planned and emits no op
+  bool noop = false;
   std::vector<uint32_t> N;      // non-diagonal targets (matrix bit order)
   std::vector<uint32_t> S;      // selector qubits
   std::vector<cplx> mats;       // 2^|S| matrices of 2^|N| x 2^|N|
@@ -656,6 +659,7 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
   }
   for (size_t gi : order) {
     const LGate& g = lg[gi];
+    if (g.noop) continue;
     if (g.grad >= 0) {
       close_elem();
       Op op{};
@@ -895,33 +899,104 @@ static bool is_swap_gate(const LGate& g) {
   return true;
 }
 
+// LDS slots of the encoder's layout transpose for a tile whose bits move by
+// the swaps `ts` (each a pair of tile qubits, pairwise disjoint): the value
+// at tile position p = 64 v + l goes to pi(p) = 64 v' + l', whose slot is
+// RS v' + l' + (l' >> PS) (the padded layout of the wave kernel).  pi(p)
+// splits into a part from l and a part from v with disjoint bits, so the
+// slot is tperm_lane[l] + tperm_row[v].
+static void set_tile_permutation(PassArgs& a, const std::vector<std::pair<uint32_t, uint32_t>>& ts) {
+  const uint32_t RB = a.t - 6, KV = 1u << RB;
+  const uint32_t PS = KV == 16 ? 4u : 3u, RS = 64u + (64u >> PS);
+  uint32_t perm[16];
+  for (uint32_t i = 0; i < a.t; ++i) perm[i] = i;
+  auto tbit = [&](uint32_t q) -> uint32_t {
+    if (q < 6) return q;
+    for (uint32_t j = 0; j < a.nE; ++j)
+      if (a.E[j] == q) return 6 + j;
+    throw std::logic_error("planner: swapped qubit outside the tile");
+  };
+  for (auto& pr : ts) std::swap(perm[tbit(pr.first)], perm[tbit(pr.second)]);
+  auto slot = [&](uint32_t p) { const uint32_t l = p & 63u, v = p >> 6; return RS * v + l + (l >> PS); };
+  for (uint32_t l = 0; l < 64; ++l) {
+    uint32_t p = 0;
+    for (uint32_t i = 0; i < 6; ++i) p |= ((l >> i) & 1u) << perm[i];
+    a.tperm_lane[l] = (uint16_t)slot(p);
+  }
+  for (uint32_t v = 0; v < KV; ++v) {
+    uint32_t p = 0;
+    for (uint32_t j = 0; j < RB; ++j) p |= ((v >> j) & 1u) << perm[6 + j];
+    a.tperm_row[v] = (uint16_t)slot(p);
+  }
+  a.tperm = 1;
+}
+
 static void plan_program(Program& P, const std::vector<LGate>& lg0, uint32_t L, uint32_t tmax, uint32_t cmin,
                          bool at_least_one_pass, bool codec_ends = false) {
-  // Trailing SWAP gates on qubits at or above the chunk bits of a compressed
-  // program move whole chunks: they become the last (encoding) pass's chunk
-  // permutation instead of gates (wave engine; QOMPSIM_STORE_SWAPS=0 keeps
-  // them as gates).
+  // Trailing SWAP gates of a compressed program move data without
+  // arithmetic (wave engine; QOMPSIM_STORE_SWAPS=0 keeps them as gates):
+  // - on qubits at or above the chunk bits they move whole chunks and become
+  //   the last (encoding) pass's chunk permutation;
+  // - with a qubit below the chunk bits, when both qubits are tile bits of a
+  //   single-pass program, they become a permutation of the tile in the
+  //   encoder's layout transpose.  Such swaps are disjoint from every other
+  //   folded swap, so the order of the two permutations does not matter.
   std::vector<LGate> lg = lg0;
-  std::vector<std::pair<uint32_t, uint32_t>> swaps;
+  std::vector<std::pair<uint32_t, uint32_t>> swaps, tswaps;
+  std::vector<LGate> tswap_gates;
   const char* es = std::getenv("QOMPSIM_STORE_SWAPS");
   if (codec_ends && wave_engine_ok(lg0, L) && !(es && *es == '0')) {
-    while (!lg.empty() && swaps.size() < (size_t)kMaxStoreSwaps && is_swap_gate(lg.back()) &&
-           lg.back().N[0] >= (uint32_t)kChunkAmpsLog2 && lg.back().N[1] >= (uint32_t)kChunkAmpsLog2) {
-      swaps.emplace_back(lg.back().N[0], lg.back().N[1]);
+    uint64_t tq = 0, allq = 0;
+    while (!lg.empty() && is_swap_gate(lg.back())) {
+      const uint32_t qa = lg.back().N[0], qb = lg.back().N[1];
+      const uint64_t m = (1ull << qa) | (1ull << qb);
+      if (qa >= (uint32_t)kChunkAmpsLog2 && qb >= (uint32_t)kChunkAmpsLog2) {
+        if (swaps.size() >= (size_t)kMaxStoreSwaps || (m & tq)) break;
+        swaps.emplace_back(qa, qb);
+      } else {
+        if (tswaps.size() >= 6 || (m & allq)) break;
+        tswaps.emplace_back(qa, qb);
+        tswap_gates.push_back(lg.back());
+        tq |= m;
+      }
+      allq |= m;
       lg.pop_back();
     }
     std::reverse(swaps.begin(), swaps.end());
+    std::reverse(tswaps.begin(), tswaps.end());
+    std::reverse(tswap_gates.begin(), tswap_gates.end());
   }
   const bool wave = wave_engine_ok(lg0, L);
-  plan_program_engine(P, lg, L, tmax, cmin, at_least_one_pass || !swaps.empty(), codec_ends, wave);
-  if (wave && codec_ends && P.passes.size() == 1 && compressed_wave_bits() == kWaveTileSmall) {
-    Program Q;
-    plan_program_engine(Q, lg, L, tmax, cmin, at_least_one_pass || !swaps.empty(), codec_ends, true, kWaveTileSmall);
-    if (Q.passes.size() == 1) {
-      P.reset();
-      plan_program_engine(P, lg, L, tmax, cmin, at_least_one_pass || !swaps.empty(), codec_ends, true,
-                          kWaveTileSmall);
+  auto plan_all = [&](const std::vector<LGate>& G, bool need_pass) {
+    plan_program_engine(P, G, L, tmax, cmin, at_least_one_pass || need_pass, codec_ends, wave);
+    if (wave && codec_ends && P.passes.size() == 1 && compressed_wave_bits() == kWaveTileSmall) {
+      Program Q;
+      plan_program_engine(Q, G, L, tmax, cmin, at_least_one_pass || need_pass, codec_ends, true, kWaveTileSmall);
+      if (Q.passes.size() == 1) {
+        P.reset();
+        plan_program_engine(P, G, L, tmax, cmin, at_least_one_pass || need_pass, codec_ends, true, kWaveTileSmall);
+      }
     }
+  };
+  if (!tswaps.empty()) {
+    // the tile swaps reserve their qubits in the tile as op-free gates
+    std::vector<LGate> lgp = lg;
+    for (const auto& g : tswap_gates) {
+      lgp.push_back(g);
+      lgp.back().noop = true;
+    }
+    plan_all(lgp, true);
+    if (P.passes.size() == 1 && P.passes[0].engine == ENGINE_WAVE) {
+      set_tile_permutation(P.passes[0], tswaps);
+    } else {
+      // more than one pass: the tile swaps stay gates
+      P.reset();
+      lg.insert(lg.end(), tswap_gates.begin(), tswap_gates.end());
+      tswaps.clear();
+      plan_all(lg, !swaps.empty());
+    }
+  } else {
+    plan_all(lg, !swaps.empty());
   }
   if (!swaps.empty()) {
     PassArgs& last = P.passes.back();
@@ -941,7 +1016,8 @@ static void plan_program(Program& P, const std::vector<LGate>& lg0, uint32_t L, 
       }
       fprintf(stderr, "[plan]   engine=%s t=%u c=%u E=", a.engine == ENGINE_WAVE ? "wave" : "lds", a.t, a.c);
       for (uint32_t i = 0; i < a.nE; ++i) fprintf(stderr, "%u%s", a.E[i], i + 1 < a.nE ? "," : "");
-      fprintf(stderr, " dense=%u elem=%u terms=%u maxk=%u swaps=%u\n", nd, ne, nt, a.maxk, a.nswap);
+      fprintf(stderr, " dense=%u elem=%u terms=%u maxk=%u swaps=%u tile_perm=%u\n", nd, ne, nt, a.maxk, a.nswap,
+              a.tperm);
     }
   }
 }
@@ -950,7 +1026,7 @@ static void plan_program(Program& P, const std::vector<LGate>& lg0, uint32_t L, 
 static hipError_t launch_program_pass(const Program& P, size_t i, int in_mode, int out_mode, const PassDyn& dy,
                                       uint64_t groups, hipStream_t st) {
   const PassArgs& a = P.passes[i];
-  if (a.nswap && (a.engine != ENGINE_WAVE || out_mode != OUT_ENC)) return hipErrorInvalidValue;
+  if ((a.nswap || a.tperm) && (a.engine != ENGINE_WAVE || out_mode != OUT_ENC)) return hipErrorInvalidValue;
   if (a.engine == ENGINE_WAVE) {
     if (a.t != kWaveTile && a.t != kWaveTileSmall) return hipErrorInvalidValue;
     return launch_wave((int)a.t, in_mode, out_mode, P.dargs(i), dy, groups << (P.L - a.t), st);

@@ -187,6 +187,14 @@ struct PassArgs {
   // swp[2i] and swp[2i + 1] (all >= kChunkAmpsLog2): trailing SWAP gates
   uint32_t nswap;
   uint8_t swp[2 * kMaxStoreSwaps];
+  // tile permutation of an ENC store on the wave engine (trailing SWAP gates
+  // with a qubit below the chunk bits, both qubits tile bits): the value at
+  // tile position 64 v + l is placed at the position the swaps send it to,
+  // through the encoder's layout transpose, whose LDS slot for it is
+  // tperm_lane[l] + tperm_row[v]
+  uint32_t tperm;
+  uint16_t tperm_row[16];
+  uint16_t tperm_lane[64];
 };
 
 // Per-launch buffers of a pass (passed by value).

@@ -205,6 +205,21 @@ __device__ __forceinline__ void to_chunk_layout(double2 (&amp)[KV], double2* xb,
   for (int q = 0; q < KV; ++q) amp[q] = ca[q];
   __builtin_amdgcn_wave_barrier();
 }
+// the same, with the tile permuted on the way: wave-layout value (l, v) goes
+// to slot sl + srow[v] (the slot of its permuted position), and the chunk
+// layout is read as above
+template <int KV>
+__device__ __forceinline__ void to_chunk_layout_perm(double2 (&amp)[KV], double2* xb, uint32_t lane, uint32_t sl,
+                                                     const QC uint16_t* __restrict__ srow) {
+  double2* wa = xb + sl;
+#pragma unroll
+  for (int v = 0; v < KV; ++v) wa[srow[v]] = amp[v];
+  __builtin_amdgcn_wave_barrier();
+  const double2* ca = xb + (KV + 1) * lane;
+#pragma unroll
+  for (int q = 0; q < KV; ++q) amp[q] = ca[q];
+  __builtin_amdgcn_wave_barrier();
+}
 template <int KV>
 __device__ __forceinline__ void to_wave_layout(double2 (&amp)[KV], double2* xb, uint32_t lane) {
   constexpr uint32_t PS = x_pad_shift<KV>(), RS = x_row_slots<KV>();
@@ -476,7 +491,11 @@ __device__ __forceinline__ void k1_lane_uniform(const QC Op& op, double2 (&amp)[
     }
   } else {
     const uint32_t r = (lane >> B) & 1u;
-    const double2 md = ldc(C, op.mat_off + 3u * r), mo = ldc(C, op.mat_off + 1u + r);
+    // the four entries are wave-uniform (scalar loads); each lane takes its
+    // row's diagonal and off-diagonal entry
+    const double2 m0 = ldc(C, op.mat_off), m1 = ldc(C, op.mat_off + 1), m2 = ldc(C, op.mat_off + 2),
+                  m3 = ldc(C, op.mat_off + 3);
+    const double2 md = r ? m3 : m0, mo = r ? m2 : m1;
     if (op.real == kDenseReal) {
 #pragma unroll
       for (int v = 0; v < KV; ++v) amp[v] = mac2r(md.x, amp[v], mo.x, lane_xor2<B>(amp[v], lane));
@@ -550,12 +569,27 @@ __device__ __forceinline__ void k1_reg(const QC Op& op, double2 (&amp)[KV], cons
     }
     return;
   }
-  const bool uni = op_sel_uniform(op);
+  if (op_sel_uniform(op)) {
+    // the selector is the same on every lane: the matrix comes through
+    // scalar loads
+#pragma unroll
+    for (int v = 0; v < KV; ++v) {
+      if (v & MJ) continue;
+      QSIM_ITER_FENCE();
+      const uint32_t sel = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_sel(op, cx, v));
+      if ((op.ident_mask >> sel) & 1ull) continue;
+      const uint64_t mo = op.mat_off + 4ull * sel;
+      const double2 a = amp[v], b = amp[v | MJ];
+      amp[v] = mac2(ldc(C, mo), a, ldc(C, mo + 1), b);
+      amp[v | MJ] = mac2(ldc(C, mo + 2), a, ldc(C, mo + 3), b);
+    }
+    return;
+  }
 #pragma unroll
   for (int v = 0; v < KV; ++v) {
     if (v & MJ) continue;
     QSIM_ITER_FENCE();
-    const uint32_t sel = uni ? (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_sel(op, cx, v)) : wave_sel(op, cx, v);
+    const uint32_t sel = wave_sel(op, cx, v);
     if ((op.ident_mask >> sel) & 1ull) continue;
     const uint64_t mo = op.mat_off + 4ull * sel;
     const double2 a = amp[v], b = amp[v | MJ];
@@ -564,21 +598,10 @@ __device__ __forceinline__ void k1_reg(const QC Op& op, double2 (&amp)[KV], cons
   }
 }
 
-// two targets on register slots J0 (matrix bit 0) and J1 (matrix bit 1)
-template <int J0, int J1, int KV>
-__device__ __forceinline__ void k2_regreg(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
-  constexpr int M0 = 1 << J0, M1 = 1 << J1;
-  if constexpr (M0 >= KV || M1 >= KV) return;  // no such register qubit in this tile shape
-  const bool uni = op_sel_uniform(op);
-#pragma unroll
-  for (int v = 0; v < KV; ++v) {
-    if (v & (M0 | M1)) continue;
-    QSIM_ITER_FENCE();
-    // a wave-uniform selector keeps the matrix in scalar loads
-    const uint32_t sel = op.nsel == 0 ? 0u : uni ? (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_sel(op, cx, v))
-                                                 : wave_sel(op, cx, v);
-    if ((op.ident_mask >> sel) & 1ull) continue;
-    const uint64_t mo = op.mat_off + 16ull * sel;
+// the 4x4 product on rows v, v | M0, v | M1, v | M0 | M1 (matrix at mo)
+template <int M0, int M1, int KV>
+__device__ __forceinline__ void k2_regreg_rows(double2 (&amp)[KV], int v, const QC double* __restrict__ C, uint64_t mo) {
+  {
     const double2 x[4] = {amp[v], amp[v | M0], amp[v | M1], amp[v | M0 | M1]};
     double2 y[4];
 #pragma unroll
@@ -598,6 +621,33 @@ __device__ __forceinline__ void k2_regreg(const QC Op& op, double2 (&amp)[KV], c
   }
 }
 
+// two targets on register slots J0 (matrix bit 0) and J1 (matrix bit 1)
+template <int J0, int J1, int KV>
+__device__ __forceinline__ void k2_regreg(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
+  constexpr int M0 = 1 << J0, M1 = 1 << J1;
+  if constexpr (M0 >= KV || M1 >= KV) return;  // no such register qubit in this tile shape
+  // a wave-uniform selector keeps the matrix in scalar loads (the uniform
+  // form is a separate loop, so that the compiler sees uniform addresses)
+  if (op.nsel == 0 || op_sel_uniform(op)) {
+#pragma unroll
+    for (int v = 0; v < KV; ++v) {
+      if (v & (M0 | M1)) continue;
+      QSIM_ITER_FENCE();
+      const uint32_t sel = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_sel(op, cx, v));
+      if ((op.ident_mask >> sel) & 1ull) continue;
+      k2_regreg_rows<M0, M1, KV>(amp, v, C, op.mat_off + 16ull * sel);
+    }
+    return;
+  }
+#pragma unroll
+  for (int v = 0; v < KV; ++v) {
+    if (v & (M0 | M1)) continue;
+    QSIM_ITER_FENCE();
+    const uint32_t sel = wave_sel(op, cx, v);
+    if ((op.ident_mask >> sel) & 1ull) continue;
+    k2_regreg_rows<M0, M1, KV>(amp, v, C, op.mat_off + 16ull * sel);
+  }
+}
 // two targets: lane bit b (matrix bit il) and register slot J (matrix bit 1-il)
 template <int J, int KV>
 __device__ __forceinline__ void k2_lanereg(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C,
@@ -974,6 +1024,8 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   const bool own = kPersistent && OUT == OUT_ENC && dy.wave_subs != 0u;
   const uint32_t gwave = (uint32_t)blockIdx.x * (kThreads / 64) + (uint32_t)wave;
   uint64_t wcur = 0;
+  // this lane's slot offset of an ENC store's tile permutation
+  const uint32_t tperm_sl = OUT == OUT_ENC && a.tperm ? (uint32_t)a.tperm_lane[lane] : 0u;
   auto tile_body = [&](const uint64_t wt, auto&& pre) {
   const uint32_t L = a.L;
   const uint64_t slot = wt >> (L - TB);
@@ -1091,7 +1143,8 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
       dst[2 * roff(v) + 1] = amp[v].y;
     }
   } else {
-    to_chunk_layout(amp, xb, (uint32_t)lane);
+    if (a.tperm) to_chunk_layout_perm(amp, xb, (uint32_t)lane, tperm_sl, a.tperm_row);
+    else to_chunk_layout(amp, xb, (uint32_t)lane);
     // zero check of blocks that were null at stage start
     if (ballot64(mnull != 0u) != 0) {
       bool nz = false;

@@ -492,6 +492,49 @@ def test_swap_only_stage_moves_blocks_and_null_flags():
 
 
 @pytest.mark.parametrize("store", ["double", "regions"])
+def test_tile_swaps_in_encode(store, monkeypatch):
+    """Trailing SWAPs with a qubit inside the codec chunk (an in-chunk qubit
+    with a block qubit, two in-chunk qubits, lane qubit 5 with a chunk qubit)
+    become the encoder's tile permutation: one op-free pass whose output is
+    the permuted state, null flags following the data."""
+    n, B = 14, 8
+    x = np.zeros(1 << n, dtype=np.complex128)
+    rng = np.random.default_rng(21)
+    x[: 3 << B] = rng.normal(size=3 << B) + 1j * rng.normal(size=3 << B)  # blocks 0..2, rest null
+    x /= np.linalg.norm(x)
+    pairs = [(1, 9), (3, 12), (0, 2), (5, 4)]
+    c = QuantumCircuit(n)
+    for a, b in pairs:
+        c.add_gate(gate.SWAP(a, b))
+    cs = CompressedQuantumState(n, B, eps_rel=1e-12, eps_abs=1e-14, store=store, verbose=False)
+    cs.load(x)
+    before = cs.get_raw_vector()
+    cs.partition_and_register_circuit(c, 4)
+    assert cs.get_stage_passes() == [1]
+    cs.apply_circuit()
+    idx = np.arange(1 << n)
+    def swap_bits(v, a, b):
+        d = ((v >> a) ^ (v >> b)) & 1
+        return v ^ ((d << a) | (d << b))
+    perm = idx
+    for a, b in pairs:
+        perm = swap_bits(perm, a, b)
+    expect = np.empty_like(before)
+    expect[perm] = before
+    assert np.max(np.abs(cs.get_raw_vector() - expect)) < 1e-12
+    nz = sorted({int(b) for b in (perm[np.flatnonzero(before)] >> B)})
+    assert cs.get_nonnull_blocks() == sorted(set(nz) | {0, 1, 2})
+    # the same circuit with the swaps applied as gates gives the same bits
+    monkeypatch.setenv("QOMPSIM_STORE_SWAPS", "0")
+    cs2 = CompressedQuantumState(n, B, eps_rel=1e-12, eps_abs=1e-14, store=store, verbose=False)
+    cs2.load(x)
+    cs2.partition_and_register_circuit(c, 4)
+    cs2.apply_circuit()
+    assert np.array_equal(cs2.get_raw_vector().view(np.uint64), cs.get_raw_vector().view(np.uint64))
+    assert cs2.get_nonnull_blocks() == cs.get_nonnull_blocks()
+
+
+@pytest.mark.parametrize("store", ["double", "regions"])
 def test_hbm_budget(store):
     """hbm_budget bounds what the store holds: below the fixed descriptor and
     staging bytes the constructor refuses; a budget the run fits in gives the
