@@ -491,13 +491,26 @@ struct Program {
         if (op.kind != OP_ELEM || op.nterms < kPhaseTableMinTerms) continue;
         uint32_t slots = 0, flags = 0;
         bool ok = true;
-        // terms on one lane bit and base bits: the lane table M while it
-        // fits, else the lane-bit angles G
-        const bool use_g = L - a.t > kPhaseTableMaxTauLogM;
-        if (use_g) flags |= kPtabG;
+        // the base bits tested by slot-free parts with lane and base
+        // conditions: few of them key a small lane table K
+        uint64_t kmask = 0;
+        for (uint32_t i = 0; i < op.nterms; ++i) {
+          const PhaseTerm& T = terms[op.term_off + i];
+          if (T.bval & ~T.bmask) continue;  // padding: never matches
+          const uint32_t lm = T.tmask & 63u, rm = (T.tmask >> 6) & 15u, rv = (T.tval >> 6) & 15u;
+          const bool has_free = rm == 0u || (!(rm & (rm - 1u)) && rv == 0u);
+          if (has_free && lm && T.bmask) kmask |= T.bmask;
+        }
+        const bool keyed = (uint32_t)__builtin_popcountll(kmask) <= kPtabMaxKeyBits;
+        // otherwise terms on one lane bit and base bits: the lane table M
+        // while it fits, else the lane-bit angles G
+        const bool use_g = !keyed && L - a.t > kPhaseTableMaxTauLogM;
         auto slot_free = [&](uint32_t lm, uint32_t lv, uint64_t bm) {
-          if (use_g && lm && bm && !(lm & (lm - 1u))) {
+          if (keyed) {
+            flags |= (lm && bm) ? kPtabK : (bm ? kPtabU : kPtabQ);
+          } else if (use_g && lm && bm && !(lm & (lm - 1u))) {
             slots |= lm << 8;
+            flags |= kPtabG;
             if (!lv) flags |= kPtabU;
           } else {
             flags |= (lm && bm) ? kPtabM : (bm ? kPtabU : kPtabQ);
@@ -515,21 +528,29 @@ struct Program {
           } else {
             ok = ok && !(lm != 0u && T.bmask != 0ull);
             slots |= rm;
+            if (T.bmask) flags |= kPtabFt;
             if (rv == 0u) slot_free(lm, lv, T.bmask);
           }
         }
         if ((flags & kPtabM) && L - a.t > kPhaseTableMaxTauLogM) ok = false;
         // a lane table M takes every slot-free part (one read per tile)
         if (flags & kPtabM) flags &= ~(kPtabQ | kPtabU);
-        const uint64_t q = kPtabF + 4ull * ntau, m = q + 64 + 9ull * ntau;
-        const uint64_t len = m + ((flags & kPtabM) ? 64ull * ntau : 0ull);
+        // only the tables that hold a term take room
+        const uint32_t nk = (flags & kPtabK) ? (uint32_t)__builtin_popcountll(kmask) : 0u;
+        const uint64_t q = kPtabF + ((flags & kPtabFt) ? 4ull * ntau : 0ull);
+        const uint64_t g = q + 64 + ((flags & kPtabU) ? ntau : 0ull);
+        const uint64_t m = g + ((flags & kPtabG) ? 8ull * ntau : 0ull);
+        const uint64_t len = m + ((flags & kPtabK) ? (64ull << nk) : (flags & kPtabM) ? 64ull * ntau : 0ull);
         if (!ok || ptab_len + len > kPhaseTableMaxEntries) continue;
         op.ptab_off = ptab_len;
         op.ptab_q = (uint32_t)q;
-        op.ptab_g = (uint32_t)(q + 64 + ntau);
+        op.ptab_g = (uint32_t)g;
         op.ptab_m = (uint32_t)m;
         op.ptab_slots = slots;
         op.ptab_flags = flags;
+        op.ptab_nk = nk;
+        for (uint32_t i = 0, b = 0; b < 64 && i < nk; ++b)
+          if ((kmask >> b) & 1ull) op.ptab_kpos[i++] = (uint8_t)b;
         ptab_jobs.push_back({pi, o, ntau, len});
         ptab_len += (len + 63) & ~63ull;
       }
@@ -618,6 +639,8 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
   // avoid its qubits (the two commute), so that the pass's diagonal gates
   // gather into fewer elementwise ops, each of which evaluates its phase
   // factors once (RY RZ RY RZ ... on different qubits: one op for all RZ).
+  // A dense gate sends ahead of itself only the pending diagonal gates that
+  // it does not commute with; the rest gather further.
   // A gradient point Re<lambda|iP psi> is read where psi and lambda have been
   // carried through the same gates, so a diagonal gate D moves past it when D
   // commutes with P, i.e. avoids P's X/Y qubits.  QOMPSIM_DIAG_GATHER=0 keeps
@@ -627,12 +650,24 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
   {
     const char* eg = std::getenv("QOMPSIM_DIAG_GATHER");
     const bool gather = !(eg && eg[0] == '0');
+    // the pending diagonal gates (in order) and their qubit masks; a dense
+    // gate or gradient point emits only the pending gates on its non-diagonal
+    // qubits (in their order) before itself, the others stay pending
     std::vector<size_t> run;
-    uint64_t run_q = 0;
-    auto flush = [&]() {
-      order.insert(order.end(), run.begin(), run.end());
-      run.clear();
-      run_q = 0;
+    std::vector<uint64_t> run_m;
+    auto flush_on = [&](uint64_t conflict) {
+      size_t k = 0;
+      for (size_t i = 0; i < run.size(); ++i) {
+        if (run_m[i] & conflict) {
+          order.push_back(run[i]);
+        } else {
+          run[k] = run[i];
+          run_m[k] = run_m[i];
+          ++k;
+        }
+      }
+      run.resize(k);
+      run_m.resize(k);
     };
     for (size_t gi : pp.gates) {
       const LGate& g = lg[gi];
@@ -641,21 +676,23 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
         continue;
       }
       if (g.grad >= 0) {
-        if (g.gx & run_q) flush();
+        flush_on(g.gx);
         order.push_back(gi);
         continue;
       }
       if (g.diag) {
+        uint64_t m = 0;
+        for (auto q : g.Q) m |= 1ull << q;
         run.push_back(gi);
-        for (auto q : g.Q) run_q |= 1ull << q;
+        run_m.push_back(m);
         continue;
       }
       uint64_t nq = 0;
       for (auto q : g.N) nq |= 1ull << q;
-      if (nq & run_q) flush();
+      flush_on(nq);
       order.push_back(gi);
     }
-    flush();
+    flush_on(~0ull);
   }
   for (size_t gi : order) {
     const LGate& g = lg[gi];

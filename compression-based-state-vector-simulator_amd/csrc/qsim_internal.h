@@ -78,7 +78,9 @@ struct Op {
   uint32_t ptab_q;         // offsets from ptab_off (complex) of Q (U follows Q), G and M
   uint32_t ptab_g, ptab_m, pad2_;
   uint32_t ptab_slots;     // bit j: slot j has a factor; bit 8 + k: lane bit k has a G factor
-  uint32_t ptab_flags;     // kPtabQ | kPtabU | kPtabM | kPtabRows: those tables hold a term
+  uint32_t ptab_flags;     // kPtabQ | kPtabU | kPtabM | kPtabRows | kPtabFt | kPtabK: those tables hold a term
+  uint32_t ptab_nk;        // kPtabK: M's row is the value of these base bits (local qubits ptab_kpos[])
+  uint8_t ptab_kpos[8];
 };
 
 // Phase tables of an elementwise op on a wave tile: lanes on tile bits 0..5,
@@ -101,14 +103,21 @@ struct Op {
 //   M[ntau][64] the other slot-free parts with lane and base conditions
 //               (only when ntau <= 2^16; M then takes every slot-free part
 //               and Q, U are unused),
-// at complex offsets 0, 256, 272, ptab_q = 272 + 4 ntau, ptab_q + 64,
-// ptab_g = ptab_q + 64 + ntau and ptab_m.  Ops whose terms do not separate so (a slot
+//   K[2^nk][64] (kPtabK) replaces M when the terms with lane and base
+//               conditions test at most kPtabMaxKeyBits distinct base bits:
+//               row = the value of those bits in the tile base (Q and U stay)
+// at complex offsets 0, 256, 272 (F, 4 ntau entries when kPtabFt), ptab_q
+// (Q, 64), ptab_q + 64 (U, ntau entries when kPtabU), ptab_g (G, 8 ntau when
+// kPtabG) and ptab_m (M or K).  Ops whose terms do not separate so (a slot
 // term with both a lane and a base condition) keep the per-term loop.
 constexpr uint32_t kPtabC = 0, kPtabR = 256, kPtabF = 272;
 constexpr uint32_t kPtabQ = 1, kPtabRows = 2, kPtabU = 4, kPtabM = 8;
 // G replaces M for single-lane-bit terms when M would exceed its limit
 constexpr uint32_t kPtabG = 16;
-constexpr uint32_t kPhaseTableMinTerms = 4;     // fewer terms: the per-term loop
+constexpr uint32_t kPtabFt = 32;  // the per-tile slot factors F hold a term
+constexpr uint32_t kPtabK = 64;   // the lane table is keyed by base bits
+constexpr uint32_t kPtabMaxKeyBits = 6;
+constexpr uint32_t kPhaseTableMinTerms = 1;     // fewer terms: the per-term loop
 constexpr uint32_t kPhaseTableMaxTauLog = 20;   // F and U of at most 2^20 tiles (80 MiB)
 constexpr uint32_t kPhaseTableMaxTauLogM = 16;  // M of at most 2^16 x 64 entries (64 MiB)
 constexpr uint64_t kPhaseTableMaxEntries = 1ull << 26;  // 1 GiB of tables per program

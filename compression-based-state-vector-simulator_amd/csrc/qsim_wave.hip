@@ -445,17 +445,15 @@ __device__ __forceinline__ bool op_sel_uniform(const QC Op& op) {
   return true;
 }
 
-// one target on lane bit B, one matrix for every row
+// one target on lane bit B, one matrix (at complex offset mo) for every row
 template <int B, int KV>
 __device__ __forceinline__ void k1_lane_uniform(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx,
-                                                const QC double* __restrict__ C) {
+                                                const QC double* __restrict__ C, uint64_t mo) {
   const uint32_t lane = (uint32_t)cx.lane;
-  if (op.ident_mask & 1ull) return;
   if constexpr (B >= 4) {
     // rows in pairs: after the half-row swap each lane holds both amplitudes
     // of a pair, so the matrix entries are wave-uniform
-    const double2 m0 = ldc(C, op.mat_off), m1 = ldc(C, op.mat_off + 1), m2 = ldc(C, op.mat_off + 2),
-                  m3 = ldc(C, op.mat_off + 3);
+    const double2 m0 = ldc(C, mo), m1 = ldc(C, mo + 1), m2 = ldc(C, mo + 2), m3 = ldc(C, mo + 3);
     if (op.real == kDenseReal) {
 #pragma unroll
       for (int v = 0; v < KV; v += 2) {
@@ -493,22 +491,21 @@ __device__ __forceinline__ void k1_lane_uniform(const QC Op& op, double2 (&amp)[
     const uint32_t r = (lane >> B) & 1u;
     // the four entries are wave-uniform (scalar loads); each lane takes its
     // row's diagonal and off-diagonal entry
-    const double2 m0 = ldc(C, op.mat_off), m1 = ldc(C, op.mat_off + 1), m2 = ldc(C, op.mat_off + 2),
-                  m3 = ldc(C, op.mat_off + 3);
-    const double2 md = r ? m3 : m0, mo = r ? m2 : m1;
+    const double2 m0 = ldc(C, mo), m1 = ldc(C, mo + 1), m2 = ldc(C, mo + 2), m3 = ldc(C, mo + 3);
+    const double2 md = r ? m3 : m0, mf = r ? m2 : m1;
     if (op.real == kDenseReal) {
 #pragma unroll
-      for (int v = 0; v < KV; ++v) amp[v] = mac2r(md.x, amp[v], mo.x, lane_xor2<B>(amp[v], lane));
+      for (int v = 0; v < KV; ++v) amp[v] = mac2r(md.x, amp[v], mf.x, lane_xor2<B>(amp[v], lane));
       return;
     }
     if (op.real == kDenseRealDiagImagOff) {
-      // md is the diagonal entry of this lane's row, mo the off-diagonal one
+      // md is the diagonal entry of this lane's row, mf the off-diagonal one
 #pragma unroll
-      for (int v = 0; v < KV; ++v) amp[v] = mac2_ri(md.x, amp[v], mo.y, lane_xor2<B>(amp[v], lane));
+      for (int v = 0; v < KV; ++v) amp[v] = mac2_ri(md.x, amp[v], mf.y, lane_xor2<B>(amp[v], lane));
       return;
     }
 #pragma unroll
-    for (int v = 0; v < KV; ++v) amp[v] = mac2(md, amp[v], mo, lane_xor2<B>(amp[v], lane));
+    for (int v = 0; v < KV; ++v) amp[v] = mac2(md, amp[v], mf, lane_xor2<B>(amp[v], lane));
   }
 }
 
@@ -715,16 +712,30 @@ template <int KV>
 __device__ __forceinline__ void wave_dense(const QC Op& op, double2 (&amp)[KV], const WaveCtx& cx, const QC double* __restrict__ C) {
   if (op.k == 1) {
     const uint32_t t = op.tpos[0];
-    if (t < 6 && op.nsel != 0) {
-      k1_lane_sel(op, amp, cx, C);
-    } else if (t < 6) {
+    if (t < 6) {
+      uint64_t mo = op.mat_off;
+      if (op.nsel != 0) {
+        bool tile_sel = false;
+        for (uint32_t i = 0; i < op.nsel; ++i) tile_sel = tile_sel || op.sel_tile[i];
+        if (tile_sel) {
+          // a selector on a lane or a register slot: the matrix varies per element
+          k1_lane_sel(op, amp, cx, C);
+          return;
+        }
+        // selectors on base bits only: one matrix for the whole tile
+        const uint32_t sel = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_sel(op, cx, 0));
+        if ((op.ident_mask >> sel) & 1ull) return;
+        mo += 4ull * sel;
+      } else if (op.ident_mask & 1ull) {
+        return;
+      }
       switch (t) {
-        case 0: k1_lane_uniform<0>(op, amp, cx, C); break;
-        case 1: k1_lane_uniform<1>(op, amp, cx, C); break;
-        case 2: k1_lane_uniform<2>(op, amp, cx, C); break;
-        case 3: k1_lane_uniform<3>(op, amp, cx, C); break;
-        case 4: k1_lane_uniform<4>(op, amp, cx, C); break;
-        default: k1_lane_uniform<5>(op, amp, cx, C); break;
+        case 0: k1_lane_uniform<0>(op, amp, cx, C, mo); break;
+        case 1: k1_lane_uniform<1>(op, amp, cx, C, mo); break;
+        case 2: k1_lane_uniform<2>(op, amp, cx, C, mo); break;
+        case 3: k1_lane_uniform<3>(op, amp, cx, C, mo); break;
+        case 4: k1_lane_uniform<4>(op, amp, cx, C, mo); break;
+        default: k1_lane_uniform<5>(op, amp, cx, C, mo); break;
       }
     } else switch (t - 6) {
         case 0: k1_reg<0>(op, amp, cx, C); break;
@@ -781,12 +792,18 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], c
     const QC double* TU = (const QC double*)(cx.ptab) + 2 * op.ptab_off;
     auto ldg = [&](uint64_t i) { return make_double2(TG[2 * i], TG[2 * i + 1]); };
     const uint32_t sl = op.ptab_slots, fl = op.ptab_flags, gb = (sl >> 8) & 63u;
-    // the slot-free factor: Q[lane] M[tau][lane] U[tau] prod_k G[tau][k]^(lane bit k)
-    if ((fl & (kPtabQ | kPtabU | kPtabM)) || gb) {
+    // the slot-free factor: Q[lane] M[row][lane] U[tau] prod_k G[tau][k]^(lane bit k),
+    // M's row the tile (kPtabM) or the value of the key bits of the base (kPtabK)
+    if ((fl & (kPtabQ | kPtabU | kPtabM | kPtabK)) || gb) {
       double2 t0 = make_double2(1.0, 0.0);
       if (fl & kPtabQ) t0 = ldg(op.ptab_q + lane);
-      if (fl & kPtabM) {
-        const double2 m = ldg(op.ptab_m + (cx.tau << 6) + lane);
+      if (fl & (kPtabM | kPtabK)) {
+        uint64_t row = cx.tau;
+        if (fl & kPtabK) {
+          row = 0;
+          for (uint32_t i = 0; i < op.ptab_nk; ++i) row |= ((cx.base >> op.ptab_kpos[i]) & 1ull) << i;
+        }
+        const double2 m = ldg(op.ptab_m + (row << 6) + lane);
         t0 = (fl & kPtabQ) ? cmul(t0, m) : m;
       }
       if (fl & kPtabU) t0 = cmul(t0, ldc(TU, op.ptab_q + 64 + cx.tau));
@@ -807,27 +824,27 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], c
     const uint64_t F = kPtabF + (cx.tau << 2);
     if (sl & 1u) {
       __builtin_amdgcn_sched_barrier(0);
-      const double2 f = cmul(ldg(kPtabC + lane), ldc(TU, F + 0));
+      const double2 f = (fl & kPtabFt) ? cmul(ldg(kPtabC + lane), ldc(TU, F + 0)) : ldg(kPtabC + lane);
 #pragma unroll
       for (int v = 1; v < KV; v += 2) amp[v] = cmul(amp[v], f);
     }
     if (sl & 2u) {
       __builtin_amdgcn_sched_barrier(0);
-      const double2 f = cmul(ldg(kPtabC + 64 + lane), ldc(TU, F + 1));
+      const double2 f = (fl & kPtabFt) ? cmul(ldg(kPtabC + 64 + lane), ldc(TU, F + 1)) : ldg(kPtabC + 64 + lane);
 #pragma unroll
       for (int v = 0; v < KV; ++v)
         if (v & 2) amp[v] = cmul(amp[v], f);
     }
     if (sl & 4u) {
       __builtin_amdgcn_sched_barrier(0);
-      const double2 f = cmul(ldg(kPtabC + 128 + lane), ldc(TU, F + 2));
+      const double2 f = (fl & kPtabFt) ? cmul(ldg(kPtabC + 128 + lane), ldc(TU, F + 2)) : ldg(kPtabC + 128 + lane);
 #pragma unroll
       for (int v = 0; v < KV; ++v)
         if (v & 4) amp[v] = cmul(amp[v], f);
     }
     if (KV > 8 && (sl & 8u)) {
       __builtin_amdgcn_sched_barrier(0);
-      const double2 f = cmul(ldg(kPtabC + 192 + lane), ldc(TU, F + 3));
+      const double2 f = (fl & kPtabFt) ? cmul(ldg(kPtabC + 192 + lane), ldc(TU, F + 3)) : ldg(kPtabC + 192 + lane);
 #pragma unroll
       for (int v = 8; v < KV; ++v) amp[v] = cmul(amp[v], f);
     }
@@ -1669,7 +1686,13 @@ __global__ __launch_bounds__(kThreads) void phase_table_kernel(const PassArgs* _
   } else {
     kind = K_M;
     lane = (uint32_t)((e - mo) & 63u);
-    base = tm.base((e - mo) >> 6);
+    const uint64_t row = (e - mo) >> 6;
+    if (op.ptab_flags & kPtabK) {
+      // the row is the value of the key bits; the terms of K test no others
+      for (uint32_t i = 0; i < op.ptab_nk; ++i) base |= ((row >> i) & 1ull) << op.ptab_kpos[i];
+    } else {
+      base = tm.base(row);
+    }
   }
   double s = 0.0;
   for (uint32_t i = 0; i < op.nterms; ++i) {
