@@ -689,9 +689,14 @@ def main():
         "roofline_dec_enc": {
             "kernel": "wave_kernel<IN_DEC, OUT_ENC> (one launch per single-pass stage)",
             "launches": len(dom), "avg_ms": dom_s * 1e3 / max(1, len(dom)),
-            "achieved": dom_gbps, "frac": dom_gbps / HBM_PEAK_GBPS,
+            # the stage byte model counts 32 A bytes of decompressed amplitudes
+            # that this fused kernel keeps on chip, so it can pass the HBM peak;
+            # the measured rate is the counters' traffic over the launch time
+            "model_gbps": dom_gbps,
             "compressed_bytes_gbps": compressed_only,
             "traffic_per_launch": traffic["dec_enc_bytes_per_launch"] if traffic else None,
+            "measured_gbps": (traffic["dec_enc_bytes_per_launch"] / (dom_s / max(1, len(dom))) / 1e9
+                              if traffic and dom_s > 0 else None),
         },
         "stage_kernel_ms": gpu_s * 1e3,
         "compression_ratio": raw / max_bytes,

@@ -639,8 +639,9 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
   // avoid its qubits (the two commute), so that the pass's diagonal gates
   // gather into fewer elementwise ops, each of which evaluates its phase
   // factors once (RY RZ RY RZ ... on different qubits: one op for all RZ).
-  // A dense gate sends ahead of itself only the pending diagonal gates that
-  // it does not commute with; the rest gather further.
+  // A conflicting dense gate sends the whole pending run ahead of itself:
+  // sending only the gates it conflicts with made one op per RY of HEA's
+  // next layer.
   // A gradient point Re<lambda|iP psi> is read where psi and lambda have been
   // carried through the same gates, so a diagonal gate D moves past it when D
   // commutes with P, i.e. avoids P's X/Y qubits.  QOMPSIM_DIAG_GATHER=0 keeps
@@ -650,24 +651,12 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
   {
     const char* eg = std::getenv("QOMPSIM_DIAG_GATHER");
     const bool gather = !(eg && eg[0] == '0');
-    // the pending diagonal gates (in order) and their qubit masks; a dense
-    // gate or gradient point emits only the pending gates on its non-diagonal
-    // qubits (in their order) before itself, the others stay pending
     std::vector<size_t> run;
-    std::vector<uint64_t> run_m;
-    auto flush_on = [&](uint64_t conflict) {
-      size_t k = 0;
-      for (size_t i = 0; i < run.size(); ++i) {
-        if (run_m[i] & conflict) {
-          order.push_back(run[i]);
-        } else {
-          run[k] = run[i];
-          run_m[k] = run_m[i];
-          ++k;
-        }
-      }
-      run.resize(k);
-      run_m.resize(k);
+    uint64_t run_q = 0;
+    auto flush = [&]() {
+      order.insert(order.end(), run.begin(), run.end());
+      run.clear();
+      run_q = 0;
     };
     for (size_t gi : pp.gates) {
       const LGate& g = lg[gi];
@@ -676,23 +665,21 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
         continue;
       }
       if (g.grad >= 0) {
-        flush_on(g.gx);
+        if (g.gx & run_q) flush();
         order.push_back(gi);
         continue;
       }
       if (g.diag) {
-        uint64_t m = 0;
-        for (auto q : g.Q) m |= 1ull << q;
         run.push_back(gi);
-        run_m.push_back(m);
+        for (auto q : g.Q) run_q |= 1ull << q;
         continue;
       }
       uint64_t nq = 0;
       for (auto q : g.N) nq |= 1ull << q;
-      flush_on(nq);
+      if (nq & run_q) flush();
       order.push_back(gi);
     }
-    flush_on(~0ull);
+    flush();
   }
   for (size_t gi : order) {
     const LGate& g = lg[gi];
