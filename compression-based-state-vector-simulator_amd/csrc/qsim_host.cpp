@@ -70,6 +70,7 @@ hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* p
                                uint64_t ntiles, hipStream_t st);
 hipError_t launch_wave_pair_adjoint(int tile_bits, const PassArgs* d_args, double2* psi, double2* lam,
                                     double* partials, uint64_t ntiles, hipStream_t st);
+hipError_t launch_wave_lam(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st);
 hipError_t launch_pauli_sum(const double2* a, double2* out, uint64_t n, const PauliGroup* groups, uint32_t ng,
                             const PauliTermZ* terms, double* partials, unsigned grid, hipStream_t st);
 hipError_t launch_pauli_expect(const double2* a, uint64_t n, const PauliGroup* groups, uint32_t ng,
@@ -358,6 +359,8 @@ struct LGate {
   // a trailing SWAP folded into the encoder's tile permutation: it holds its
   // qubits in the tile while the pass is planned and emits no op
   bool noop = false;
+  // index of the circuit gate this op inverts in a reverse program, else -1
+  int64_t orig = -1;
   std::vector<uint32_t> N;      // non-diagonal targets (matrix bit order)
   std::vector<uint32_t> S;      // selector qubits
   std::vector<cplx> mats;       // 2^|S| matrices of 2^|N| x 2^|N|
@@ -803,7 +806,8 @@ static void emit_pass(Program& P, const PassPlan& pp, const std::vector<LGate>& 
 // c >= kChunkAmpsLog2 (codec chunks are 32 contiguous amplitudes); an
 // op-free pass is added at either end when needed.
 static void plan_program_engine(Program& P, const std::vector<LGate>& lg, uint32_t L, uint32_t tmax, uint32_t cmin,
-                                bool at_least_one_pass, bool codec_ends, bool wave, uint32_t wave_bits = kWaveTile) {
+                                bool at_least_one_pass, bool codec_ends, bool wave, uint32_t wave_bits = kWaveTile,
+                                std::vector<std::vector<size_t>>* pass_gates = nullptr) {
   P.L = L;
   const uint32_t t = wave ? wave_bits : std::min(tmax, L);
   if (wave) cmin = 6;
@@ -889,6 +893,7 @@ static void plan_program_engine(Program& P, const std::vector<LGate>& lg, uint32
   for (auto& pp : plans) {
     emit_pass(P, pp, lg);
     P.passes.back().engine = wave ? ENGINE_WAVE : ENGINE_LDS;
+    if (pass_gates) pass_gates->push_back(pp.gates);
   }
 }
 
@@ -1101,6 +1106,13 @@ struct qsim_sv {
   // adjoint work buffers (kept between calls): lambda, the uncomputed psi,
   // per-tile gradient partials, reduced points, Pauli-sum tables
   DBuf adj_lam, adj_psi, adj_part, adj_out, adj_stage, adj_groups, adj_terms;
+  // the lambda-only engine's copies of psi at reverse pass starts
+  std::vector<std::unique_ptr<DBuf>> adj_ck;
+  uint64_t adj_ck_bytes() const {
+    uint64_t b = 0;
+    for (auto& c : adj_ck) b += c->bytes;
+    return b;
+  }
   double2* ptr() const { return reinterpret_cast<double2*>(d.p); }
 };
 
@@ -2654,11 +2666,42 @@ static std::vector<LGate> cluster_gradient_points(const std::vector<LGate>& lg) 
   return out;
 }
 
+// Moves each diagonal op of a reverse program back over earlier ops it
+// commutes with (other diagonal ops; dense ops whose non-diagonal targets
+// avoid its qubits; gradient points whose X/Y qubits avoid them), so that
+// the diagonal gates of a layer (HEA's RZ^-1 and CZ^-1) gather in one place
+// instead of one per qubit between the points.
+static std::vector<LGate> gather_reverse_diagonals(const std::vector<LGate>& lg) {
+  auto qmask = [](const std::vector<uint32_t>& v) {
+    uint64_t m = 0;
+    for (auto q : v) m |= 1ull << q;
+    return m;
+  };
+  std::vector<LGate> out;
+  std::vector<uint64_t> blk;  // per output op: the qubits a diagonal op may not cross
+  out.reserve(lg.size());
+  blk.reserve(lg.size());
+  for (const LGate& g : lg) {
+    const bool dg = g.grad < 0 && g.diag;
+    uint64_t b = 0;
+    if (g.grad >= 0) b = g.gx;
+    else if (!g.diag) b = qmask(g.N);
+    size_t at = out.size();
+    if (dg) {
+      const uint64_t q = qmask(g.Q);
+      while (at > 0 && !(blk[at - 1] & q)) --at;
+    }
+    out.insert(out.begin() + (ptrdiff_t)at, g);
+    blk.insert(blk.begin() + (ptrdiff_t)at, b);
+  }
+  return out;
+}
+
 // wave_tile: 0 for the default wave form (adjoint_wave_tile), else the tile
 // bits of the wave passes
 static void plan_reverse(Program& P, const std::vector<LGate>& lg0, uint32_t n, uint32_t tb, bool wave,
                          uint32_t wave_tile = 0) {
-  const std::vector<LGate> lg = cluster_gradient_points(lg0);
+  const std::vector<LGate> lg = cluster_gradient_points(gather_reverse_diagonals(lg0));
   const uint32_t wt = wave_tile ? wave_tile : adjoint_wave_tile();
   if (wave && n >= wt) plan_program_engine(P, lg, n, wt, 6, false, false, true, wt);
   else plan_program_engine(P, lg, n, tb, std::min<uint32_t>(5, tb), false, false, false);
@@ -2783,6 +2826,171 @@ static void reverse_sweep(qsim_sv* sv, double2* psi_w, double2* lam, const std::
     for (uint32_t k = 0; k < np; ++k) gacc[params[pi][k]].merge(hout[2 * k], hout[2 * k + 1]);
   }
 }
+
+// ------------------------------------------------ lambda-only reverse sweep
+// QOMPSIM_ADJ_ENGINE=lam, when device memory allows.  The reverse program is
+// planned so that every gradient point sits at the start of a pass; the
+// forward pass runs the same passes (gates un-inverted, in reverse order) and
+// keeps psi at the start of every reverse pass that has points; the reverse
+// sweep then carries lambda alone and reads psi from those copies.  Points at
+// pass starts cut the passes at every run of points (HEA-28: 49 passes
+// instead of 23, and 48 copies of psi), so the paired sweep stays the default.
+static bool adjoint_lam_engine() {
+  const char* e = std::getenv("QOMPSIM_ADJ_ENGINE");
+  return e && std::string(e) == "lam";
+}
+
+static bool backprop_lam(qsim_sv* sv, const std::vector<HGate>& gs, uint64_t n_terms, const uint64_t* xm,
+                         const uint64_t* zm, const uint32_t* ny, const double* cre, const double* cim,
+                         std::vector<DDAcc>& gacc, double* energy) {
+  const uint32_t n = sv->n;
+  if (!adjoint_lam_engine() || n < kWaveTile || gs.empty()) return false;
+  std::vector<int> qmap(n);
+  std::iota(qmap.begin(), qmap.end(), 0);
+  std::vector<LGate> lg;
+  for (size_t i = gs.size(); i-- > 0;) {
+    if (gs[i].parametric()) lg.push_back(grad_point(gs[i]));
+    lg.push_back(lower_gate(inverse_gate(gs[i]), qmap));
+    lg.back().orig = (int64_t)i;
+  }
+  if (!wave_engine_ok(lg, n)) return false;
+  lg = cluster_gradient_points(gather_reverse_diagonals(lg));
+  // reverse program: one planning call per segment starting at a run of
+  // points, so that points land at pass starts (a point the planner defers
+  // is taken first by a later pass of its segment)
+  Program R;
+  std::vector<std::vector<size_t>> rgates;  // per reverse pass: indices into lg
+  {
+    size_t b = 0;
+    while (b < lg.size()) {
+      size_t e = b + 1;
+      while (e < lg.size() && !(lg[e].grad >= 0 && lg[e - 1].grad < 0)) ++e;
+      std::vector<LGate> seg(lg.begin() + (ptrdiff_t)b, lg.begin() + (ptrdiff_t)e);
+      std::vector<std::vector<size_t>> pg;
+      plan_program_engine(R, seg, n, kWaveTile, 6, false, false, true, kWaveTile, &pg);
+      for (auto& v : pg) {
+        for (auto& i : v) i += b;
+        rgates.push_back(v);
+      }
+      b = e;
+    }
+  }
+  const size_t m = R.passes.size();
+  if (m == 0) return false;
+  std::vector<char> has_pts(m, 0);
+  for (size_t r = 0; r < m; ++r) {
+    const PassArgs& a = R.passes[r];
+    if (a.engine != ENGINE_WAVE || a.t != kWaveTile) return false;
+    bool lead = true;
+    for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
+      if (R.ops[o].kind == OP_GRAD) {
+        if (!lead) return false;  // a point inside a pass: the paired engine takes the circuit
+        has_pts[r] = 1;
+      } else {
+        lead = false;
+      }
+    }
+  }
+  // device memory: a copy of psi per reverse pass with points (pass 0 reads
+  // the state itself), a work copy, lambda
+  const uint64_t bytes = sv->dim * sizeof(double2);
+  uint64_t nck = 0;
+  for (size_t r = 1; r < m; ++r) nck += has_pts[r];
+  {
+    size_t free_b = 0, total_b = 0;
+    HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t held = sv->adj_ck_bytes() + sv->adj_lam.bytes + sv->adj_psi.bytes;
+    const uint64_t want = (nck + 2) * bytes;
+    if (want > held + free_b - std::min<uint64_t>(free_b, 1ull << 30)) return false;
+  }
+  // forward: reverse pass r's gates un-inverted, in reverse order, for r =
+  // m-1 .. 0; the output of segment r is psi at the start of reverse pass r
+  Program F;
+  std::vector<size_t> fbeg(m, 0), fend(m, 0);  // segment r: F passes [fbeg[r], fend[r])
+  for (size_t r = m; r-- > 0;) {
+    std::vector<LGate> fg;
+    for (size_t k = rgates[r].size(); k-- > 0;) {
+      const LGate& g = lg[rgates[r][k]];
+      if (g.grad >= 0) continue;
+      fg.push_back(lower_gate(gs[(size_t)g.orig], qmap));
+    }
+    fbeg[r] = F.passes.size();
+    if (!fg.empty()) plan_program_engine(F, fg, n, default_tile_bits(), 5, false, false, wave_engine_ok(fg, n));
+    fend[r] = F.passes.size();
+  }
+  if (std::getenv("QOMPSIM_PLAN_LOG"))
+    fprintf(stderr, "[adjoint] lambda-only sweep: %zu reverse passes, %zu forward passes, %llu copies of psi\n", m,
+            F.passes.size(), (unsigned long long)nck);
+  while (sv->adj_ck.size() < nck) sv->adj_ck.push_back(std::make_unique<DBuf>());
+  for (uint64_t k = 0; k < nck; ++k) sv->adj_ck[k]->ensure(bytes);
+  sv->adj_psi.ensure(bytes);  // the work copy
+  sv->adj_lam.ensure(bytes);
+  std::vector<double2*> ck(m, nullptr);
+  {
+    size_t k = 0;
+    ck[0] = sv->ptr();
+    for (size_t r = 1; r < m; ++r)
+      if (has_pts[r]) ck[r] = sv->adj_ck[k++]->as<double2>();
+  }
+  F.upload_all();
+  double2* cur = sv->ptr();
+  double2* work = sv->adj_psi.as<double2>();
+  for (size_t r = m; r-- > 0;) {
+    double2* dst = ck[r] ? ck[r] : work;
+    if (fbeg[r] == fend[r]) {
+      if (dst != cur) HIPCHECK(hipMemcpyAsync(dst, cur, bytes, hipMemcpyDeviceToDevice, stream()));
+    } else {
+      for (size_t i = fbeg[r]; i < fend[r]; ++i) {
+        PassDyn dy{};
+        const bool first = i == fbeg[r];
+        if (first && dst != cur && F.passes[i].engine != ENGINE_WAVE) {
+          HIPCHECK(hipMemcpyAsync(dst, cur, bytes, hipMemcpyDeviceToDevice, stream()));
+        }
+        const bool oop = first && dst != cur && F.passes[i].engine == ENGINE_WAVE;
+        dy.raw = reinterpret_cast<double*>(oop ? cur : dst);
+        dy.raw_out = oop ? reinterpret_cast<double*>(dst) : nullptr;
+        HIPCHECK(launch_program_pass(F, i, IN_RAW, OUT_RAW, dy, 1, stream()));
+      }
+    }
+    cur = dst;
+  }
+  // lambda = 2 H psi at the end of the circuit
+  *energy = lambda_and_energy(sv, sv->ptr(), sv->adj_lam.as<double2>(), n_terms, xm, zm, ny, cre, cim);
+  // reverse: points of pass r against psi at its start, then lambda through
+  // the pass's inverted gates
+  std::vector<std::vector<int32_t>> params(m);
+  uint32_t maxpts = 0;
+  for (size_t r = 0; r < m; ++r) {
+    for (uint32_t o = R.passes[r].op_begin; o < R.passes[r].op_end; ++o)
+      if (R.ops[o].kind == OP_GRAD) {
+        params[r].push_back((int32_t)R.ops[o].gidx);
+        R.ops[o].gidx = (uint32_t)(params[r].size() - 1);
+      }
+    maxpts = std::max<uint32_t>(maxpts, (uint32_t)params[r].size());
+  }
+  R.upload_all();
+  const uint64_t ntiles = sv->dim >> kWaveTile;
+  sv->adj_part.ensure(std::max<uint64_t>(1, (uint64_t)maxpts * ntiles) * 2 * sizeof(double));
+  sv->adj_out.ensure(std::max<uint32_t>(1, maxpts) * 2 * sizeof(double));
+  sv->adj_stage.ensure((uint64_t)std::max<uint32_t>(1, maxpts) * kGradReduceChunks * 2 * sizeof(double));
+  std::vector<double> hout(2 * std::max<uint32_t>(1, maxpts));
+  for (size_t r = 0; r < m; ++r) {
+    PassDyn dy{};
+    dy.raw = sv->adj_lam.as<double>();
+    dy.psi = has_pts[r] ? reinterpret_cast<const double*>(ck[r]) : nullptr;
+    dy.partials = sv->adj_part.as<double>();
+    dy.part_tiles = ntiles;
+    HIPCHECK(launch_wave_lam(R.dargs(r), dy, ntiles, stream()));
+    const uint32_t np = (uint32_t)params[r].size();
+    if (np == 0) continue;
+    HIPCHECK(launch_grad_reduce(sv->adj_part.as<double>(), ntiles, np, sv->adj_out.as<double>(),
+                                sv->adj_stage.as<double>(), stream()));
+    HIPCHECK(hipMemcpyAsync(hout.data(), sv->adj_out.p, np * 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    HIPCHECK(hipStreamSynchronize(stream()));
+    for (uint32_t k = 0; k < np; ++k) gacc[params[r][k]].merge(hout[2 * k], hout[2 * k + 1]);
+  }
+  return true;
+}
 }  // namespace qsim
 
 int qsim_sv_backprop(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, const double* pool, uint64_t pool_len,
@@ -2795,16 +3003,18 @@ int qsim_sv_backprop(qsim_sv* sv, const qsim_gate* gates, uint64_t n_gates, cons
   check_qubits(gs, sv->n);
   validate_backprop_gates(gs, n_params);
   const uint64_t bytes = sv->dim * sizeof(double2);
-  // forward: psi = U psi_in
-  sv_apply_gates(sv, gs);
-  // lambda = 2 H psi; <H> = Re<psi|lambda> / 2
-  sv->adj_lam.ensure(bytes);
-  sv->adj_psi.ensure(bytes);
-  *energy = lambda_and_energy(sv, sv->ptr(), sv->adj_lam.as<double2>(), n_terms, xm, zm, ny, cre, cim);
-  HIPCHECK(hipMemcpyAsync(sv->adj_psi.p, sv->d.p, bytes, hipMemcpyDeviceToDevice, stream()));
-  // reverse sweep: gradient point, then the inverse, gate by gate from the end
   std::vector<DDAcc> gacc(n_params);
-  reverse_sweep(sv, sv->adj_psi.as<double2>(), sv->adj_lam.as<double2>(), gs, 0, gs.size(), gacc);
+  if (!backprop_lam(sv, gs, n_terms, xm, zm, ny, cre, cim, gacc, energy)) {
+    // forward: psi = U psi_in
+    sv_apply_gates(sv, gs);
+    // lambda = 2 H psi; <H> = Re<psi|lambda> / 2
+    sv->adj_lam.ensure(bytes);
+    sv->adj_psi.ensure(bytes);
+    *energy = lambda_and_energy(sv, sv->ptr(), sv->adj_lam.as<double2>(), n_terms, xm, zm, ny, cre, cim);
+    HIPCHECK(hipMemcpyAsync(sv->adj_psi.p, sv->d.p, bytes, hipMemcpyDeviceToDevice, stream()));
+    // reverse sweep: gradient point, then the inverse, gate by gate from the end
+    reverse_sweep(sv, sv->adj_psi.as<double2>(), sv->adj_lam.as<double2>(), gs, 0, gs.size(), gacc);
+  }
   HIPCHECK(hipStreamSynchronize(stream()));
   for (uint64_t k = 0; k < n_params; ++k) grads[k] = gacc[k].value() / 2.0;
   API_END

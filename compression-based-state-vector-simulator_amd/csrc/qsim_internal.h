@@ -228,6 +228,15 @@ struct PassDyn {
   uint32_t* overflow;
   uint8_t* nonzero;         // [nb]: set when a block null at start holds data
   uint64_t tile0;           // first tile of this launch
+  // RAW->RAW wave passes: output rows go to raw_out when set (out of place)
+  double* raw_out;
+  // lambda-only reverse passes (raw holds lambda): the gradient points at the
+  // start of the pass read psi at the same positions from `psi` (the stored
+  // forward state) and write their tile sums to
+  // partials[(gidx * part_tiles + tile) * 2]
+  const double* psi;
+  double* partials;
+  uint64_t part_tiles;
 };
 
 // One instance of the persistent batch kernel: a small circuit run start to

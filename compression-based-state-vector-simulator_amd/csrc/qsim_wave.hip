@@ -977,6 +977,83 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], c
   }
 }
 
+// one DPP move of a double (both halves)
+template <int CTRL, int RMASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, RMASK, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, RMASK, 0xf, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+// wave64 sum through DPP row shifts and row broadcasts; lane 63 ends with
+// the total
+__device__ __forceinline__ double wave_sum_f64(double x) {
+  x += dpp_f64<0x111, 0xf>(x);  // row_shr:1
+  x += dpp_f64<0x112, 0xf>(x);  // row_shr:2
+  x += dpp_f64<0x114, 0xf>(x);  // row_shr:4
+  x += dpp_f64<0x118, 0xf>(x);  // row_shr:8
+  x += dpp_f64<0x142, 0xa>(x);  // row_bcast:15
+  x += dpp_f64<0x143, 0xc>(x);  // row_bcast:31
+  return x;
+}
+
+// ---------------------------------------------- points of lambda-only passes
+// Rows of one gradient point Re<lambda| i P |psi> on a tile (lambda in lam,
+// psi in ps at the same positions): P flips tile rows by FR and lanes by fl,
+// with the Z/Y sign of the partner index and Re or Im by kr, as the paired
+// kernel forms it.
+template <int FR, int KV>
+__device__ __forceinline__ void point_rows(const double2 (&lam)[KV], const double2 (&ps)[KV], uint32_t fl, uint32_t fx,
+                                           uint32_t zt, uint32_t kr, uint32_t lsgn, double& acc0, double& acc1) {
+  if constexpr (FR >= KV) {
+    return;  // no such row flip in this tile shape
+  } else {
+    const bool odd = (kr & 1u) != 0u;
+#pragma unroll
+    for (int v = 0; v < KV; ++v) {
+      double2 x = ps[v ^ FR];
+      if (fl) x = shfl_xor2(x, (int)fl);
+      const double2 l = lam[v];
+      const uint32_t sg = lsgn ^ ((uint32_t)(__popc((((uint32_t)v << 6) ^ (fx & ~63u)) & zt) & 1) << 31);
+      double t = odd ? fma(l.x, x.y, -(l.y * x.x)) : fma(l.x, x.x, l.y * x.y);
+      t = __longlong_as_double(__double_as_longlong(t) ^ ((long long)sg << 32));
+      if (v & 1) acc1 += t;
+      else acc0 += t;
+    }
+  }
+}
+// The tile sum of a gradient point (the same value in every lane).
+template <int KV>
+__device__ __forceinline__ double point_tile_sum(const QC Op& op, const double2 (&lam)[KV], const double2 (&ps)[KV],
+                                                 uint32_t lane, uint64_t base) {
+  const uint32_t fx = (uint32_t)op.gfx, zt = (uint32_t)op.gzt;
+  const uint32_t kr = (1u + op.nY) & 3u;  // i * i^nY = i^kr
+  const bool flip_all = (__popcll(base & op.gzb) & 1) ^ (kr == 1u || kr == 2u);
+  const uint32_t fl = fx & 63u, fr = fx >> 6;
+  const uint32_t lsgn = (uint32_t)(__popc((lane ^ fx) & zt & 63u) & 1) << 31;
+  double acc0 = 0.0, acc1 = 0.0;
+  switch (fr) {
+    case 0: point_rows<0>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 1: point_rows<1>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 2: point_rows<2>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 3: point_rows<3>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 4: point_rows<4>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 5: point_rows<5>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 6: point_rows<6>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 7: point_rows<7>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 8: point_rows<8>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 9: point_rows<9>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 10: point_rows<10>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 11: point_rows<11>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 12: point_rows<12>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 13: point_rows<13>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    case 14: point_rows<14>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+    default: point_rows<15>(lam, ps, fl, fx, zt, kr, lsgn, acc0, acc1); break;
+  }
+  const double tot = __shfl(wave_sum_f64(acc0 + acc1), 63, 64);
+  return flip_all ? -tot : tot;
+}
+
 // ------------------------------------------------------------- the kernel
 // KV complex rows per lane: a wave tile of 64 KV amplitudes (lanes on local
 // qubits 0..5, log2 KV register qubits).  KV = 16 is the default tile; KV = 8
@@ -984,9 +1061,12 @@ __device__ __forceinline__ void wave_elem(const QC Op& op, double2 (&amp)[KV], c
 // whose non-diagonal gates need only three register qubits.  In chunk layout
 // 32/KV lanes share one 32-amplitude codec chunk: lane L holds amplitudes
 // KV g .. KV g + KV - 1 (g = L mod 32/KV) of chunk L / (32/KV).
-template <int IN, int OUT, int KV>
+// PT: a lambda-only reverse pass (RAW->RAW on lambda, 16 rows) whose op list
+// may start with gradient points, evaluated against the stored psi.
+template <int IN, int OUT, int KV, bool PT = false>
 __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const PassArgs* __restrict__ pa, PassDyn dy,
                                                                           uint64_t ntiles) {
+  static_assert(!PT || (IN == IN_RAW && OUT == OUT_RAW && KV == 16), "points run on 16-row RAW passes");
   constexpr int RB = KV == 16 ? 4 : 3;            // register qubits
   constexpr uint32_t TB = 6 + RB;                 // tile bits
   constexpr uint32_t LPC = 32 / KV;               // lanes per chunk
@@ -1041,6 +1121,8 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   const bool own = kPersistent && OUT == OUT_ENC && dy.wave_subs != 0u;
   const uint32_t gwave = (uint32_t)blockIdx.x * (kThreads / 64) + (uint32_t)wave;
   uint64_t wcur = 0;
+  // a lambda-only pass whose op list starts with gradient points
+  const bool pts = PT && a.op_begin < a.op_end && ops[a.op_begin].kind == OP_GRAD;
   // this lane's slot offset of an ENC store's tile permutation
   const uint32_t tperm_sl = OUT == OUT_ENC && a.tperm ? (uint32_t)a.tperm_lane[lane] : 0u;
   auto tile_body = [&](const uint64_t wt, auto&& pre) {
@@ -1056,6 +1138,7 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   };
   const WaveCtx cx{lane, base, tau, a.ptab};
   double2 amp[KV];
+  [[maybe_unused]] double2 ps[PT ? KV : 1];
   double2* xb = reinterpret_cast<double2*>(wbuf[wave]);
   const uint32_t g = (uint32_t)lane & (LPC - 1u);  // part of the chunk (chunk layout)
   const uint32_t d0 = 2u * KV * g;                 // first double of this lane's part
@@ -1102,6 +1185,15 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
 #pragma unroll
     for (int v = 0; v < KV; ++v) amp[v] = pf[64 * v + lane];
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    if constexpr (PT) {
+      // psi rows of the points, issued before the next tile's rows so that
+      // waiting for them does not wait for those
+      if (pts) {
+        const QG double* sp = (const QG double*)(dy.psi) + 2 * ((slot << L) + base + lane);
+#pragma unroll
+        for (int v = 0; v < KV; ++v) ps[v] = make_double2(sp[2 * roff(v)], sp[2 * roff(v) + 1]);
+      }
+    }
     pre();  // the next tile's rows into the same buffer
     __builtin_amdgcn_sched_barrier(0);
   } else if constexpr (IN == IN_RAW) {
@@ -1145,7 +1237,21 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   }
 
   // ---- ops
-  for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
+  uint32_t o0 = a.op_begin;
+  if constexpr (PT) {
+    // the points at the start of a lambda-only pass: amp holds lambda, ps psi
+    if (pts) {
+      for (; o0 < a.op_end && ops[o0].kind == OP_GRAD; ++o0) {
+        const QC Op& op = ops[o0];
+        const double tot = point_tile_sum<KV>(op, amp, ps, (uint32_t)lane, base);
+        if (lane == 0) {
+          dy.partials[((uint64_t)op.gidx * dy.part_tiles + wt) * 2] = tot;
+          dy.partials[((uint64_t)op.gidx * dy.part_tiles + wt) * 2 + 1] = 0.0;
+        }
+      }
+    }
+  }
+  for (uint32_t o = o0; o < a.op_end; ++o) {
     const QC Op& op = ops[o];
     if (op.kind == OP_DENSE) wave_dense(op, amp, cx, coefs);
     else wave_elem(op, amp, cx, terms, diags, coefs);
@@ -1153,7 +1259,7 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
 
   // ---- store
   if constexpr (OUT == OUT_RAW) {
-    QG double* dst = (QG double*)(dy.raw) + 2 * ((slot << L) + base + lane);
+    QG double* dst = (QG double*)(dy.raw_out ? dy.raw_out : dy.raw) + 2 * ((slot << L) + base + lane);
 #pragma unroll
     for (int v = 0; v < KV; ++v) {
       dst[2 * roff(v)] = amp[v].x;
@@ -1271,26 +1377,6 @@ __device__ __forceinline__ void grad_rows(const double2 (&amp)[kV], uint32_t zt,
     dd_acc(s, c, grad_term(amp[R + v], amp[v ^ M], bneg ^ (bool)(__popc(q & zt) & 1), kr));
   }
 }
-// one DPP move of a double (both halves)
-template <int CTRL, int RMASK>
-__device__ __forceinline__ double dpp_f64(double v) {
-  const uint64_t b = (uint64_t)__double_as_longlong(v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, RMASK, 0xf, false);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, RMASK, 0xf, false);
-  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-// wave64 sum through DPP row shifts and row broadcasts; lane 63 ends with
-// the total
-__device__ __forceinline__ double wave_sum_f64(double x) {
-  x += dpp_f64<0x111, 0xf>(x);  // row_shr:1
-  x += dpp_f64<0x112, 0xf>(x);  // row_shr:2
-  x += dpp_f64<0x114, 0xf>(x);  // row_shr:4
-  x += dpp_f64<0x118, 0xf>(x);  // row_shr:8
-  x += dpp_f64<0x142, 0xa>(x);  // row_bcast:15
-  x += dpp_f64<0x143, 0xc>(x);  // row_bcast:31
-  return x;
-}
-
 // Single-wave form of the adjoint pass (ParametricQuantumCircuit::
 // backprop_inner_product, src/vqcsim/parametric_circuit.cpp:180-279) on
 // 512-amplitude tiles: lanes on qubits 0..5 and three register qubits
@@ -1777,14 +1863,14 @@ uint64_t wave_persistent_waves(uint64_t ntiles) { return persistent_waves(ntiles
 // grid of at most kWaveGroups work-groups (each wave walks its tiles with a
 // stride of the grid).  Other forms: one tile per wave, launches split at
 // kMaxLaunchBlocks work-groups.
-template <int IN, int OUT, int KV>
+template <int IN, int OUT, int KV, bool PT = false>
 static hipError_t launch_wave_mode(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
   constexpr uint64_t per_block = kThreads / 64;
   if constexpr ((IN == IN_DEC && OUT == OUT_ENC) || (IN == IN_RAW && OUT == OUT_RAW && KV == 16)) {
     static_assert(per_block == 4, "persistent_waves assumes four waves per work-group");
     const uint64_t groups = persistent_waves(ntiles) / per_block;
     if (groups == 0) return hipSuccess;
-    hipLaunchKernelGGL((wave_kernel<IN, OUT, KV>), dim3((unsigned)groups), dim3(kThreads), 0, st, d_args, dy,
+    hipLaunchKernelGGL((wave_kernel<IN, OUT, KV, PT>), dim3((unsigned)groups), dim3(kThreads), 0, st, d_args, dy,
                        dy.tile0 + ntiles);
   } else {
     const uint64_t maxb = max_launch_groups();
@@ -1792,7 +1878,7 @@ static hipError_t launch_wave_mode(const PassArgs* d_args, const PassDyn& dy, ui
     for (uint64_t t0 = 0; t0 < ntiles; t0 += maxb * per_block) {
       const uint64_t cnt = std::min<uint64_t>(ntiles - t0, maxb * per_block);
       d.tile0 = dy.tile0 + t0;
-      hipLaunchKernelGGL((wave_kernel<IN, OUT, KV>), dim3((unsigned)((cnt + per_block - 1) / per_block)),
+      hipLaunchKernelGGL((wave_kernel<IN, OUT, KV, PT>), dim3((unsigned)((cnt + per_block - 1) / per_block)),
                          dim3(kThreads), 0, st, d_args, d, d.tile0 + cnt);
     }
   }
@@ -1806,6 +1892,12 @@ static hipError_t launch_wave_kv(int in_mode, int out_mode, const PassArgs* d_ar
   if (in_mode == IN_DEC && out_mode == OUT_RAW) return launch_wave_mode<IN_DEC, OUT_RAW, KV>(d_args, dy, ntiles, st);
   if (in_mode == IN_RAW && out_mode == OUT_ENC) return launch_wave_mode<IN_RAW, OUT_ENC, KV>(d_args, dy, ntiles, st);
   return launch_wave_mode<IN_DEC, OUT_ENC, KV>(d_args, dy, ntiles, st);
+}
+
+// A lambda-only reverse pass on 1024-amplitude tiles (dy.raw: lambda,
+// dy.psi: the stored forward state, dy.partials: the points' tile sums).
+hipError_t launch_wave_lam(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
+  return launch_wave_mode<IN_RAW, OUT_RAW, 16, true>(d_args, dy, ntiles, st);
 }
 
 // ntiles counts wave tiles (2^tile_bits amplitudes each, tile_bits 10 or 9)

@@ -21,12 +21,16 @@ from ._lib import CsConfig, call, pack_gates, ptr
 from .state import QuantumState
 
 
-def adjoint_gradient(circuit, obs, state=None):
-    """(<H>, dE/dtheta) for `circuit` applied to `state` (|0...0> when None)."""
+def adjoint_gradient(circuit, obs, state=None, register=None):
+    """(<H>, dE/dtheta) for `circuit` applied to `state` (|0...0> when None).
+    `register`, a QuantumState, is used as the working state when given and
+    holds U|state> afterwards."""
     n = circuit.get_qubit_count()
-    psi = QuantumState(n)
+    psi = register if register is not None else QuantumState(n)
     if state is not None:
         psi.load(state)
+    elif register is not None:
+        psi.set_zero_state()
     npar = circuit.get_parameter_count() if hasattr(circuit, "get_parameter_count") else 0
     grads = np.zeros(max(npar, 1), dtype=np.float64)
     rec, pool = pack_gates(circuit.gate_list)

@@ -159,6 +159,52 @@ def test_adjoint_engines_agree(n, rng, monkeypatch):
         assert np.max(np.abs(g - g0)) < 1e-10, eng
 
 
+def _two_qubit_circuit(n, rng, layers=3):
+    """Parametric X/Y/Z rotations, two-qubit Pauli rotations (ZZ, XX, XY, YZ)
+    and fixed H, CNOT and CZ gates: every generator and gate has at most two
+    X/Y qubits, the shape the lambda-only reverse sweep takes."""
+    c = ParametricQuantumCircuit(n)
+    for _ in range(layers):
+        for q in range(n):
+            if rng.random() < 0.3:
+                c.add_gate(G.H(q))
+            kind = rng.integers(3)
+            (c.add_parametric_RX_gate, c.add_parametric_RY_gate, c.add_parametric_RZ_gate)[kind](
+                q, rng.uniform(0, 2 * np.pi))
+        for q in range(0, n - 1, 2):
+            c.add_gate(G.CNOT(q, q + 1))
+        for q in range(1, n - 1, 3):
+            c.add_gate(G.CZ(q, q + 1))
+        for ps in ([3, 3], [1, 1], [1, 2], [2, 3]):
+            a, b = (int(x) for x in rng.choice(n, size=2, replace=False))
+            c.add_parametric_multi_Pauli_rotation_gate([a, b], ps, rng.uniform(0, 2 * np.pi))
+    return c
+
+
+@pytest.mark.parametrize("n", [10, 12, 15])
+def test_lambda_only_sweep_matches_oracle(n, rng, monkeypatch, capfd):
+    """The lambda-only reverse sweep (QOMPSIM_ADJ_ENGINE=lam: psi kept at
+    reverse pass starts) gives the oracle's energy and gradients, the paired
+    sweep's gradients, and leaves the final state in the register; the HEA and
+    a circuit of two-qubit Pauli rotations both run on it."""
+    monkeypatch.setenv("QOMPSIM_PLAN_LOG", "1")
+    for c, obs in ((_two_qubit_circuit(n, rng), _mixed_observable(n, rng)),
+                   (W.hea(n, layers=3, seed=n), W.tfim_observable(n))):
+        e0, g0 = O.backprop(c, obs)
+        monkeypatch.setenv("QOMPSIM_ADJ_ENGINE", "lam")
+        s = QuantumState(n)
+        e, g = adjoint_gradient(c, obs, register=s)
+        assert "lambda-only sweep" in capfd.readouterr().err
+        assert abs(e - e0) < 1e-11
+        assert np.max(np.abs(g - g0)) < 1e-10
+        exact = O.apply_circuit(O.zero_state(n), c.gate_list)
+        assert np.max(np.abs(s.get_vector() - exact)) < 1e-12
+        monkeypatch.setenv("QOMPSIM_ADJ_ENGINE", "pair")
+        e2, g2 = adjoint_gradient(c, obs)
+        assert "lambda-only sweep" not in capfd.readouterr().err
+        assert np.max(np.abs(g2 - g)) < 1e-10
+
+
 def _energy(c, obs, th):
     cp = c.copy()
     cp.set_parameters(th)
