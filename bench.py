@@ -570,6 +570,8 @@ def main():
                     help="skip the two rocprofv3 --pmc child runs that measure roofline.traffic")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="(internal) the child run profiled by the traffic measurement")
+    ap.add_argument("--stage-times", action="store_true",
+                    help="log every stage of the last timed step (ms, passes, compressed bytes in and out)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one 4 GiB device-to-device copy (counter calibration for profiling)")
     args = ap.parse_args()
@@ -654,6 +656,10 @@ def main():
     dom_bytes = sum(stage_bytes(s) for s in dom)
     dom_gbps = dom_bytes / dom_s / 1e9 if dom_s > 0 else 0.0
     compressed_only = sum(s["bytes_in"] + s["bytes_out"] for s in dom) / dom_s / 1e9 if dom_s > 0 else 0.0
+    if args.stage_times:
+        for i, st in enumerate(stats[-n_stages:]):
+            log(f"stage {i:2d}: {st['gpu_seconds'] * 1e3:8.2f} ms, {int(st['passes'])} pass(es), "
+                f"{st['bytes_in'] / 1e9:6.2f} -> {st['bytes_out'] / 1e9:6.2f} GB compressed")
     hist = cs.get_bytes_history()
     raw = 16.0 * (1 << n)
     max_bytes = max(hist[-(n_stages + 1):])
