@@ -960,6 +960,114 @@ static void set_tile_permutation(PassArgs& a, const std::vector<std::pair<uint32
   a.tperm = 1;
 }
 
+// M (4x4, index bit 0 <-> the first target) as (A (x) B) diag(d): A on the
+// second target, B on the first, each column c of M being d_c times
+// A[:, c >> 1] (x) B[:, c & 1].  The columns of A and B are read off M's
+// columns, scaled so that their diagonal entries are real and non-negative, d
+// follows by projection, and the product must give M back to 1e-14.
+static bool factor_product(const cplx* M, cplx* A, cplx* Bm, cplx* d) {
+  auto K = [&](int c, int r1, int r0) { return M[(2 * r1 + r0) * 4 + c]; };
+  cplx a[2][2], b[2][2];
+  for (int c1 = 0; c1 < 2; ++c1) {
+    const int c = 2 * c1;
+    const int r0 = std::norm(K(c, 0, 1)) + std::norm(K(c, 1, 1)) > std::norm(K(c, 0, 0)) + std::norm(K(c, 1, 0)) ? 1 : 0;
+    const double nn = std::sqrt(std::norm(K(c, 0, r0)) + std::norm(K(c, 1, r0)));
+    if (!(nn > 1e-150)) return false;
+    a[c1][0] = K(c, 0, r0) / nn;
+    a[c1][1] = K(c, 1, r0) / nn;
+  }
+  for (int c0 = 0; c0 < 2; ++c0) {
+    const int c = c0;
+    const int r1 = std::norm(K(c, 1, 0)) + std::norm(K(c, 1, 1)) > std::norm(K(c, 0, 0)) + std::norm(K(c, 0, 1)) ? 1 : 0;
+    const double nn = std::sqrt(std::norm(K(c, r1, 0)) + std::norm(K(c, r1, 1)));
+    if (!(nn > 1e-150)) return false;
+    b[c0][0] = K(c, r1, 0) / nn;
+    b[c0][1] = K(c, r1, 1) / nn;
+  }
+  for (int k = 0; k < 2; ++k) {
+    if (std::abs(a[k][k]) > 1e-12) {
+      const cplx ph = std::conj(a[k][k]) / std::abs(a[k][k]);
+      a[k][0] *= ph;
+      a[k][1] *= ph;
+    }
+    if (std::abs(b[k][k]) > 1e-12) {
+      const cplx ph = std::conj(b[k][k]) / std::abs(b[k][k]);
+      b[k][0] *= ph;
+      b[k][1] *= ph;
+    }
+  }
+  for (int c = 0; c < 4; ++c) {
+    const int c1 = c >> 1, c0 = c & 1;
+    cplx s = 0.0;
+    for (int r1 = 0; r1 < 2; ++r1)
+      for (int r0 = 0; r0 < 2; ++r0) s += std::conj(a[c1][r1] * b[c0][r0]) * K(c, r1, r0);
+    d[c] = s;
+    for (int r1 = 0; r1 < 2; ++r1)
+      for (int r0 = 0; r0 < 2; ++r0)
+        if (std::abs(K(c, r1, r0) - s * a[c1][r1] * b[c0][r0]) > 1e-14) return false;
+  }
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c) {
+      A[r * 2 + c] = a[c][r];
+      Bm[r * 2 + c] = b[c][r];
+    }
+  return true;
+}
+
+// A two-target dense gate whose matrix is a product of one-qubit matrices and
+// a diagonal, M = (A (x) B) D or M = D (A (x) B), becomes the diagonal and the
+// two one-qubit gates (QuantumCircuitOptimizer merges RZZ with the RX gates on
+// its qubits into such matrices): two one-target products and a phase op
+// instead of a 4x4 complex product per four amplitudes, and the one-qubit
+// gates may take different passes.  QOMPSIM_SPLIT_PRODUCTS=0 keeps the 4x4
+// matrices.
+static std::vector<LGate> split_product_gates(const std::vector<LGate>& lg) {
+  const char* e = std::getenv("QOMPSIM_SPLIT_PRODUCTS");
+  if (e && *e == '0') return lg;
+  std::vector<LGate> out;
+  out.reserve(lg.size());
+  for (const LGate& g : lg) {
+    if (g.grad >= 0 || g.noop || g.diag || g.N.size() != 2 || !g.S.empty() || g.mats.size() != 16 ||
+        is_swap_gate(g)) {
+      out.push_back(g);
+      continue;
+    }
+    cplx A[4], Bm[4], dg[4];
+    bool diag_first = factor_product(g.mats.data(), A, Bm, dg);
+    if (!diag_first) {
+      cplx Mt[16];
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) Mt[c * 4 + r] = g.mats[r * 4 + c];
+      cplx At[4], Bt[4];
+      if (!factor_product(Mt, At, Bt, dg)) {
+        out.push_back(g);
+        continue;
+      }
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) {
+          A[r * 2 + c] = At[c * 2 + r];
+          Bm[r * 2 + c] = Bt[c * 2 + r];
+        }
+    }
+    LGate D;
+    D.diag = true;
+    D.Q = {g.N[0], g.N[1]};
+    D.d.assign(dg, dg + 4);
+    auto one = [&](uint32_t q, const cplx* m) {
+      LGate G;
+      G.N = {q};
+      G.mats.assign(m, m + 4);
+      if (is_identity(m, 2)) G.ident = 1;
+      return G;
+    };
+    if (diag_first) out.push_back(D);
+    out.push_back(one(g.N[1], A));
+    out.push_back(one(g.N[0], Bm));
+    if (!diag_first) out.push_back(D);
+  }
+  return out;
+}
+
 static void plan_program(Program& P, const std::vector<LGate>& lg0, uint32_t L, uint32_t tmax, uint32_t cmin,
                          bool at_least_one_pass, bool codec_ends = false) {
   // Trailing SWAP gates of a compressed program move data without
@@ -970,7 +1078,7 @@ static void plan_program(Program& P, const std::vector<LGate>& lg0, uint32_t L, 
   //   single-pass program, they become a permutation of the tile in the
   //   encoder's layout transpose.  Such swaps are disjoint from every other
   //   folded swap, so the order of the two permutations does not matter.
-  std::vector<LGate> lg = lg0;
+  std::vector<LGate> lg = split_product_gates(lg0);
   std::vector<std::pair<uint32_t, uint32_t>> swaps, tswaps;
   std::vector<LGate> tswap_gates;
   const char* es = std::getenv("QOMPSIM_STORE_SWAPS");

@@ -7,7 +7,7 @@ import pytest
 
 import pyoracle as O
 from gpu_util import rand_state, random_circuit
-from qompsim import Observable, QuantumState, inner_product
+from qompsim import Observable, QuantumCircuit, QuantumState, inner_product
 from qompsim import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -194,3 +194,35 @@ def test_diagonal_gates_gathered_within_passes(rng, monkeypatch):
     assert np.max(np.abs(v1 - v2)) < TOL
     assert np.max(np.abs(v1 - O.apply_circuit(psi, c.gate_list))) < TOL
     assert abs(e1 - e2) < TOL and np.max(np.abs(np.asarray(g1) - np.asarray(g2))) < TOL
+
+
+@pytest.mark.parametrize("n", [12, 16])
+def test_product_matrices_split(n, monkeypatch):
+    """Two-qubit matrices that factor as (A x B) D or D (A x B) (the
+    optimizer's merge of RZZ with RX gates, and of phases after one-qubit
+    gates) are applied as the diagonal and two one-qubit gates: the oracle's
+    amplitudes within 1e-12, and within 1e-13 of the 4x4 products
+    (QOMPSIM_SPLIT_PRODUCTS=0)."""
+    from qompsim import QuantumCircuitOptimizer
+
+    rng = np.random.default_rng(n)
+    c = QuantumCircuit(n)
+    for i in range(n - 1):
+        c.add_multi_Pauli_rotation_gate([i, i + 1], [3, 3], float(rng.uniform(0, 6.3)))
+    for q in range(n):
+        c.add_RX_gate(q, float(rng.uniform(0, 6.3)))
+    for q in range(0, n - 1, 2):
+        c.add_RY_gate(q, float(rng.uniform(0, 6.3)))
+        c.add_gate(W.cphase(q, q + 1, float(rng.uniform(0, 6.3))))
+    QuantumCircuitOptimizer().optimize(c, 2)
+    psi = rand_state(n, rng)
+    s1 = QuantumState(n)
+    s1.load(psi)
+    c.update_quantum_state(s1)
+    monkeypatch.setenv("QOMPSIM_SPLIT_PRODUCTS", "0")
+    s2 = QuantumState(n)
+    s2.load(psi)
+    c.update_quantum_state(s2)
+    ref = O.apply_circuit(psi, c.gate_list)
+    assert np.max(np.abs(s1.get_vector() - ref)) < 1e-12
+    assert np.max(np.abs(s1.get_vector() - s2.get_vector())) < 1e-13
