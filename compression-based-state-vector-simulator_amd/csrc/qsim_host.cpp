@@ -1019,14 +1019,69 @@ static bool factor_product(const cplx* M, cplx* A, cplx* Bm, cplx* d) {
 // two one-qubit gates (QuantumCircuitOptimizer merges RZZ with the RX gates on
 // its qubits into such matrices): two one-target products and a phase op
 // instead of a 4x4 complex product per four amplitudes, and the one-qubit
-// gates may take different passes.  QOMPSIM_SPLIT_PRODUCTS=0 keeps the 4x4
-// matrices.
+// gates may take different passes.  Likewise a one-target gate whose matrices
+// for the selector values differ by a diagonal factor (the optimizer's H(j)
+// merged with CP(j, k): H on j selected by k) becomes that diagonal and one
+// matrix without selectors, which may be real.  QOMPSIM_SPLIT_PRODUCTS=0 keeps
+// the merged matrices.
 static std::vector<LGate> split_product_gates(const std::vector<LGate>& lg) {
   const char* e = std::getenv("QOMPSIM_SPLIT_PRODUCTS");
   if (e && *e == '0') return lg;
   std::vector<LGate> out;
   out.reserve(lg.size());
   for (const LGate& g : lg) {
+    if (g.grad < 0 && !g.noop && !g.diag && g.N.size() == 1 && !g.S.empty() &&
+        g.mats.size() == ((size_t)4 << g.S.size())) {
+      // one target with selectors: M_s = B D_s (or D_s B) for every selector
+      // value s, with B = M_0 and D_s diagonal, is the diagonal over the
+      // target and the selectors, then B without selectors
+      const size_t ns = g.mats.size() / 4;
+      const cplx* B0 = g.mats.data();
+      const cplx det = B0[0] * B0[3] - B0[1] * B0[2];
+      bool done = false;
+      if (std::abs(det) > 1e-12) {
+        const cplx Bi[4] = {B0[3] / det, -B0[1] / det, -B0[2] / det, B0[0] / det};
+        for (int after = 0; after < 2 && !done; ++after) {
+          std::vector<cplx> dd(2 * ns);
+          bool ok = true;
+          for (size_t sv = 0; sv < ns && ok; ++sv) {
+            const cplx* Ms = &g.mats[4 * sv];
+            cplx Ds[4];
+            for (int r = 0; r < 2; ++r)
+              for (int c = 0; c < 2; ++c)
+                Ds[r * 2 + c] = after ? Ms[r * 2] * Bi[c] + Ms[r * 2 + 1] * Bi[2 + c]   // M_s B^-1
+                                      : Bi[r * 2] * Ms[c] + Bi[r * 2 + 1] * Ms[2 + c];  // B^-1 M_s
+            if (std::abs(Ds[1]) > 1e-14 || std::abs(Ds[2]) > 1e-14) ok = false;
+            dd[2 * sv] = Ds[0];
+            dd[2 * sv + 1] = Ds[3];
+          }
+          if (!ok) continue;
+          // the product must give every M_s back
+          for (size_t sv = 0; sv < ns && ok; ++sv)
+            for (int r = 0; r < 2; ++r)
+              for (int c = 0; c < 2; ++c) {
+                const cplx v = after ? dd[2 * sv + r] * B0[r * 2 + c] : B0[r * 2 + c] * dd[2 * sv + c];
+                if (std::abs(v - g.mats[4 * sv + r * 2 + c]) > 1e-14) ok = false;
+              }
+          if (!ok) continue;
+          LGate D;
+          D.diag = true;
+          D.Q.push_back(g.N[0]);
+          D.Q.insert(D.Q.end(), g.S.begin(), g.S.end());
+          D.d = dd;  // index bit 0 <-> the target, bit 1 + j <-> selector j
+          LGate Bg;
+          Bg.N = g.N;
+          Bg.mats.assign(B0, B0 + 4);
+          if (is_identity(B0, 2)) Bg.ident = 1;
+          if (!after) out.push_back(D);
+          out.push_back(Bg);
+          if (after) out.push_back(D);
+          done = true;
+        }
+      }
+      if (!done) out.push_back(g);
+      continue;
+    }
     if (g.grad >= 0 || g.noop || g.diag || g.N.size() != 2 || !g.S.empty() || g.mats.size() != 16 ||
         is_swap_gate(g)) {
       out.push_back(g);

@@ -215,14 +215,18 @@ def test_product_matrices_split(n, monkeypatch):
         c.add_RY_gate(q, float(rng.uniform(0, 6.3)))
         c.add_gate(W.cphase(q, q + 1, float(rng.uniform(0, 6.3))))
     QuantumCircuitOptimizer().optimize(c, 2)
-    psi = rand_state(n, rng)
-    s1 = QuantumState(n)
-    s1.load(psi)
-    c.update_quantum_state(s1)
-    monkeypatch.setenv("QOMPSIM_SPLIT_PRODUCTS", "0")
-    s2 = QuantumState(n)
-    s2.load(psi)
-    c.update_quantum_state(s2)
-    ref = O.apply_circuit(psi, c.gate_list)
-    assert np.max(np.abs(s1.get_vector() - ref)) < 1e-12
-    assert np.max(np.abs(s1.get_vector() - s2.get_vector())) < 1e-13
+    qft = W.qft_on_product(n, seed=n)  # H merged with controlled phases: selected one-qubit matrices
+    QuantumCircuitOptimizer().optimize(qft, 2)
+    for circ in (c, qft):
+        monkeypatch.delenv("QOMPSIM_SPLIT_PRODUCTS", raising=False)
+        psi = rand_state(n, rng)
+        s1 = QuantumState(n)
+        s1.load(psi)
+        circ.update_quantum_state(s1)
+        monkeypatch.setenv("QOMPSIM_SPLIT_PRODUCTS", "0")
+        s2 = QuantumState(n)
+        s2.load(psi)
+        circ.update_quantum_state(s2)
+        ref = O.apply_circuit(psi, circ.gate_list)
+        assert np.max(np.abs(s1.get_vector() - ref)) < 1e-12
+        assert np.max(np.abs(s1.get_vector() - s2.get_vector())) < 1e-13
