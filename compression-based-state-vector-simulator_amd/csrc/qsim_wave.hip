@@ -1666,63 +1666,73 @@ __global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kerne
 #pragma unroll
   for (int v = 0; v < KV; ++v) amp[v] = make_double2(sv[2 * roff(v)], sv[2 * roff(v) + 1]);
   uint32_t pb = 0;
-  double2* xb = pbuf[0];
-  for (uint32_t o = a.op_begin; o < a.op_end; ++o) {
+  for (uint32_t o = a.op_begin; o < a.op_end;) {
     const QC Op& op = ops[o];
     if (op.kind == OP_DENSE) {
       wave_dense(op, amp, cx, coefs);
-    } else if (op.kind == OP_ELEM) {
+      ++o;
+      continue;
+    }
+    if (op.kind == OP_ELEM) {
       wave_elem(op, amp, cx, terms, diags, coefs);
-    } else {
-      // adjacent points see the same psi tile: it is parked once per run
-      if (o == a.op_begin || ops[o - 1].kind != OP_GRAD) {
-        xb = pbuf[pb];
-        pb ^= 1u;
-        if (!is_lam) {
+      ++o;
+      continue;
+    }
+    // a run of adjacent points sees one psi tile, parked once.  The whole run
+    // is taken in this iteration: the points leave the amplitudes as they
+    // are, so a run costs one pass through the loop's latch (where the
+    // amplitudes return to the registers the loop header holds them in)
+    // instead of one per point.
+    double2* xb = pbuf[pb];
+    pb ^= 1u;
+    if (!is_lam) {
 #pragma unroll
-          for (int v = 0; v < KV; ++v) xb[64 * v + lane] = amp[v];
+      for (int v = 0; v < KV; ++v) xb[64 * v + lane] = amp[v];
+    }
+    __syncthreads();
+    if (!is_lam) {
+      while (o < a.op_end && ops[o].kind == OP_GRAD) ++o;
+      continue;
+    }
+    for (; o < a.op_end && ops[o].kind == OP_GRAD; ++o) {
+      const QC Op& pt = ops[o];
+      // Re(i^kr conj(l) x) times the Z/Y sign (-1)^pc(q & zt) of the
+      // partner index q: i^kr picks Re(conj(l) x) or Im(conj(l) x) and an
+      // overall sign; the per-element sign is a flip of the high dword
+      const uint32_t fx = (uint32_t)pt.gfx, zt = (uint32_t)pt.gzt;
+      const uint32_t kr = (1u + pt.nY) & 3u;  // i * i^nY = i^kr
+      const bool flip_all = (__popcll(base & pt.gzb) & 1) ^ (kr == 1u || kr == 2u);
+      const uint32_t lsgn = (uint32_t)(__popc(((uint32_t)lane ^ fx) & zt & 63u) & 1) << 31;
+      double acc0 = 0.0, acc1 = 0.0;
+      if ((kr & 1u) == 0u) {
+#pragma unroll
+        for (int v = 0; v < KV; ++v) {
+          const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
+          const double2 x = xb[q], l = amp[v];
+          const uint32_t sg = lsgn ^ ((uint32_t)(__popc((((uint32_t)v << 6) ^ (fx & ~63u)) & zt) & 1) << 31);
+          double t = fma(l.x, x.x, l.y * x.y);
+          t = __longlong_as_double(__double_as_longlong(t) ^ ((long long)sg << 32));
+          if (v & 1) acc1 += t;
+          else acc0 += t;
         }
-        __syncthreads();
+      } else {
+#pragma unroll
+        for (int v = 0; v < KV; ++v) {
+          const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
+          const double2 x = xb[q], l = amp[v];
+          const uint32_t sg = lsgn ^ ((uint32_t)(__popc((((uint32_t)v << 6) ^ (fx & ~63u)) & zt) & 1) << 31);
+          double t = fma(l.x, x.y, -(l.y * x.x));
+          t = __longlong_as_double(__double_as_longlong(t) ^ ((long long)sg << 32));
+          if (v & 1) acc1 += t;
+          else acc0 += t;
+        }
       }
-      if (is_lam) {
-        // Re(i^kr conj(l) x) times the Z/Y sign (-1)^pc(q & zt) of the
-        // partner index q: i^kr picks Re(conj(l) x) or Im(conj(l) x) and an
-        // overall sign; the per-element sign is a flip of the high dword
-        const uint32_t fx = (uint32_t)op.gfx, zt = (uint32_t)op.gzt;
-        const uint32_t kr = (1u + op.nY) & 3u;  // i * i^nY = i^kr
-        const bool flip_all = (__popcll(base & op.gzb) & 1) ^ (kr == 1u || kr == 2u);
-        const uint32_t lsgn = (uint32_t)(__popc(((uint32_t)lane ^ fx) & zt & 63u) & 1) << 31;
-        double acc0 = 0.0, acc1 = 0.0;
-        if ((kr & 1u) == 0u) {
-#pragma unroll
-          for (int v = 0; v < KV; ++v) {
-            const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
-            const double2 x = xb[q], l = amp[v];
-            const uint32_t sg = lsgn ^ ((uint32_t)(__popc((((uint32_t)v << 6) ^ (fx & ~63u)) & zt) & 1) << 31);
-            double t = fma(l.x, x.x, l.y * x.y);
-            t = __longlong_as_double(__double_as_longlong(t) ^ ((long long)sg << 32));
-            if (v & 1) acc1 += t;
-            else acc0 += t;
-          }
-        } else {
-#pragma unroll
-          for (int v = 0; v < KV; ++v) {
-            const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
-            const double2 x = xb[q], l = amp[v];
-            const uint32_t sg = lsgn ^ ((uint32_t)(__popc((((uint32_t)v << 6) ^ (fx & ~63u)) & zt) & 1) << 31);
-            double t = fma(l.x, x.y, -(l.y * x.x));
-            t = __longlong_as_double(__double_as_longlong(t) ^ ((long long)sg << 32));
-            if (v & 1) acc1 += t;
-            else acc0 += t;
-          }
-        }
-        // tile sum: the lanes' sums added in a DPP tree
-        double tot = __shfl(wave_sum_f64(acc0 + acc1), 63, 64);
-        tot = flip_all ? -tot : tot;
-        if (lane == 0) {
-          partials[((uint64_t)op.gidx * ntiles + wt) * 2] = tot;
-          partials[((uint64_t)op.gidx * ntiles + wt) * 2 + 1] = 0.0;
-        }
+      // tile sum: the lanes' sums added in a DPP tree
+      double tot = __shfl(wave_sum_f64(acc0 + acc1), 63, 64);
+      tot = flip_all ? -tot : tot;
+      if (lane == 0) {
+        partials[((uint64_t)pt.gidx * ntiles + wt) * 2] = tot;
+        partials[((uint64_t)pt.gidx * ntiles + wt) * 2 + 1] = 0.0;
       }
     }
   }
