@@ -997,6 +997,73 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
   return x;
 }
 
+// ---------------------------------------------- points of the paired sweep
+// Products of a gradient point on the paired kernel's lambda wave: element v
+// of the lane pairs lambda row v with the parked psi element at row v ^ R,
+// lane lane ^ (flip mask & 63).  t[v] = Re(conj(l) x) (im false) or
+// Im(conj(l) x) (im true).  R is a template argument so that every LDS read
+// has a constant offset from the lane's address.
+template <int KV, int R>
+__device__ __forceinline__ void pair_point_products(const double2* xr, const double2 (&amp)[KV], bool im,
+                                                    double (&t)[KV]) {
+  if (im) {
+#pragma unroll
+    for (int v = 0; v < KV; ++v) {
+      const double2 x = xr[64 * (v ^ R)], l = amp[v];
+      t[v] = fma(l.x, x.y, -(l.y * x.x));
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < KV; ++v) {
+      const double2 x = xr[64 * (v ^ R)], l = amp[v];
+      t[v] = fma(l.x, x.x, l.y * x.y);
+    }
+  }
+}
+template <int KV>
+__device__ __forceinline__ void pair_point_products_r(const double2* xr, const double2 (&amp)[KV], uint32_t r, bool im,
+                                                      double (&t)[KV]) {
+  switch (r) {
+    case 0: pair_point_products<KV, 0>(xr, amp, im, t); break;
+    case 1: pair_point_products<KV, 1>(xr, amp, im, t); break;
+    case 2: pair_point_products<KV, 2>(xr, amp, im, t); break;
+    case 3: pair_point_products<KV, 3>(xr, amp, im, t); break;
+    case 4: pair_point_products<KV, 4>(xr, amp, im, t); break;
+    case 5: pair_point_products<KV, 5>(xr, amp, im, t); break;
+    case 6: pair_point_products<KV, 6>(xr, amp, im, t); break;
+    case 7: pair_point_products<KV, 7>(xr, amp, im, t); break;
+    default:
+      if constexpr (KV == 16) {
+        switch (r) {
+          case 8: pair_point_products<KV, 8>(xr, amp, im, t); break;
+          case 9: pair_point_products<KV, 9>(xr, amp, im, t); break;
+          case 10: pair_point_products<KV, 10>(xr, amp, im, t); break;
+          case 11: pair_point_products<KV, 11>(xr, amp, im, t); break;
+          case 12: pair_point_products<KV, 12>(xr, amp, im, t); break;
+          case 13: pair_point_products<KV, 13>(xr, amp, im, t); break;
+          case 14: pair_point_products<KV, 14>(xr, amp, im, t); break;
+          default: pair_point_products<KV, 15>(xr, amp, im, t); break;
+        }
+      }
+      break;
+  }
+}
+// sum over v of (-1)^popc(v & zr) t[v]: one level per row bit, the rows with
+// the bit set taken with the sign of zr's bit (a Walsh coefficient of t)
+template <int KV>
+__device__ __forceinline__ double walsh_fold(const double (&t)[KV], uint32_t zr) {
+  double f[KV];
+#pragma unroll
+  for (int v = 0; v < KV; ++v) f[v] = t[v];
+#pragma unroll
+  for (int h = 1; h < KV; h <<= 1) {
+    const double sg = (zr & (uint32_t)h) ? -1.0 : 1.0;
+#pragma unroll
+    for (int v = 0; v < KV; v += 2 * h) f[v] = fma(sg, f[v + h], f[v]);
+  }
+  return f[0];
+}
+
 // ---------------------------------------------- points of lambda-only passes
 // Rows of one gradient point Re<lambda| i P |psi> on a tile (lambda in lam,
 // psi in ps at the same positions): P flips tile rows by FR and lanes by fl,
@@ -1694,41 +1761,27 @@ __global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kerne
       while (o < a.op_end && ops[o].kind == OP_GRAD) ++o;
       continue;
     }
+    // point P: Re(i^kr conj(l) x) over the tile, x the psi partner at index
+    // q = (64 v + lane) ^ fx, times the Z/Y sign (-1)^popc(q & zt).  With r
+    // and lf the row and lane parts of fx and zr, zl those of zt, that sign
+    // is (-1)^(popc(v & zr) + popc(r & zr) + popc((lane ^ lf) & zl)): the
+    // first factor is a Walsh fold of the row products, the second uniform,
+    // the third per lane.  i^kr picks Re(conj(l) x) or Im(conj(l) x) and an
+    // overall sign.
     for (; o < a.op_end && ops[o].kind == OP_GRAD; ++o) {
       const QC Op& pt = ops[o];
-      // Re(i^kr conj(l) x) times the Z/Y sign (-1)^pc(q & zt) of the
-      // partner index q: i^kr picks Re(conj(l) x) or Im(conj(l) x) and an
-      // overall sign; the per-element sign is a flip of the high dword
       const uint32_t fx = (uint32_t)pt.gfx, zt = (uint32_t)pt.gzt;
       const uint32_t kr = (1u + pt.nY) & 3u;  // i * i^nY = i^kr
-      const bool flip_all = (__popcll(base & pt.gzb) & 1) ^ (kr == 1u || kr == 2u);
-      const uint32_t lsgn = (uint32_t)(__popc(((uint32_t)lane ^ fx) & zt & 63u) & 1) << 31;
-      double acc0 = 0.0, acc1 = 0.0;
-      if ((kr & 1u) == 0u) {
-#pragma unroll
-        for (int v = 0; v < KV; ++v) {
-          const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
-          const double2 x = xb[q], l = amp[v];
-          const uint32_t sg = lsgn ^ ((uint32_t)(__popc((((uint32_t)v << 6) ^ (fx & ~63u)) & zt) & 1) << 31);
-          double t = fma(l.x, x.x, l.y * x.y);
-          t = __longlong_as_double(__double_as_longlong(t) ^ ((long long)sg << 32));
-          if (v & 1) acc1 += t;
-          else acc0 += t;
-        }
-      } else {
-#pragma unroll
-        for (int v = 0; v < KV; ++v) {
-          const uint32_t q = (((uint32_t)v << 6) | (uint32_t)lane) ^ fx;
-          const double2 x = xb[q], l = amp[v];
-          const uint32_t sg = lsgn ^ ((uint32_t)(__popc((((uint32_t)v << 6) ^ (fx & ~63u)) & zt) & 1) << 31);
-          double t = fma(l.x, x.y, -(l.y * x.x));
-          t = __longlong_as_double(__double_as_longlong(t) ^ ((long long)sg << 32));
-          if (v & 1) acc1 += t;
-          else acc0 += t;
-        }
-      }
+      const uint32_t r = (fx >> 6) & (uint32_t)(KV - 1), lf = fx & 63u;
+      const uint32_t zr = (zt >> 6) & (uint32_t)(KV - 1), zl = zt & 63u;
+      double t[KV];
+      pair_point_products_r<KV>(xb + ((uint32_t)lane ^ lf), amp, r, (kr & 1u) != 0u, t);
+      const double w = walsh_fold<KV>(t, zr);
+      const bool lneg = __popc(((uint32_t)lane ^ lf) & zl) & 1;
+      const bool flip_all =
+          (__popcll(base & pt.gzb) & 1) ^ (kr == 1u || kr == 2u) ^ (__popc(r & zr) & 1);
       // tile sum: the lanes' sums added in a DPP tree
-      double tot = __shfl(wave_sum_f64(acc0 + acc1), 63, 64);
+      double tot = __shfl(wave_sum_f64(lneg ? -w : w), 63, 64);
       tot = flip_all ? -tot : tot;
       if (lane == 0) {
         partials[((uint64_t)pt.gidx * ntiles + wt) * 2] = tot;
