@@ -827,16 +827,32 @@ __global__ __launch_bounds__(kThreads) void pauli_sum_kernel(const double2* __re
     for (uint32_t g = 0; g < ng; ++g) {
       const PauliGroup G = groups[g];
       const uint64_t y = x ^ G.flip;
+      // the group's coefficient at y: the terms' coefficients with the signs
+      // of their Z bits.  Term fields are wave-uniform, so terms without Z
+      // bits and real coefficients take branches that skip the parity and
+      // the imaginary half.
       double fr = 0.0, fi = 0.0;
+      bool cplx = false;
       for (uint32_t t = G.t0; t < G.t0 + G.nt; ++t) {
         const PauliTermZ T = terms[t];
-        const bool neg = __popcll(y & T.z) & 1;
-        fr += neg ? -T.cr : T.cr;
-        fi += neg ? -T.ci : T.ci;
+        cplx |= T.ci != 0.0;
+        if (T.z == 0ull) {
+          fr += T.cr;
+          fi += T.ci;
+          continue;
+        }
+        const double sg = (__popcll(y & T.z) & 1) ? -1.0 : 1.0;
+        fr = fma(sg, T.cr, fr);
+        if (T.ci != 0.0) fi = fma(sg, T.ci, fi);
       }
       const double2 v = a[y];
-      re += fr * v.x - fi * v.y;
-      im += fr * v.y + fi * v.x;
+      if (cplx) {
+        re += fr * v.x - fi * v.y;
+        im += fr * v.y + fi * v.x;
+      } else {
+        re = fma(fr, v.x, re);
+        im = fma(fr, v.y, im);
+      }
     }
     const double2 ax = a[x];
     if constexpr (STORE) {
