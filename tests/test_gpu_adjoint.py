@@ -181,6 +181,33 @@ def _two_qubit_circuit(n, rng, layers=3):
     return c
 
 
+def test_paired_points_all_placements(rng, monkeypatch):
+    """Two-qubit Pauli rotation points on the paired kernel (12 qubits: the
+    1024-amplitude wave tile; every gate has at most two non-diagonal
+    targets) for qubit pairs on lane bits, register rows and base bits and
+    every ordered Pauli pair: the Z/Y signs split into a row fold, a uniform
+    part and a lane part, and the flips into a row flip and a lane flip.
+    Gradients within 1e-10 of the oracle's backprop."""
+    monkeypatch.delenv("QOMPSIM_ADJ_ENGINE", raising=False)
+    n = 12
+    c = ParametricQuantumCircuit(n)
+    for q in range(n):
+        c.add_parametric_RY_gate(q, rng.uniform(0, 2 * np.pi))
+        c.add_parametric_RX_gate(q, rng.uniform(0, 2 * np.pi))
+    for q in range(n - 1):
+        c.add_gate(G.CNOT(q, q + 1))
+    pairs = [(0, 1), (2, 7), (6, 8), (3, 11), (9, 10), (5, 10), (7, 4), (11, 6)]
+    for i, (a, b) in enumerate(pairs):
+        for pa, pb in ((1, 2), (2, 3), (3, 1), (1, 1), (2, 2), (3, 3), (2, 1), (3, 2)):
+            c.add_parametric_multi_Pauli_rotation_gate([a, b], [pa, pb], rng.uniform(0, 2 * np.pi))
+        c.add_gate(G.H((i * 5) % n))
+    obs = _mixed_observable(n, rng)
+    e, g = adjoint_gradient(c, obs)
+    e0, g0 = O.backprop(c, obs)
+    assert abs(e - e0) < 1e-11
+    assert np.max(np.abs(g - g0)) < 1e-10
+
+
 @pytest.mark.parametrize("n", [10, 12, 15])
 def test_lambda_only_sweep_matches_oracle(n, rng, monkeypatch, capfd):
     """The lambda-only reverse sweep (QOMPSIM_ADJ_ENGINE=lam: psi kept at
