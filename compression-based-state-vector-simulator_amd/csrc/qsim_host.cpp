@@ -2974,20 +2974,28 @@ static void reverse_sweep(qsim_sv* sv, double2* psi_w, double2* lam, const std::
   P.upload_all();
   uint64_t maxtiles = 1;
   for (size_t pi = 0; pi < P.passes.size(); ++pi) maxtiles = std::max(maxtiles, reverse_tiles(P, pi, sv->n));
+  // each pass's reduced sums go to their own slots of adj_out; one copy and
+  // one wait after the last pass, then the merges in pass order
+  std::vector<uint64_t> out_off(P.passes.size() + 1, 0);
+  for (size_t pi = 0; pi < P.passes.size(); ++pi) out_off[pi + 1] = out_off[pi] + params[pi].size();
+  const uint64_t nout = out_off.back();
   sv->adj_part.ensure(std::max<uint64_t>(1, (uint64_t)maxpts * maxtiles) * 2 * sizeof(double));
-  sv->adj_out.ensure(std::max<uint32_t>(1, maxpts) * 2 * sizeof(double));
+  sv->adj_out.ensure(std::max<uint64_t>(1, nout) * 2 * sizeof(double));
   sv->adj_stage.ensure((uint64_t)std::max<uint32_t>(1, maxpts) * kGradReduceChunks * 2 * sizeof(double));
-  std::vector<double> hout(2 * std::max<uint32_t>(1, maxpts));
   for (size_t pi = 0; pi < P.passes.size(); ++pi) {
     HIPCHECK(launch_reverse_pass(P, pi, sv->n, psi_w, lam, sv->adj_part.as<double>(), stream()));
     const uint32_t np = (uint32_t)params[pi].size();
     if (np == 0) continue;
-    HIPCHECK(launch_grad_reduce(sv->adj_part.as<double>(), reverse_tiles(P, pi, sv->n), np, sv->adj_out.as<double>(),
-                                sv->adj_stage.as<double>(), stream()));
-    HIPCHECK(hipMemcpyAsync(hout.data(), sv->adj_out.p, np * 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
-    HIPCHECK(hipStreamSynchronize(stream()));
-    for (uint32_t k = 0; k < np; ++k) gacc[params[pi][k]].merge(hout[2 * k], hout[2 * k + 1]);
+    HIPCHECK(launch_grad_reduce(sv->adj_part.as<double>(), reverse_tiles(P, pi, sv->n), np,
+                                sv->adj_out.as<double>() + 2 * out_off[pi], sv->adj_stage.as<double>(), stream()));
   }
+  if (nout == 0) return;
+  std::vector<double> hout(2 * nout);
+  HIPCHECK(hipMemcpyAsync(hout.data(), sv->adj_out.p, nout * 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  HIPCHECK(hipStreamSynchronize(stream()));
+  for (size_t pi = 0; pi < P.passes.size(); ++pi)
+    for (uint32_t k = 0; k < (uint32_t)params[pi].size(); ++k)
+      gacc[params[pi][k]].merge(hout[2 * (out_off[pi] + k)], hout[2 * (out_off[pi] + k) + 1]);
 }
 
 // ------------------------------------------------ lambda-only reverse sweep
