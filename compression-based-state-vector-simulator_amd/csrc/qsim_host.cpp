@@ -3349,6 +3349,21 @@ int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** 
   if (sbytes == 0) {
     const char* e = std::getenv("QOMPSIM_SCRATCH_GIB");
     sbytes = e ? (uint64_t)(std::atof(e) * (1ull << 30)) : (8ull << 30);
+    if (!e && !cs->regions) {
+      // The double store's multi-pass stages stream their groups through
+      // this buffer in batches: larger batches are fewer, longer launches
+      // (QFT-32's five-pass stage: 156 ms at 8 GiB, 143 ms with the whole
+      // 64 GiB state in one batch).  Default: a quarter of the free device
+      // memory (and of the HBM budget), a power of two, at least 8 GiB.
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+        uint64_t q = fr / 4;
+        if (c.hbm_budget) q = std::min<uint64_t>(q, c.hbm_budget / 4);
+        uint64_t p2 = 1;
+        while (p2 <= q / 2) p2 <<= 1;
+        sbytes = std::max<uint64_t>(sbytes, p2);
+      }
+    }
   }
   sbytes = std::min<uint64_t>(sbytes, raw);
   sbytes = std::max<uint64_t>(sbytes, (1ull << B) * 16);
