@@ -46,6 +46,7 @@ hipError_t launch_sv_pauli(const double2* a, uint64_t n, uint64_t f, uint64_t z,
                            hipStream_t st);
 hipError_t launch_sv_apply_pauli(const double2* a, double2* out, uint64_t n, uint64_t f, uint64_t z, double cr,
                                  double ci, int acc, hipStream_t st);
+// out: 2 + 128 words (sum and maximum, then per-block partials)
 hipError_t launch_cursor_stats(const unsigned long long* cur, uint32_t n, unsigned long long* out, hipStream_t st);
 hipError_t launch_adjoint_pass(int tb, const PassArgs* d_args, double2* psi, double2* lam, double* partials,
                                uint64_t ntiles, hipStream_t st);
@@ -3392,7 +3393,7 @@ int qsim_cs_create(uint32_t n, uint32_t B, const qsim_cs_config* cfg, qsim_cs** 
   cs->null_flags.alloc(cs->nb + 8);  // whole dwords are read around a flag
   cs->nonzero.alloc(cs->nb);
   cs->overflow.alloc(16);
-  cs->cstats.alloc(16);
+  cs->cstats.alloc(8 * (2 + 2 * 64));  // sum and maximum, then 64 partial pairs
   cs->h_null.assign(cs->nb, 1);
   HIPCHECK(hipEventCreate(&cs->ev0));
   HIPCHECK(hipEventCreate(&cs->ev1));

@@ -507,11 +507,15 @@ __global__ __launch_bounds__(256) void block_bytes_kernel(const uint64_t* __rest
 }
 
 // sum and maximum of the allocation cursors (stride 16 words)
+// Sum and maximum of the n allocation cursors (one per 128-byte line):
+// kCursorBlocks work-groups each reduce a slice into a partial pair
+// (part[2b], part[2b + 1]); one more work-group combines the pairs into out.
+constexpr unsigned kCursorBlocks = 64;
 __global__ __launch_bounds__(256) void cursor_stats_kernel(const unsigned long long* __restrict__ cur, uint32_t n,
-                                                           unsigned long long* out) {
+                                                           unsigned long long* __restrict__ part) {
   __shared__ unsigned long long ss[256], sm[256];
   unsigned long long s = 0, m = 0;
-  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += kCursorBlocks * 256) {
     const unsigned long long v = cur[(uint64_t)i * 16];
     s += v;
     m = v > m ? v : m;
@@ -527,9 +531,20 @@ __global__ __launch_bounds__(256) void cursor_stats_kernel(const unsigned long l
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = ss[0];
-    out[1] = sm[0];
+    part[2 * blockIdx.x] = ss[0];
+    part[2 * blockIdx.x + 1] = sm[0];
   }
+}
+__global__ __launch_bounds__(64) void cursor_stats_final_kernel(const unsigned long long* __restrict__ part,
+                                                                unsigned long long* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  unsigned long long s = 0, m = 0;
+  for (unsigned b = 0; b < kCursorBlocks; ++b) {
+    s += part[2 * b];
+    m = part[2 * b + 1] > m ? part[2 * b + 1] : m;
+  }
+  out[0] = s;
+  out[1] = m;
 }
 
 // Descriptors of a launch's blocks in bmap order (the region store encodes
@@ -991,8 +1006,10 @@ hipError_t launch_block_bytes(const uint64_t* desc, const uint8_t* nullf, uint64
   return hipGetLastError();
 }
 
+// out: 2 + 2 kCursorBlocks words (the sum and the maximum, then the partials)
 hipError_t launch_cursor_stats(const unsigned long long* cur, uint32_t n, unsigned long long* out, hipStream_t st) {
-  hipLaunchKernelGGL(cursor_stats_kernel, dim3(1), dim3(256), 0, st, cur, n, out);
+  hipLaunchKernelGGL(cursor_stats_kernel, dim3(kCursorBlocks), dim3(256), 0, st, cur, n, out + 2);
+  hipLaunchKernelGGL(cursor_stats_final_kernel, dim3(1), dim3(64), 0, st, out + 2, out);
   return hipGetLastError();
 }
 
