@@ -497,7 +497,7 @@ __global__ __launch_bounds__(256) void block_bytes_kernel(const uint64_t* __rest
       for (uint64_t c = threadIdx.x; c < cpb; c += 256) s += desc_w(desc[b * cpb + c]);
     sh[threadIdx.x] = s;
     __syncthreads();
-    for (int d = 128; d >= 1; d >>= 1) {
+    for (unsigned d = 128; d >= 1; d >>= 1) {
       if (threadIdx.x < d) sh[threadIdx.x] += sh[threadIdx.x + d];
       __syncthreads();
     }
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(256) void cursor_stats_kernel(const unsigned long l
   ss[threadIdx.x] = s;
   sm[threadIdx.x] = m;
   __syncthreads();
-  for (int d = 128; d >= 1; d >>= 1) {
+  for (unsigned d = 128; d >= 1; d >>= 1) {
     if (threadIdx.x < d) {
       ss[threadIdx.x] += ss[threadIdx.x + d];
       sm[threadIdx.x] = sm[threadIdx.x] > sm[threadIdx.x + d] ? sm[threadIdx.x] : sm[threadIdx.x + d];

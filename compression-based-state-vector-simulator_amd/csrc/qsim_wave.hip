@@ -24,7 +24,6 @@ namespace qsim {
 
 constexpr int kRegBits = 4;
 constexpr int kV = 1 << kRegBits;  // complex values per lane
-constexpr int kWaveTileBits = 6 + kRegBits;
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
   const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, l);
@@ -189,7 +188,6 @@ template <int KV>
 constexpr uint32_t x_row_slots() { return 64u + (64u >> x_pad_shift<KV>()); }
 template <int KV>
 constexpr uint32_t x_slots() { return x_row_slots<KV>() * KV; }
-constexpr uint32_t kXSlots = x_slots<kV>();
 
 // wave layout (lane l, row v: tile position 64v + l) -> chunk layout
 // (lane L, row q: tile position KV L + q), through the wave's LDS buffer
@@ -1800,7 +1798,7 @@ __global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kerne
 // qsim_internal.h): one entry per thread, the angles of the op's matching
 // terms summed in term order and turned into cis().
 __global__ __launch_bounds__(kThreads) void phase_table_kernel(const PassArgs* __restrict__ pa,
-                                                               const Op* __restrict__ opp, uint64_t ntau,
+                                                               const Op* __restrict__ opp, [[maybe_unused]] uint64_t ntau,
                                                                uint64_t total, double2* __restrict__ ptab) {
   const PassArgs& a = *pa;
   const Op& op = *opp;
