@@ -1190,7 +1190,59 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   const bool pts = PT && a.op_begin < a.op_end && ops[a.op_begin].kind == OP_GRAD;
   // this lane's slot offset of an ENC store's tile permutation
   const uint32_t tperm_sl = OUT == OUT_ENC && a.tperm ? (uint32_t)a.tperm_lane[lane] : 0u;
-  auto tile_body = [&](const uint64_t wt, auto&& pre) {
+  // Chunk metadata of tile wt (chunk layout): the LPC lanes of chunk c all
+  // describe it; its first amplitude is tile position 32c.  The block-map
+  // load (meta_bpos) and the null-flag and descriptor loads that depend on it
+  // (meta_rest) are separate steps, so that the persistent DEC->ENC loop can
+  // issue a tile's block-map load one tile early and its other loads during
+  // the previous tile's encode.
+  struct TileMeta {
+    uint64_t desc;
+    uint32_t bid, ci, nul;
+  };
+  auto meta_loc = [&](const uint64_t wt) -> uint64_t {
+    const TileMapConst tm{TB, 6u, (uint32_t)RB, a.E};
+    return tm.base(wt & ((1ull << (a.L - TB)) - 1ull)) | tm.off(32u * ((uint32_t)lane / LPC));
+  };
+  auto meta_bpos = [&](const uint64_t wt) -> uint64_t {
+    return ((wt >> (a.L - TB)) << a.khi) + (meta_loc(wt) >> a.B);
+  };
+  auto meta_rest = [&](const uint64_t wt, const uint32_t bid) -> TileMeta {
+    const uint64_t slot = wt >> (a.L - TB), loc = meta_loc(wt);
+    const uint64_t bpos = (slot << a.khi) + (loc >> a.B);
+    TileMeta m{0ull, bid, (uint32_t)((loc & ((1ull << a.B) - 1ull)) >> kChunkAmpsLog2), 1u};
+    m.nul = null_in[bid];
+    if constexpr (IN == IN_DEC) {
+      // the descriptor table spans every block, so the descriptor load need
+      // not wait for the null flag (a null block's stale entry is discarded)
+      const uint64_t d = desc_in[(dy.desc_staged ? bpos : (uint64_t)bid) * a.cpb + m.ci];
+      m.desc = m.nul ? 0ull : d;
+    }
+    if constexpr (OUT == OUT_ENC) {
+      // the store's chunk permutation: from here on the metadata is that of
+      // the chunk's output position
+      if (a.nswap) {
+        uint64_t lo = loc;
+        for (uint32_t i = 0; i < a.nswap; ++i) {
+          const uint32_t p0 = a.swp[2 * i], p1 = a.swp[2 * i + 1];
+          const uint64_t d = ((lo >> p0) ^ (lo >> p1)) & 1ull;
+          lo ^= (d << p0) | (d << p1);
+        }
+        m.bid = bmap[(slot << a.khi) + (lo >> a.B)];
+        m.ci = (uint32_t)((lo & ((1ull << a.B) - 1ull)) >> kChunkAmpsLog2);
+        m.nul = null_in[m.bid];
+      }
+    }
+    return m;
+  };
+  auto meta_of = [&](const uint64_t wt) -> TileMeta {
+    if constexpr (IN == IN_DEC || OUT == OUT_ENC) return meta_rest(wt, bmap[meta_bpos(wt)]);
+    else return TileMeta{0ull, 0u, 0u, 1u};
+  };
+  // pre(): for RAW->RAW on 1024-amplitude tiles, the next tile's row loads
+  // (called after this tile's rows are in registers); for the persistent
+  // DEC->ENC loop, the next tile's metadata loads (called after the ops)
+  auto tile_body = [&](const uint64_t wt, const TileMeta m, auto&& pre) {
   const uint32_t L = a.L;
   const uint64_t slot = wt >> (L - TB);
   const uint64_t tau = wt & ((1ull << (L - TB)) - 1ull);
@@ -1208,38 +1260,8 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   const uint32_t g = (uint32_t)lane & (LPC - 1u);  // part of the chunk (chunk layout)
   const uint32_t d0 = 2u * KV * g;                 // first double of this lane's part
 
-  // chunk metadata (chunk layout): the LPC lanes of chunk c all describe it;
-  // its first amplitude is tile position 32c
-  uint64_t mdesc = 0;
-  uint32_t mbid = 0, mci = 0, mnull = 1;
-  if constexpr (IN == IN_DEC || OUT == OUT_ENC) {
-    const uint64_t loc = base | tm.off(32u * ((uint32_t)lane / LPC));
-    const uint64_t bpos = (slot << a.khi) + (loc >> a.B);
-    mbid = bmap[bpos];
-    mci = (uint32_t)((loc & ((1ull << a.B) - 1ull)) >> kChunkAmpsLog2);
-    mnull = null_in[mbid];
-    if constexpr (IN == IN_DEC) {
-      // the descriptor table spans every block, so the descriptor load need
-      // not wait for the null flag (a null block's stale entry is discarded)
-      const uint64_t d = desc_in[(dy.desc_staged ? bpos : (uint64_t)mbid) * a.cpb + mci];
-      mdesc = mnull ? 0ull : d;
-    }
-    if constexpr (OUT == OUT_ENC) {
-      // the store's chunk permutation: from here on the metadata is that of
-      // the chunk's output position
-      if (a.nswap) {
-        uint64_t lo = loc;
-        for (uint32_t i = 0; i < a.nswap; ++i) {
-          const uint32_t p0 = a.swp[2 * i], p1 = a.swp[2 * i + 1];
-          const uint64_t d = ((lo >> p0) ^ (lo >> p1)) & 1ull;
-          lo ^= (d << p0) | (d << p1);
-        }
-        mbid = bmap[(slot << a.khi) + (lo >> a.B)];
-        mci = (uint32_t)((lo & ((1ull << a.B) - 1ull)) >> kChunkAmpsLog2);
-        mnull = null_in[mbid];
-      }
-    }
-  }
+  const uint64_t mdesc = m.desc;
+  const uint32_t mbid = m.bid, mci = m.ci, mnull = m.nul;
 
   // ---- load
   if constexpr (kPipeRaw) {
@@ -1321,6 +1343,7 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
     if (op.kind == OP_DENSE) wave_dense(op, amp, cx, coefs);
     else wave_elem(op, amp, cx, terms, diags, coefs);
   }
+  if constexpr (kPersistent) pre();
 
   // ---- store
   if constexpr (OUT == OUT_RAW) {
@@ -1404,16 +1427,29 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
     const uint64_t stride = (uint64_t)gridDim.x * (kThreads / 64);
     if (wt0 < ntiles) stage_rows(wt0);
     for (uint64_t wt = wt0; wt < ntiles; wt += stride)
-      tile_body(wt, [&] {
+      tile_body(wt, meta_of(wt), [&] {
         if (wt + stride < ntiles) stage_rows(wt + stride);
       });
   } else if constexpr (kPersistent) {
     const uint64_t stride = (uint64_t)gridDim.x * (kThreads / 64);
-    for (uint64_t wt = wt0; wt < ntiles; wt += stride) tile_body(wt, [] {});
+    if (wt0 < ntiles) {
+      TileMeta m = meta_of(wt0);
+      for (uint64_t wt = wt0; wt < ntiles; wt += stride) {
+        const uint64_t wn = wt + stride;
+        const bool more = wn < ntiles;  // uniform over the wave
+        uint32_t nbid = 0;
+        if (more) nbid = bmap[meta_bpos(wn)];
+        TileMeta mn{0ull, 0u, 0u, 1u};
+        tile_body(wt, m, [&] {
+          if (more) mn = meta_rest(wn, nbid);
+        });
+        m = mn;
+      }
+    }
     // the sub-arena's demand (all tiles counted, also past an overflow)
     if (own && lane == 0) dy.cursors[(uint64_t)gwave * 16] = wcur;
   } else {
-    if (wt0 < ntiles) tile_body(wt0, [] {});
+    if (wt0 < ntiles) tile_body(wt0, meta_of(wt0), [] {});
   }
 }
 
