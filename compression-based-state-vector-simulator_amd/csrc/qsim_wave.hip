@@ -1155,10 +1155,12 @@ __global__ __launch_bounds__(kThreads, KV == 16 ? 2 : 3) void wave_kernel(const 
   QG uint64_t* arena_out = (QG uint64_t*)(dy.arena_out);
   QG uint8_t* nonzero = (QG uint8_t*)(dy.nonzero);
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  // The fused DEC->ENC form runs a persistent grid (wave w of group b takes
-  // tiles tile0 + 4 (b + k gridDim) + w); the other forms take one tile per
-  // wave.
-  constexpr bool kPersistent = IN == IN_DEC && OUT == OUT_ENC;
+  // The forms that decode run a persistent grid (wave w of group b takes
+  // tiles tile0 + 4 (b + k gridDim) + w) and prefetch the next tile's chunk
+  // metadata.  RAW->ENC takes one tile per wave: its persistent form spills
+  // 16 B per lane and ran 4% slower (QFT-32's 2^32-amplitude last pass,
+  // 22.5 ms against 21.7 ms).
+  constexpr bool kPersistent = IN == IN_DEC;
   // RAW->RAW passes on 1024-amplitude tiles also run persistently and
   // pipeline their input: the rows of a wave's next tile are copied into the
   // wave's LDS buffer (unused by this form) by global-to-LDS loads while the
@@ -1956,14 +1958,15 @@ hipError_t launch_wave_adjoint(int tile_bits, const PassArgs* d_args, double2* p
 
 uint64_t wave_persistent_waves(uint64_t ntiles) { return persistent_waves(ntiles); }
 
-// DEC->ENC, and RAW->RAW on 1024-amplitude tiles: one launch, a persistent
-// grid of at most kWaveGroups work-groups (each wave walks its tiles with a
-// stride of the grid).  Other forms: one tile per wave, launches split at
-// kMaxLaunchBlocks work-groups.
+// Forms that decode, and RAW->RAW on 1024-amplitude tiles: one launch, a
+// persistent grid of at most persistent_groups() work-groups (each wave walks
+// its tiles with a stride of the grid).  RAW->ENC, and RAW->RAW on
+// 512-amplitude tiles: one tile per wave, launches split at kMaxLaunchBlocks
+// work-groups.
 template <int IN, int OUT, int KV, bool PT = false>
 static hipError_t launch_wave_mode(const PassArgs* d_args, const PassDyn& dy, uint64_t ntiles, hipStream_t st) {
   constexpr uint64_t per_block = kThreads / 64;
-  if constexpr ((IN == IN_DEC && OUT == OUT_ENC) || (IN == IN_RAW && OUT == OUT_RAW && KV == 16)) {
+  if constexpr (IN == IN_DEC || (IN == IN_RAW && OUT == OUT_RAW && KV == 16)) {
     static_assert(per_block == 4, "persistent_waves assumes four waves per work-group");
     const uint64_t groups = persistent_waves(ntiles) / per_block;
     if (groups == 0) return hipSuccess;
