@@ -1061,6 +1061,21 @@ __device__ __forceinline__ double walsh_fold(const double (&t)[KV], uint32_t zr)
   }
   return f[0];
 }
+// In place: t[z] <- sum_v (-1)^popc(v & z) t[v] (the Walsh-Hadamard
+// transform over the row bits; entry z is walsh_fold(t, z))
+template <int KV>
+__device__ __forceinline__ void walsh_transform(double (&t)[KV]) {
+#pragma unroll
+  for (int h = 1; h < KV; h <<= 1) {
+#pragma unroll
+    for (int v = 0; v < KV; ++v) {
+      if (v & h) continue;
+      const double x = t[v], y = t[v + h];
+      t[v] = x + y;
+      t[v + h] = x - y;
+    }
+  }
+}
 
 // ---------------------------------------------- points of lambda-only passes
 // Rows of one gradient point Re<lambda| i P |psi> on a tile (lambda in lam,
@@ -1747,6 +1762,8 @@ __global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kerne
                                                                    uint64_t ntiles) {
   constexpr int RB = KV == 16 ? 4 : 3;
   __shared__ __align__(16) double2 pbuf[2][64 * KV];
+  // the lambda wave's Walsh transform of a run's Z-only products, row z at 64 z
+  __shared__ __align__(16) double wh[64 * KV];
   const QC PassArgs& a = *(const QC PassArgs*)(pa);
   const QC Op* ops = (const QC Op*)(a.ops);
   const QC PhaseTerm* terms = (const QC PhaseTerm*)(a.terms);
@@ -1804,15 +1821,33 @@ __global__ __launch_bounds__(128, KV == 16 ? 2 : 3) void wave_pair_adjoint_kerne
     // first factor is a Walsh fold of the row products, the second uniform,
     // the third per lane.  i^kr picks Re(conj(l) x) or Im(conj(l) x) and an
     // overall sign.
+    bool hat = false;  // wh holds this run's transform
     for (; o < a.op_end && ops[o].kind == OP_GRAD; ++o) {
       const QC Op& pt = ops[o];
       const uint32_t fx = (uint32_t)pt.gfx, zt = (uint32_t)pt.gzt;
       const uint32_t kr = (1u + pt.nY) & 3u;  // i * i^nY = i^kr
       const uint32_t r = (fx >> 6) & (uint32_t)(KV - 1), lf = fx & 63u;
       const uint32_t zr = (zt >> 6) & (uint32_t)(KV - 1), zl = zt & 63u;
-      double t[KV];
-      pair_point_products_r<KV>(xb + ((uint32_t)lane ^ lf), amp, r, (kr & 1u) != 0u, t);
-      const double w = walsh_fold<KV>(t, zr);
+      double w;
+      if (fx == 0u && (kr & 1u)) {
+        // a Z-only point (no flips, so nY = 0 and kr = 1): its row fold is
+        // entry zr of the Walsh transform of the run's Im(conj(l) x)
+        // products, formed at the run's first such point; each lane reads
+        // back only what it wrote
+        if (!hat) {
+          double t[KV];
+          pair_point_products<KV, 0>(xb + lane, amp, true, t);
+          walsh_transform<KV>(t);
+#pragma unroll
+          for (int v = 0; v < KV; ++v) wh[64 * v + lane] = t[v];
+          hat = true;
+        }
+        w = wh[64 * zr + (uint32_t)lane];
+      } else {
+        double t[KV];
+        pair_point_products_r<KV>(xb + ((uint32_t)lane ^ lf), amp, r, (kr & 1u) != 0u, t);
+        w = walsh_fold<KV>(t, zr);
+      }
       const bool lneg = __popc(((uint32_t)lane ^ lf) & zl) & 1;
       const bool flip_all =
           (__popcll(base & pt.gzb) & 1) ^ (kr == 1u || kr == 2u) ^ (__popc(r & zr) & 1);
