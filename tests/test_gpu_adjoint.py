@@ -208,6 +208,24 @@ def test_paired_points_all_placements(rng, monkeypatch):
     assert np.max(np.abs(g - g0)) < 1e-10
 
 
+@pytest.mark.parametrize("n", [12, 16])
+def test_shared_parameter_z_points_vs_oracle(n, rng, monkeypatch):
+    """QAOA MaxCut with one parameter per layer angle on the paired kernel:
+    a cost layer's RZZ points form a run of Z-only points of one parameter,
+    which the kernel evaluates from one Walsh transform of the run's
+    products; the mixer's RX points take the flip path.  Gradients within
+    1e-10 of the oracle's backprop."""
+    monkeypatch.delenv("QOMPSIM_ADJ_ENGINE", raising=False)
+    p = 3
+    edges = W.random_3regular(n, rng)
+    c = W.qaoa_maxcut_shared(n, edges, rng.uniform(0, np.pi, p), rng.uniform(0, np.pi, p))
+    obs = W.maxcut_observable(n, edges)
+    e, g = adjoint_gradient(c, obs)
+    e0, g0 = O.backprop(c, obs)
+    assert abs(e - e0) < 1e-11
+    assert np.max(np.abs(g[: 2 * p] - g0[: 2 * p])) < 1e-10
+
+
 @pytest.mark.parametrize("n", [10, 12, 15])
 def test_lambda_only_sweep_matches_oracle(n, rng, monkeypatch, capfd):
     """The lambda-only reverse sweep (QOMPSIM_ADJ_ENGINE=lam: psi kept at
