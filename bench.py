@@ -270,9 +270,13 @@ def adjoint_bench(n=28, layers=8, reps=3):
     t_fwd = timed(fwd, reps)
     t_eval = timed(fwd_energy, reps)
     out = {}
+    # one working state for every call, as the forward reuses s: the library
+    # keeps its lambda and psi buffers with the state, so the timed calls do
+    # not allocate (a fresh state per call allocates about 3 x 4 GiB)
+    reg = QuantumState(n)
 
     def adj():
-        out["eg"] = adjoint_gradient(c, obs)
+        out["eg"] = adjoint_gradient(c, obs, register=reg)
 
     t_adj = timed(adj, reps)
     energy, grads = out["eg"]
