@@ -3202,7 +3202,12 @@ struct qsim_adj {
   uint32_t n = 0;
   uint64_t seg = 0;
   std::vector<qsim::HGate> gs;
+  // checkpoint stores, kept across forward calls while the qubit count,
+  // block size and codec settings stay the same (a load resets a store
+  // completely), so a repeated forward allocates nothing
   std::vector<qsim_cs*> ckpt;
+  uint32_t ck_B = 0;
+  qsim_cs_config ck_codec{};
   bool ready = false;
   void clear() {
     for (auto* c : ckpt) qsim_cs_destroy(c);
@@ -3211,6 +3216,12 @@ struct qsim_adj {
   }
   ~qsim_adj() { clear(); }
 };
+
+static bool same_codec(const qsim_cs_config& x, const qsim_cs_config& y) {
+  return x.eps_rel == y.eps_rel && x.eps_abs == y.eps_abs && x.guard_bits == y.guard_bits &&
+         x.zero_mode == y.zero_mode && x.zero_threshold == y.zero_threshold && x.scratch_bytes == y.scratch_bytes &&
+         x.tile_bits == y.tile_bits && x.store_mode == y.store_mode && x.hbm_budget == y.hbm_budget;
+}
 
 int qsim_adj_create(qsim_adj** out) {
   API_BEGIN
@@ -3231,20 +3242,35 @@ int qsim_adj_forward(qsim_adj* adj, qsim_sv* sv, const qsim_gate* gates, uint64_
   require(adj != nullptr && sv != nullptr && codec != nullptr, "null argument");
   require(segment_gates >= 1, "segment_gates must be at least 1");
   require(sv->n >= 5, "checkpointed adjoint needs at least 5 qubits");
-  adj->clear();
-  adj->gs = parse_gates(gates, n_gates, pool, pool_len);
-  check_qubits(adj->gs, sv->n);
+  std::vector<HGate> gs = parse_gates(gates, n_gates, pool, pool_len);
+  check_qubits(gs, sv->n);
+  const uint32_t B = std::min<uint32_t>(std::max<uint32_t>(block_size, 5), sv->n);
+  if (adj->n != sv->n || adj->ck_B != B || !same_codec(adj->ck_codec, *codec)) adj->clear();
+  adj->ready = false;
+  adj->gs = std::move(gs);
   adj->n = sv->n;
   adj->seg = segment_gates;
-  const uint32_t B = std::min<uint32_t>(std::max<uint32_t>(block_size, 5), sv->n);
+  adj->ck_B = B;
+  adj->ck_codec = *codec;
   const uint64_t G = adj->gs.size();
-  for (uint64_t b = 0; b < G || b == 0; b += segment_gates) {
-    qsim_cs* c = nullptr;
-    if (qsim_cs_create(sv->n, B, codec, &c) != 0) throw std::runtime_error(g_err);
-    adj->ckpt.push_back(c);
-    cs_load_device(c, sv->ptr());
+  size_t k = 0;
+  for (uint64_t b = 0; b < G || b == 0; b += segment_gates, ++k) {
+    if (k == adj->ckpt.size()) {
+      qsim_cs* c = nullptr;
+      if (qsim_cs_create(sv->n, B, codec, &c) != 0) throw std::runtime_error(g_err);
+      adj->ckpt.push_back(c);
+    }
+    cs_load_device(adj->ckpt[k], sv->ptr());
     apply_gates_raw(sv->ptr(), sv->n, adj->gs, b, std::min<uint64_t>(G, b + segment_gates));
-    if (G == 0) break;
+    if (G == 0) {
+      ++k;
+      break;
+    }
+  }
+  // stores beyond this circuit's checkpoints
+  while (adj->ckpt.size() > k) {
+    qsim_cs_destroy(adj->ckpt.back());
+    adj->ckpt.pop_back();
   }
   adj->ready = true;
   API_END

@@ -270,7 +270,10 @@ uint32_t qsim_cs_block_size(const qsim_cs* cs);
  * forms lambda = 2 H psi from `sv` (which must still hold the forward
  * result), and for each segment from the last restores its checkpoint,
  * recomputes the segment and sweeps it backwards with lambda: energy and
- * gradients as qsim_sv_backprop, with drift confined to one segment. */
+ * gradients as qsim_sv_backprop, with drift confined to one segment.
+ * A handle keeps its checkpoint stores between qsim_adj_forward calls with
+ * the same qubit count, block size and codec settings, so a repeated forward
+ * allocates nothing. */
 typedef struct qsim_adj qsim_adj;
 int qsim_adj_create(qsim_adj** out);
 int qsim_adj_destroy(qsim_adj* adj);

@@ -296,6 +296,30 @@ def test_checkpointed_adjoint_matches_oracle(seg, rng):
     assert e2 == e and np.array_equal(g2, g)
 
 
+def test_checkpointed_forward_reuses_stores(rng):
+    """Repeated forwards on one handle reuse its checkpoint stores (same qubit
+    count, block size and codec; a load resets a store), with new parameters,
+    more segments and then fewer: the oracle's gradients after each forward,
+    and as many checkpoints as segments."""
+    from qompsim.adjoint import CheckpointedAdjoint
+
+    n = 12
+    c = W.hea(n, layers=3, seed=6)
+    obs = W.tfim_observable(n)
+    adj = CheckpointedAdjoint(c, 40, block_size=8)
+    for seg in (40, 40, 9, 83):
+        adj.segment_gates = seg
+        th = c.get_parameters()
+        for i in range(len(th)):
+            c.set_parameter(i, th[i] + rng.uniform(-0.3, 0.3))
+        adj.forward_with_checkpoints()
+        e, g = adj.adjoint(obs)
+        e0, g0 = O.backprop(c, obs)
+        assert abs(e - e0) < 1e-11
+        assert np.max(np.abs(g - g0)) < 1e-10
+        assert adj.checkpoint_info()[0] == -(-len(c.gate_list) // seg)
+
+
 def test_checkpointed_adjoint_lossy_codec_bound(rng):
     """Checkpoints with an absolute bound of 1e-7 per double: each segment
     restarts from a state within that bound, so the gradient error stays of
