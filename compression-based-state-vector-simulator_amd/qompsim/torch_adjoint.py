@@ -22,16 +22,16 @@ def _set_theta(theta, circuit):
 
 class AdjointExpectation(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, theta, circuit, obs):
+    def forward(ctx, theta, circuit, obs, register=None):
         _set_theta(theta, circuit)
-        energy, grads = adjoint_gradient(circuit, obs)
+        energy, grads = adjoint_gradient(circuit, obs, register=register)
         ctx.save_for_backward(torch.as_tensor(np.asarray(grads), dtype=theta.dtype, device=theta.device))
         return torch.as_tensor(energy, dtype=theta.dtype, device=theta.device)
 
     @staticmethod
     def backward(ctx, grad_output):
         (grads,) = ctx.saved_tensors
-        return grad_output * grads, None, None
+        return grad_output * grads, None, None, None
 
 
 class CheckpointedExpectation(torch.autograd.Function):
@@ -54,9 +54,13 @@ class CheckpointedExpectation(torch.autograd.Function):
         return grad_output * g, None, None, None, None, None
 
 
-def expectation(theta, circuit, obs, segment_gates=None, block_size=20, codec=None):
+def expectation(theta, circuit, obs, segment_gates=None, block_size=20, codec=None, register=None):
     """<H> as a differentiable function of theta; with `segment_gates` the
-    backward pass uses compressed forward checkpoints every that many gates."""
+    backward pass uses compressed forward checkpoints every that many gates.
+    Without checkpoints, `register` (a QuantumState of the circuit's qubit
+    count) is the working state of every call and holds U|0...0> afterwards;
+    the library keeps its lambda and psi buffers with it, so repeated calls
+    allocate nothing.  Without it each call allocates a fresh state."""
     if segment_gates is None:
-        return AdjointExpectation.apply(theta, circuit, obs)
+        return AdjointExpectation.apply(theta, circuit, obs, register)
     return CheckpointedExpectation.apply(theta, circuit, obs, int(segment_gates), int(block_size), codec)

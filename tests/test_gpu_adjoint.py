@@ -143,6 +143,28 @@ def test_torch_autograd_function():
     assert np.max(np.abs(th.grad.numpy() - 3.0 * g0)) < 1e-10
 
 
+def test_torch_autograd_function_with_register():
+    """Two steps through one reused working state (`register`): each equals
+    the oracle at its parameters, and the register holds U|0...0> after."""
+    import torch
+
+    from qompsim.torch_adjoint import expectation
+
+    n = 10
+    c = W.hea(n, layers=2, seed=7)
+    obs = W.tfim_observable(n)
+    reg = QuantumState(n)
+    for shift in (0.0, 0.25):
+        th = torch.tensor(np.asarray(c.get_parameters()) + shift, dtype=torch.float64, requires_grad=True)
+        E = expectation(th, c, obs, register=reg)
+        E.backward()
+        e0, g0 = O.backprop(c, obs)
+        assert abs(E.item() - e0) < 1e-12
+        assert np.max(np.abs(th.grad.numpy() - g0)) < 1e-10
+    ref = O.apply_circuit(O.zero_state(n), c.gate_list)
+    assert np.max(np.abs(reg.get_vector() - ref)) < 1e-12
+
+
 @pytest.mark.parametrize("n", [10, 13, 16])
 def test_adjoint_engines_agree(n, rng, monkeypatch):
     """Paired-wave, single-wave (512-amplitude tiles) and LDS-tile reverse
